@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the narrow-band level-set path on one MI355X.
+
+Workload (one bench step = one full rollout, inputs resident in HBM):
+Zalesak slotted sphere (r = 0.15 at (0.5, 0.75, 0.5), slot 0.05 wide, 0.125
+deep, cut by csg_difference), voxel size 1/512, band +-3 voxels, f32; single
+vortex velocity u = (sin^2(pi x) sin(2 pi y), -sin^2(pi y) sin(2 pi x), 0)
+* cos(pi t / 8); 100 RK3 steps at CFL 0.5 (dt = 0.5 vs, max |u| = 1), band
+rebuild every 5 steps, marching cubes every 10 steps.
+
+Metric: active-voxel updates per second (sum over steps of the active voxels
+advected, divided by wall time of the rollouts), plus the advect kernel's
+achieved HBM bandwidth against the 8 TB/s roofline, measured with HIP events
+on the library's stream.  The CPU baseline is the C++ restatement of the
+reference's CPU path (oracle/), timed single-threaded on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK_GBPS = 8000.0
+
+VS = 1.0 / 512
+HW = 3.0
+PERIOD = 8.0
+STEPS = 100
+REBUILD_EVERY = 5
+MESH_EVERY = 10
+SPHERE = (0.5, 0.75, 0.5, 0.15)
+SLOT = (0.475, 0.55, 0.3, 0.525, 0.725, 0.7)
+
+
+def load_package():
+    name = "volrus_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "sharc.volrus_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per advect launch from the committed rocprofv3 counter summary, if any."""
+    p = os.path.join(ROOT, "profiles", "advect_vortex_slotted512_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(seconds_budget):
+    """Oracle (literal C++ restatement of the reference CPU path) on a bounded
+    sample of the same workload: the first steps of the rollout, single thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    g = orc.scene([(0, SPHERE), (1, SLOT)], [0, 2], VS, HW)
+    dt = 0.5 * VS
+    total_s, updates, steps = 0.0, 0, 0
+    while total_s < seconds_budget and steps < STEPS:
+        a = g.active_count()
+        tf = orc.stage_factors(PERIOD, steps * dt, dt, 3)
+        do_rb = (steps + 1) % REBUILD_EVERY == 0
+        do_mc = (steps + 1) % MESH_EVERY == 0
+        total_s += orc.time_step(g, 4, (), dt, 3, tf, do_rb, HW, do_mc, 0.0)
+        updates += a
+        steps += 1
+    return {
+        "value": updates / total_s,
+        "unit": "active-voxel updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {steps} of {STEPS} rollout steps (RK3 advect, rebuild every {REBUILD_EVERY} in the "
+                  f"bounded-radius form proven equal to the brute force, MC every {MESH_EVERY}) on the same "
+                  f"slotted sphere, {total_s:.1f} s of single-thread CPU time",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    vr = load_package()
+    ctx = vr.Context(0)
+    prims = [(vr.PRIM_SPHERE, 0, SPHERE), (vr.PRIM_BOX, vr.CSG_DIFFERENCE, SLOT)]
+    base = ctx.from_primitives(prims, VS, HW)
+    leaves0, active0 = base.counts()
+    log(f"slotted sphere at 1/512: {leaves0} leaves, {active0} active voxels")
+    vel = ctx.velocity_analytic(vr.VEL_VORTEX, (), PERIOD)
+    dt = 0.5 * VS
+    mesh = vr.Mesh(ctx)
+    grids = [base.clone() for _ in range(args.warmup + args.steps + 1)]
+    ctx.synchronize()
+
+    def rollout(g):
+        return g.rollout(vel, 0.0, dt, vr.RK3, STEPS, REBUILD_EVERY, HW, MESH_EVERY, 0.0, mesh)
+
+    for i in range(args.warmup):
+        rollout(grids[i])
+    ctx.synchronize()
+    log(f"warmup done ({args.warmup})")
+    updates = 0
+    stats = None
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats = rollout(grids[args.warmup + i])
+        updates += stats.voxel_updates
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    log(f"timed {args.steps} rollouts in {wall:.4f} s; last: {stats.as_dict()}")
+
+    # Per-kernel device time with HIP events on the library stream (one more rollout).
+    ctx.set_timing(True)
+    rollout(grids[-1])
+    kern = {}
+    for name, cls in [("advect", vr.TIMING_ADVECT), ("crossings", vr.TIMING_CROSSINGS),
+                      ("rebuild", vr.TIMING_REBUILD), ("marching_cubes", vr.TIMING_MC)]:
+        ms, n, b = ctx.timing(cls)
+        if n:
+            kern[name] = {"launches": n, "avg_us": 1e3 * ms / n, "algorithmic_bytes_per_launch": b / n,
+                          "achieved_GBps": (b / n) / (ms / n * 1e-3) / 1e9}
+    ctx.set_timing(False)
+    adv = kern["advect"]
+    traffic = traffic_from_profiles()
+    roofline = {"bound": "hbm", "achieved": round(adv["achieved_GBps"], 1), "peak": PEAK_GBPS, "unit": "GB/s",
+                "frac": round(adv["achieved_GBps"] / PEAK_GBPS, 4), "traffic": traffic}
+    log(f"kernels: {json.dumps(kern)}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(args.cpu_seconds)
+        cpu["value"] = round(cpu["value"], 1)
+
+    value = updates / wall
+    out = {
+        "metric": "active-voxel updates/s (advect+MC)",
+        "value": round(value, 1),
+        "unit": "active-voxel updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (slotted sphere generated on the device)",
+        "config": {
+            "workload": "config 2: Zalesak slotted sphere r=0.15, vs=1/512, band 3, single vortex T=8, "
+                        "100 RK3 steps at CFL 0.5, rebuild every 5, marching cubes every 10; one step = one rollout",
+            "leaves": leaves0, "active_voxels": active0, "voxel_updates_per_rollout": stats.voxel_updates,
+            "final_triangles": stats.last_triangle_count, "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kern,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/* volrus.h — C ABI of the MI355X (gfx950) narrow-band level-set library.
+ *
+ * Plain C: opaque handles, pointers and sizes, int status codes.  Every call
+ * returns VR_OK (0) or an error code and never aborts; the text of the last
+ * error on a context is available from vr_last_error().
+ *
+ * Grids cross the boundary in the reference's NanoGrid byte layout
+ * (src/nano/layout.rs:13-52: 320-byte header, then 2128-byte leaves sorted
+ * by origin), so a Rust shim can pass NanoGrid::as_bytes() straight in and
+ * rebuild a Grid with NanoGrid::from_bytes(..).to_grid() on the way out.
+ *
+ * Threading: one context owns one HIP stream; calls on a context must be
+ * serialised by the caller, different contexts may be used concurrently.
+ * Kernels are asynchronous; calls that return sizes or copy to host memory
+ * synchronise the context's stream.
+ */
+#ifndef VOLRUS_H
+#define VOLRUS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VR_OK = 0,
+  VR_ERR_INVALID_ARGUMENT = 1,
+  VR_ERR_INVALID_NANO = 2, /* NanoGrid::from_bytes would return Err */
+  VR_ERR_OUT_OF_MEMORY = 3,
+  VR_ERR_HIP = 4,
+  VR_ERR_UNSUPPORTED = 5,
+  VR_ERR_CAPACITY = 6 /* caller buffer too small; required size reported */
+};
+
+/* Integration schemes for vr_advect. */
+enum { VR_SCHEME_RK1 = 1, VR_SCHEME_RK3 = 3 };
+
+/* Analytic velocity kinds for vr_velocity_analytic. */
+enum {
+  VR_VEL_UNIFORM = 2, /* u = (p0, p1, p2) */
+  VR_VEL_RIGID = 3,   /* u = w x (x - c), w = (p0, p1, p2), c = (p3, p4, p5) */
+  VR_VEL_VORTEX = 4   /* u = (sin^2(pi x) sin(2 pi y), -sin^2(pi y) sin(2 pi x), 0) */
+};
+
+typedef struct vr_ctx vr_ctx;
+typedef struct vr_grid vr_grid;
+typedef struct vr_velocity vr_velocity;
+typedef struct vr_mesh vr_mesh;
+
+/* ---- context ------------------------------------------------------------ */
+/* No counterpart in the reference (single-threaded CPU code). */
+int vr_ctx_create(int device, vr_ctx** out);
+int vr_ctx_destroy(vr_ctx* ctx);
+const char* vr_last_error(const vr_ctx* ctx);
+int vr_ctx_synchronize(vr_ctx* ctx);
+/* The context's hipStream_t, for callers that time or order work on it. */
+void* vr_ctx_stream(vr_ctx* ctx);
+
+/* ---- grids (NanoGrid boundary) ------------------------------------------ */
+/* Replaces NanoGrid::from_bytes (src/nano/layout.rs:202-251) plus the upload
+ * the reference never performs: validates magic, version, size and leaf
+ * order with the same rules, then copies the leaves to HBM and builds the
+ * root/internal/neighbour lookup arrays.  Also rejects leaf origins that are
+ * not multiples of 8 (NanoGrid::from_grid never writes them). */
+int vr_grid_upload_nano(vr_ctx* ctx, const uint8_t* bytes, size_t len, vr_grid** out);
+/* Replaces NanoGrid::from_grid + as_bytes (src/nano/layout.rs:89-170,197-199)
+ * for a device grid: writes exactly the bytes from_grid would produce for the
+ * grid's contents.  With out == NULL or cap too small, *len receives the size
+ * and VR_ERR_CAPACITY is returned (NULL/0 is the size query, returns VR_OK). */
+int vr_grid_download_nano(vr_ctx* ctx, const vr_grid* g, uint8_t* out, size_t cap, size_t* len);
+/* Grid::leaf_count / active_voxel_count (src/grid.rs:505-513). */
+int vr_grid_counts(vr_ctx* ctx, const vr_grid* g, uint64_t* leaf_count, uint64_t* active_count);
+int vr_grid_destroy(vr_ctx* ctx, vr_grid* g);
+/* Device-side deep copy of a grid's leaves (lookup arrays are shared). */
+int vr_grid_clone(vr_ctx* ctx, const vr_grid* g, vr_grid** out);
+
+/* ---- synthetic level sets (fixture generation on the device) ----------- */
+enum { VR_PRIM_SPHERE = 0, VR_PRIM_BOX = 1, VR_PRIM_NOISY_SPHERE = 2 };
+enum { VR_CSG_UNION = 0, VR_CSG_INTERSECTION = 1, VR_CSG_DIFFERENCE = 2 };
+/* sphere:       p = (cx, cy, cz, r)
+ * box:          p = (xmin, ymin, zmin, xmax, ymax, zmax), exact box distance
+ * noisy sphere: p = (cx, cy, cz, r, amplitude, base frequency, seed)
+ * `op` combines primitive i (i >= 1) with the result of 0..i-1. */
+typedef struct vr_primitive {
+  int32_t kind;
+  int32_t op;
+  double p[8];
+} vr_primitive;
+/* Builds ((p0 op1 p1) op2 p2) ... on the device, each primitive rasterised
+ * like make_level_set_sphere (src/tools/sphere.rs:17-59) and combined like
+ * csg_union / csg_intersection / csg_difference (src/tools/csg.rs:15-78). */
+int vr_grid_from_primitives(vr_ctx* ctx, const vr_primitive* prims, int count, double voxel_size, double half_width,
+                            vr_grid** out);
+
+/* ---- velocity ----------------------------------------------------------- */
+/* VelocityField { vx, vy, vz } (src/tools/advection.rs:10-24): three uploaded
+ * grids, each sampled with sample_trilinear.  The grids are borrowed and must
+ * outlive the velocity object. */
+int vr_velocity_grids3(vr_ctx* ctx, const vr_grid* vx, const vr_grid* vy, const vr_grid* vz, vr_velocity** out);
+/* Dense n^3 field per component (3 * n^3 floats, component-major, index
+ * (i * n + j) * n + k) at nodes (i h, j h, k h), background 0 outside; sampled
+ * like sample_trilinear on a grid of voxel size h.  No reference counterpart. */
+int vr_velocity_dense(vr_ctx* ctx, const float* uvw, int n, double h, vr_velocity** out);
+/* Analytic fields evaluated in double on the device.  No reference counterpart. */
+int vr_velocity_analytic(vr_ctx* ctx, int kind, const double params[6], vr_velocity** out);
+/* Multiply the field by cospi(t / period) at each stage time (period > 0),
+ * or by 1 (period <= 0, the default). */
+int vr_velocity_set_period(vr_velocity* v, double period);
+int vr_velocity_destroy(vr_ctx* ctx, vr_velocity* v);
+
+/* ---- the path ----------------------------------------------------------- */
+/* advect_level_set(&Grid<f32>, &VelocityField, dt) -> Grid<f32>
+ * (src/tools/advection.rs:32-56) for scheme VR_SCHEME_RK1 and a time-free
+ * field; VR_SCHEME_RK3 (SSP back-trace) and time factors are extensions.
+ * t is the time at the start of the step. */
+int vr_advect(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel, double dt, double t, int scheme,
+              vr_grid** out);
+/* find_zero_crossings(&Grid<f32>) -> Vec<Coord> (src/tools/level_set_track.rs:14-47).
+ * Writes *count coordinates (x, y, z) in leaf order into xyz (3 * cap ints). */
+int vr_zero_crossings(vr_ctx* ctx, const vr_grid* g, int32_t* xyz, size_t cap, size_t* count);
+/* trim_narrow_band(&mut Grid<f32>, half_width) (src/tools/level_set_track.rs:53-68). */
+int vr_trim_band(vr_ctx* ctx, vr_grid* g, float half_width);
+/* rebuild_narrow_band(&mut Grid<f32>, half_width) (src/tools/level_set_track.rs:80-114).
+ * Requires ceil(half_width) <= 8 (VR_ERR_UNSUPPORTED otherwise). */
+int vr_rebuild_band(vr_ctx* ctx, vr_grid* g, float half_width);
+/* volume_to_mesh(&Grid<f32>, iso) -> Mesh (src/tools/volume_to_mesh.rs:46-101). */
+int vr_marching_cubes(vr_ctx* ctx, const vr_grid* g, float iso, vr_mesh** out);
+/* Mesh::vertex_count / tri_count (src/tools/volume_to_mesh.rs:18-25). */
+int vr_mesh_counts(vr_ctx* ctx, const vr_mesh* m, uint64_t* vertex_count, uint64_t* tri_count);
+/* Copies Mesh::vertices ([f64; 3] each) and Mesh::triangles ([u32; 3] each). */
+int vr_mesh_copy(vr_ctx* ctx, const vr_mesh* m, double* xyz, size_t vcap, uint32_t* tri, size_t tcap);
+int vr_mesh_destroy(vr_ctx* ctx, vr_mesh* m);
+/* An empty mesh object for vr_rollout to fill (and reuse across calls). */
+int vr_mesh_create(vr_ctx* ctx, vr_mesh** out);
+
+/* ---- rollout ------------------------------------------------------------ */
+/* No counterpart in the reference (nothing there chains its tools).  Runs
+ * `steps` advection steps from time t0 (step s starts at t0 + s * dt), calls
+ * rebuild_narrow_band(half_width) after every `rebuild_every`-th step and
+ * volume_to_mesh(iso) after every `mesh_every`-th step (0 disables either),
+ * in place on g.  The last mesh is left in `mesh` (may be NULL). */
+typedef struct vr_rollout_stats {
+  uint64_t voxel_updates; /* sum over steps of the active voxels advected */
+  uint64_t advect_steps;
+  uint64_t rebuilds;
+  uint64_t meshes;
+  uint64_t last_vertex_count;
+  uint64_t last_triangle_count;
+  uint64_t final_leaf_count;
+  uint64_t final_active_count;
+} vr_rollout_stats;
+int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, double t0, double dt, int scheme, int steps,
+               int rebuild_every, float half_width, int mesh_every, float iso, vr_mesh* mesh,
+               vr_rollout_stats* stats);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Kernel classes for vr_ctx_timing. */
+enum { VR_TIMING_ADVECT = 0, VR_TIMING_CROSSINGS = 1, VR_TIMING_REBUILD = 2, VR_TIMING_COMPACT = 3, VR_TIMING_MC = 4 };
+/* Turns per-launch HIP-event timing on the context stream on or off and
+ * clears the totals. */
+int vr_ctx_set_timing(vr_ctx* ctx, int enable);
+/* Device time, launch count and algorithmic bytes (each input and output
+ * byte counted once) accumulated for one kernel class since timing was on. */
+int vr_ctx_timing(vr_ctx* ctx, int kernel_class, double* total_ms, uint64_t* launches, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOLRUS_H */

@@ -1,0 +1,330 @@
+"""ctypes binding of libvolrus_amd.so (include/volrus.h) for Python callers.
+
+The library runs the narrow-band level-set path on an MI355X (gfx950):
+NanoGrid upload/download, semi-Lagrangian advection (RK1 as in
+src/tools/advection.rs:32-56, plus RK3), zero crossings, band trim/rebuild
+(src/tools/level_set_track.rs:14-114) and marching cubes
+(src/tools/volume_to_mesh.rs:46-201).  There is no CPU fallback: loading
+fails loudly when the library is missing, and calls fail when no GPU is
+present.
+
+The directory name contains a dot, so import it by path::
+
+    spec = importlib.util.spec_from_file_location(
+        "volrus_amd", "<repo>/sharc.volrus_amd/__init__.py")
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvolrus_amd.so")
+
+VR_OK = 0
+VR_ERR_INVALID_ARGUMENT = 1
+VR_ERR_INVALID_NANO = 2
+VR_ERR_OUT_OF_MEMORY = 3
+VR_ERR_HIP = 4
+VR_ERR_UNSUPPORTED = 5
+VR_ERR_CAPACITY = 6
+
+RK1 = 1
+RK3 = 3
+VEL_UNIFORM = 2
+VEL_RIGID = 3
+VEL_VORTEX = 4
+
+PRIM_SPHERE = 0
+PRIM_BOX = 1
+PRIM_NOISY_SPHERE = 2
+CSG_UNION = 0
+CSG_INTERSECTION = 1
+CSG_DIFFERENCE = 2
+
+PRIM_DTYPE = [("kind", "<i4"), ("op", "<i4"), ("p", "<f8", 8)]
+
+NANO_HEADER = 320
+NANO_LEAF = 2128
+
+# Every symbol include/volrus.h declares, with its ctypes signature.
+_P = ctypes.c_void_p
+_SIGS = {
+    "vr_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "vr_ctx_destroy": (ctypes.c_int, [_P]),
+    "vr_last_error": (ctypes.c_char_p, [_P]),
+    "vr_ctx_synchronize": (ctypes.c_int, [_P]),
+    "vr_ctx_stream": (_P, [_P]),
+    "vr_grid_upload_nano": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "vr_grid_download_nano": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "vr_grid_counts": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "vr_grid_destroy": (ctypes.c_int, [_P, _P]),
+    "vr_grid_from_primitives": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                               ctypes.POINTER(_P)]),
+    "vr_velocity_grids3": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "vr_velocity_dense": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_P)]),
+    "vr_velocity_analytic": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.POINTER(_P)]),
+    "vr_velocity_set_period": (ctypes.c_int, [_P, ctypes.c_double]),
+    "vr_velocity_destroy": (ctypes.c_int, [_P, _P]),
+    "vr_advect": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_P)]),
+    "vr_zero_crossings": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "vr_trim_band": (ctypes.c_int, [_P, _P, ctypes.c_float]),
+    "vr_rebuild_band": (ctypes.c_int, [_P, _P, ctypes.c_float]),
+    "vr_marching_cubes": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.POINTER(_P)]),
+    "vr_mesh_counts": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "vr_mesh_copy": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+    "vr_mesh_destroy": (ctypes.c_int, [_P, _P]),
+    "vr_mesh_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "vr_grid_clone": (ctypes.c_int, [_P, _P, ctypes.POINTER(_P)]),
+    "vr_rollout": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, _P, _P]),
+    "vr_ctx_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "vr_ctx_timing": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
+}
+
+TIMING_ADVECT = 0
+TIMING_CROSSINGS = 1
+TIMING_REBUILD = 2
+TIMING_COMPACT = 3
+TIMING_MC = 4
+
+
+class RolloutStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "voxel_updates", "advect_steps", "rebuilds", "meshes", "last_vertex_count", "last_triangle_count",
+        "final_leaf_count", "final_active_count")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads the built library and binds every C ABI symbol (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built; run `python sharc.volrus_amd/build.py` (there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class VolrusError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"volrus error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """One device, one HIP stream (vr_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.vr_ctx_create(device, ctypes.byref(h))
+        if rc != VR_OK:
+            raise VolrusError(rc, "vr_ctx_create failed (is a GPU present?)")
+        self.h = h
+
+    def check(self, rc):
+        if rc != VR_OK:
+            raise VolrusError(rc, self.lib.vr_last_error(self.h).decode())
+
+    def synchronize(self):
+        self.check(self.lib.vr_ctx_synchronize(self.h))
+
+    def set_timing(self, on):
+        self.check(self.lib.vr_ctx_set_timing(self.h, 1 if on else 0))
+
+    def timing(self, kernel_class):
+        """(total device ms, launches, algorithmic bytes) since set_timing(True)."""
+        ms, n, b = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
+        self.check(self.lib.vr_ctx_timing(self.h, kernel_class, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)))
+        return ms.value, n.value, b.value
+
+    def stream(self):
+        return self.lib.vr_ctx_stream(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.vr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- grids
+    def upload(self, nano_bytes):
+        b = np.frombuffer(bytes(nano_bytes), dtype=np.uint8)
+        h = ctypes.c_void_p()
+        self.check(self.lib.vr_grid_upload_nano(self.h, _ptr(b), b.size, ctypes.byref(h)))
+        return Grid(self, h)
+
+    def from_primitives(self, prims, voxel_size, half_width):
+        """prims: list of (kind, op, params); op is ignored for the first."""
+        a = np.zeros(len(prims), dtype=PRIM_DTYPE)
+        for i, (kind, op, params) in enumerate(prims):
+            a[i]["kind"], a[i]["op"] = kind, op
+            a[i]["p"][: len(params)] = params
+        h = ctypes.c_void_p()
+        self.check(self.lib.vr_grid_from_primitives(self.h, _ptr(a), len(prims), voxel_size, half_width,
+                                                    ctypes.byref(h)))
+        return Grid(self, h)
+
+    def velocity_grids3(self, vx, vy, vz):
+        h = ctypes.c_void_p()
+        self.check(self.lib.vr_velocity_grids3(self.h, vx.h, vy.h, vz.h, ctypes.byref(h)))
+        return Velocity(self, h, keep=(vx, vy, vz))
+
+    def velocity_dense(self, uvw, n, h_spacing):
+        u = np.ascontiguousarray(uvw, dtype=np.float32).reshape(-1)
+        assert u.size == 3 * n ** 3
+        h = ctypes.c_void_p()
+        self.check(self.lib.vr_velocity_dense(self.h, _ptr(u), n, h_spacing, ctypes.byref(h)))
+        return Velocity(self, h)
+
+    def velocity_analytic(self, kind, params=(0, 0, 0, 0, 0, 0), period=0.0):
+        p = np.zeros(6, dtype=np.float64)
+        p[: len(params)] = params
+        h = ctypes.c_void_p()
+        self.check(self.lib.vr_velocity_analytic(self.h, kind, _ptr(p), ctypes.byref(h)))
+        v = Velocity(self, h)
+        if period:
+            self.check(self.lib.vr_velocity_set_period(h, period))
+        return v
+
+
+class Grid:
+    """A level set resident in HBM (vr_grid)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def download(self):
+        n = ctypes.c_size_t()
+        self.ctx.check(self.ctx.lib.vr_grid_download_nano(self.ctx.h, self.h, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value, dtype=np.uint8)
+        self.ctx.check(self.ctx.lib.vr_grid_download_nano(self.ctx.h, self.h, _ptr(out), out.size, ctypes.byref(n)))
+        return out.tobytes()
+
+    def clone(self):
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.vr_grid_clone(self.ctx.h, self.h, ctypes.byref(h)))
+        return Grid(self.ctx, h)
+
+    def counts(self):
+        lc, ac = ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.vr_grid_counts(self.ctx.h, self.h, ctypes.byref(lc), ctypes.byref(ac)))
+        return lc.value, ac.value
+
+    def advect(self, vel, dt, t=0.0, scheme=RK1):
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.vr_advect(self.ctx.h, self.h, vel.h, dt, t, scheme, ctypes.byref(h)))
+        return Grid(self.ctx, h)
+
+    def rollout(self, vel, t0, dt, scheme, steps, rebuild_every=0, half_width=3.0, mesh_every=0, iso=0.0,
+                mesh=None):
+        """In-place multi-step rollout; returns RolloutStats."""
+        st = RolloutStats()
+        self.ctx.check(self.ctx.lib.vr_rollout(self.ctx.h, self.h, vel.h, t0, dt, scheme, steps, rebuild_every,
+                                               half_width, mesh_every, iso, mesh.h if mesh else None,
+                                               ctypes.byref(st)))
+        return st
+
+    def zero_crossings(self):
+        n = ctypes.c_size_t()
+        self.ctx.check(self.ctx.lib.vr_zero_crossings(self.ctx.h, self.h, None, 0, ctypes.byref(n)))
+        out = np.empty((max(n.value, 1), 3), dtype=np.int32)
+        self.ctx.check(self.ctx.lib.vr_zero_crossings(self.ctx.h, self.h, _ptr(out), n.value, ctypes.byref(n)))
+        return out[: n.value]
+
+    def trim(self, half_width):
+        self.ctx.check(self.ctx.lib.vr_trim_band(self.ctx.h, self.h, half_width))
+
+    def rebuild(self, half_width):
+        self.ctx.check(self.ctx.lib.vr_rebuild_band(self.ctx.h, self.h, half_width))
+
+    def mesh(self, iso=0.0):
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.vr_marching_cubes(self.ctx.h, self.h, iso, ctypes.byref(h)))
+        try:
+            nv, nt = ctypes.c_uint64(), ctypes.c_uint64()
+            self.ctx.check(self.ctx.lib.vr_mesh_counts(self.ctx.h, h, ctypes.byref(nv), ctypes.byref(nt)))
+            v = np.empty((max(nv.value, 1), 3), dtype=np.float64)
+            t = np.empty((max(nt.value, 1), 3), dtype=np.uint32)
+            self.ctx.check(self.ctx.lib.vr_mesh_copy(self.ctx.h, h, _ptr(v), nv.value, _ptr(t), nt.value))
+            return v[: nv.value], t[: nt.value]
+        finally:
+            self.ctx.lib.vr_mesh_destroy(self.ctx.h, h)
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.vr_grid_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mesh:
+    """A reusable device mesh (vr_mesh) that vr_rollout fills."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.vr_mesh_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def arrays(self):
+        nv, nt = ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.vr_mesh_counts(self.ctx.h, self.h, ctypes.byref(nv), ctypes.byref(nt)))
+        v = np.empty((max(nv.value, 1), 3), dtype=np.float64)
+        t = np.empty((max(nt.value, 1), 3), dtype=np.uint32)
+        self.ctx.check(self.ctx.lib.vr_mesh_copy(self.ctx.h, self.h, _ptr(v), nv.value, _ptr(t), nt.value))
+        return v[: nv.value], t[: nt.value]
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.vr_mesh_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Velocity:
+    def __init__(self, ctx, h, keep=()):
+        self.ctx, self.h, self._keep = ctx, h, keep
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.vr_velocity_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,141 @@
+// Host-side objects behind the C ABI handles.
+#pragma once
+
+#include <cstdarg>
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/volrus.h"
+#include "vr_kernels.h"
+
+#define NANO_HEADER_BYTES 320
+
+// A growable device allocation reused across calls.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  hipError_t ensure(size_t n);
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Leaf-set lookup arrays shared by every grid with the same leaf origins
+// (an advected grid shares its input's).
+struct Topo {
+  int cap = 0;        // capacity of origins / nbr
+  int count = 0;      // host copy of the live leaf count
+  bool has_empty = false;  // some leaf has no active voxel
+  int* d_count = nullptr;
+  int4* d_origins = nullptr;
+  std::vector<int4> roots;  // sorted internal-node origins (host copy)
+  int4* d_roots = nullptr;
+  int* d_child = nullptr;
+  int* d_nbr = nullptr;
+  Topo() = default;
+  Topo(const Topo&) = delete;
+  Topo& operator=(const Topo&) = delete;
+  ~Topo();
+};
+
+enum ScratchSlot {
+  SCR_POS = 0, SCR_SUMS, SCR_SUMS2, SCR_TOTAL, SCR_KEEP, SCR_CROSS, SCR_ANY, SCR_WORDS, SCR_COORDS,
+  SCR_MC_KEEP, SCR_MC_POS, SCR_MC_TILES, SCR_MC_TLEAF, SCR_MC_TTILE, SCR_MC_META, SCR_MC_TV, SCR_MC_TT,
+  SCR_MC_VB, SCR_MC_TB, SCR_NEWCOUNT, SCR_TMP_LEAVES,
+  SCR_COUNT
+};
+
+// Kernel classes timed with HIP events when timing is on.
+enum TimingClass { TC_ADVECT = 0, TC_CROSSINGS, TC_REBUILD, TC_COMPACT, TC_MC, TC_COUNT };
+
+struct TimingRec {
+  hipEvent_t a, b;
+  int cls;
+  double bytes;
+};
+
+struct vr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  DevBuf scratch[SCR_COUNT];
+  bool timing = false;
+  std::vector<TimingRec> pending;
+  std::vector<hipEvent_t> event_pool;
+  double t_ms[TC_COUNT] = {0, 0, 0, 0, 0};
+  double t_bytes[TC_COUNT] = {0, 0, 0, 0, 0};
+  uint64_t t_n[TC_COUNT] = {0, 0, 0, 0, 0};
+  ~vr_ctx();
+};
+
+// Records an event pair around a kernel launch when ctx->timing is set.
+struct TimedLaunch {
+  vr_ctx* ctx;
+  int cls;
+  double bytes;
+  hipEvent_t a = nullptr;
+  TimedLaunch(vr_ctx* c, int k, double b);
+  ~TimedLaunch();
+};
+
+struct vr_grid {
+  std::shared_ptr<Topo> topo;
+  NanoLeaf* d_leaves = nullptr;
+  int cap = 0;
+  uint32_t grid_class = 0;
+  double voxel_size = 1.0;
+  float background = 0.0f;
+  double mat[4][4];
+  double inv[4][4];
+  vr_grid() = default;
+  vr_grid(const vr_grid&) = delete;
+  vr_grid& operator=(const vr_grid&) = delete;
+  ~vr_grid();
+};
+
+struct vr_velocity {
+  int kind = 0;
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  double period = 0.0;
+  const vr_grid* g[3] = {nullptr, nullptr, nullptr};
+  float* d_dense = nullptr;
+  int n = 0;
+  double h = 1.0;
+  vr_velocity() = default;
+  vr_velocity(const vr_velocity&) = delete;
+  vr_velocity& operator=(const vr_velocity&) = delete;
+  ~vr_velocity();
+};
+
+struct vr_mesh {
+  uint64_t nv = 0, nt = 0;
+  uint64_t cap_v = 0, cap_t = 0;
+  double* d_v = nullptr;
+  uint32_t* d_t = nullptr;
+  vr_mesh() = default;
+  vr_mesh(const vr_mesh&) = delete;
+  vr_mesh& operator=(const vr_mesh&) = delete;
+  ~vr_mesh();
+};
+
+struct MCScratch {};
+
+int vr_fail(vr_ctx* ctx, int code, const char* fmt, ...);
+int vr_hip_fail(vr_ctx* ctx, hipError_t e, const char* where);
+GridView make_view(const vr_grid* g);
+VelView make_vel_view(const vr_velocity* v);
+int build_topology(vr_ctx* ctx, vr_grid* g, std::shared_ptr<Topo> t);
+int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, bool has_empty_after);
+int advect_into(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel, double dt, double t, int scheme,
+                vr_grid* o);
+int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any);
+int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* s, vr_mesh* m);
+int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp);
+void copy_header_into(vr_grid* dst, const vr_grid* src);
+double vr_time_factor(double period, double t);
+void vr_stage_factors(double period, double t, double dt, int scheme, double tf[3]);

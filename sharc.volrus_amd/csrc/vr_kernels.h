@@ -1,0 +1,41 @@
+// Launch wrappers for the gfx950 kernels (host side of vr_kernels.hip).
+#pragma once
+
+#include "vr_device.h"
+
+hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, const int8_t* tri,
+                               const int8_t* corner, const int8_t* edge, const int8_t* cand);
+
+hipError_t vr_fill_int(int* p, int v, size_t n, hipStream_t s);
+hipError_t vr_build_child(const NanoLeaf* leaves, const int* count, int cap, const int4* roots, int root_count,
+                          int* child, int4* origins, hipStream_t s);
+hipError_t vr_build_nbr(const GridView& g, int cap, int* nbr, hipStream_t s);
+
+size_t vr_scan_scratch_ints(size_t n);
+hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, int* total, hipStream_t s);
+
+hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* out, int cap, double dt,
+                            const double tf[3], int scheme, hipStream_t s);
+hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, hipStream_t s);
+hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, NanoLeaf* out,
+                             int* keep, int R, float vs, float band, hipStream_t s);
+hipError_t vr_launch_trim(const GridView& g, int cap, NanoLeaf* out, int* keep, float threshold, hipStream_t s);
+hipError_t vr_launch_nonempty(const NanoLeaf* leaves, const int* count, int cap, int* keep, hipStream_t s);
+hipError_t vr_launch_compact(const NanoLeaf* src, const int* count, int cap, const int* keep, const int* pos,
+                             NanoLeaf* dst, hipStream_t s);
+hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s);
+hipError_t vr_launch_popcount_words(const uint64_t* w, const int* count, int cap, int* out, hipStream_t s);
+hipError_t vr_launch_crossing_coords(const int4* origins, const int* count, int cap, const uint64_t* cross,
+                                     const int* pos, int* xyz, hipStream_t s);
+
+hipError_t vr_launch_mc_keep(const GridView& g, int cap, int* keep, hipStream_t s);
+hipError_t vr_launch_mc_scatter(const GridView& g, int cap, const int* keep, const int* pos, const int* ntiles,
+                                int4* tiles, hipStream_t s);
+hipError_t vr_launch_mc_tile_nbr(const GridView& g, const int4* tiles, const int* ntiles, int tcap, int* tleaf,
+                                 int* ttile, hipStream_t s);
+hipError_t vr_launch_mc_count(const GridView& g, const int4* tiles, const int* ntiles, int tcap, const int* tleaf,
+                              const int* ttile, float iso, uint32_t* meta, int* tv, int* tt, hipStream_t s);
+hipError_t vr_launch_mc_emit(const GridView& g, const int4* tiles, const int* ntiles, int tcap, const int* tleaf,
+                             const int* ttile, float iso, double vs, const uint32_t* meta, const int* vbase,
+                             const int* tbase, double* verts, uint32_t* tris, hipStream_t s);
+hipError_t vr_launch_leaf_popcount(const NanoLeaf* leaves, const int* count, int cap, int* out, hipStream_t s);

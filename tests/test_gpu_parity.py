@@ -1,0 +1,267 @@
+"""HIP path vs the CPU oracle, through the C ABI, on the same inputs.
+
+Bit-exact for masks, leaf topology, values (NanoGrid bytes) and MC triangle
+indices; vertex positions within 1e-5 voxel (they come out bit-exact).
+"""
+import math
+import struct
+
+import numpy as np
+import pytest
+
+from util import (NANO_HEADER, active_voxels, assert_nano_equal, config1_dt, config1_sphere, header, leaf_array,
+                  mesh_equal, rigid_velocity_grids, slotted_sphere)
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+# ---------------------------------------------------------------- NanoGrid boundary
+def test_upload_download_round_trip(orc, ctx):
+    announce("nano round trip")
+    cases = [config1_sphere(orc), orc.new_grid(5.0, 1.0)]
+    g = orc.new_grid(-1.0, 1.0)
+    g.set(-5, -10, -3, 77.0)
+    g.set(-100, -200, -300, 88.0)
+    g.set(130, 130, 130, 4.0)
+    cases.append(g)
+    e = orc.new_grid(0.0, 1.0)
+    e.set(0, 0, 0, 1.0)
+    e.set(9, 0, 0, 2.0)
+    e.deactivate(9, 0, 0)  # leaf with no active voxel stays allocated
+    cases.append(e)
+    for og in cases:
+        b = og.to_nano()
+        got = ctx.upload(b).download()
+        assert_nano_equal(got, b, "round trip")
+
+
+def test_upload_rejects_invalid_nano(orc, ctx, vr):
+    announce("nano validation")
+    b = bytearray(config1_sphere(orc).to_nano())
+    bad = [bytes(b[:10]), b"XYZW" + bytes(b[4:]), bytes(b[:4]) + struct.pack("<I", 2) + bytes(b[8:]),
+           bytes(b[:-1])]
+    sw = bytearray(b)  # swap two leaves: not sorted
+    a0, a1 = NANO_HEADER, NANO_HEADER + 2128
+    sw[a0:a1], sw[a1:a1 + 2128] = b[a1:a1 + 2128], b[a0:a1]
+    bad.append(bytes(sw))
+    for x in bad:
+        with pytest.raises(vr.VolrusError) as ei:
+            ctx.upload(x)
+        assert ei.value.code == vr.VR_ERR_INVALID_NANO
+
+
+# ---------------------------------------------------------------- advection
+def test_advect_rk1_grids3_config1(orc, ctx):
+    """Config 1: reference VelocityField path (vx/vy/vz grids), one RK1 step."""
+    announce("advect rk1 grids3 config1")
+    sdf = config1_sphere(orc)
+    b = sdf.to_nano()
+    vxyz = rigid_velocity_grids(orc, b, 1.0 / 128)
+    want = orc.advect_grids3(sdf, *vxyz, config1_dt()).to_nano()
+    g = ctx.upload(b)
+    vg = [ctx.upload(v.to_nano()) for v in vxyz]
+    vel = ctx.velocity_grids3(*vg)
+    got = g.advect(vel, config1_dt(), 0.0, 1).download()
+    assert_nano_equal(got, want, "advect grids3")
+
+
+def test_uniform_velocity_moves_sphere_sdf(orc, ctx):
+    """advection.rs:152 through the GPU path (background-encoded field)."""
+    announce("advection.rs:152")
+    grid = orc.sphere(5.0, (0, 0, 0), 0.5, 3.0)
+    vs = []
+    for c in range(3):
+        vv = (1.0, 0.0, 0.0)[c]
+        gg = orc.new_grid(vv, 0.5)
+        gg.set(-30, -30, -30, vv)
+        gg.set(30, 30, 30, vv)
+        vs.append(gg)
+    want = orc.advect_grids3(grid, *vs, 1.0).to_nano()
+    vel = ctx.velocity_grids3(*[ctx.upload(v.to_nano()) for v in vs])
+    got = ctx.upload(grid.to_nano()).advect(vel, 1.0).download()
+    assert_nano_equal(got, want, "uniform")
+    adv = orc.from_nano(got)
+    assert abs(adv.sample(6.0, 0, 0)) < 1.0 and adv.sample(5.0, 0, 0) < 0.0 and adv.sample(7.0, 0, 0) > 0.0
+
+
+@pytest.mark.parametrize("kind,params,scheme,period", [
+    (3, (0, 0, 2 * math.pi, 0.5, 0.5, 0.5), 1, 0.0),
+    (3, (0, 0, 2 * math.pi, 0.5, 0.5, 0.5), 3, 0.0),
+    (2, (0.3, -0.7, 0.2), 3, 0.0),
+    (4, (), 1, 8.0),
+    (4, (), 3, 8.0),
+])
+def test_advect_analytic(orc, ctx, kind, params, scheme, period):
+    announce(f"advect analytic kind={kind} scheme={scheme}")
+    sdf = config1_sphere(orc)
+    dt, t = config1_dt(), 0.37
+    tf = orc.stage_factors(period, t, dt, scheme)
+    want = orc.advect_analytic(sdf, kind, params, dt, scheme, tf).to_nano()
+    vel = ctx.velocity_analytic(kind, params, period)
+    got = ctx.upload(sdf.to_nano()).advect(vel, dt, t, scheme).download()
+    assert_nano_equal(got, want, f"analytic {kind}/{scheme}")
+
+
+def test_advect_dense_velocity_large_dt(orc, ctx):
+    """Dense field, dt large enough that departure cells leave the leaf halo."""
+    announce("advect dense")
+    rng = np.random.default_rng(5)
+    n = 16
+    u = rng.normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32)
+    sdf = orc.sphere(0.3, (0.5, 0.5, 0.5), 1.0 / 64, 3.0)
+    for dt, scheme in [(0.5 / 64 / 1.5, 1), (6.0 / 64, 1), (3.0 / 64, 3)]:
+        want = orc.advect_dense(sdf, u, n, 1.0 / n, dt, scheme).to_nano()
+        vel = ctx.velocity_dense(u, n, 1.0 / n)
+        got = ctx.upload(sdf.to_nano()).advect(vel, dt, 0.0, scheme).download()
+        assert_nano_equal(got, want, f"dense dt={dt} scheme={scheme}")
+
+
+def test_advect_input_with_empty_leaf(orc, ctx):
+    announce("advect empty leaf")
+    g = orc.sphere(5.0, (0, 0, 0), 1.0, 3.0)
+    g.set(40, 40, 40, 1.0)
+    g.deactivate(40, 40, 40)
+    want = orc.advect_analytic(g, 2, (0.25, 0.0, -0.5), 1.0).to_nano()
+    got = ctx.upload(g.to_nano()).advect(ctx.velocity_analytic(2, (0.25, 0.0, -0.5)), 1.0).download()
+    assert_nano_equal(got, want, "empty leaf")
+
+
+# ---------------------------------------------------------------- band tools
+def crossing_set(a):
+    return set(map(tuple, np.asarray(a).tolist()))
+
+
+def test_zero_crossings_match(orc, ctx):
+    announce("zero crossings")
+    for og in [orc.sphere(5.0, (0, 0, 0), 1.0, 3.0), config1_sphere(orc), slotted_sphere(orc, 1.0 / 128)]:
+        want = crossing_set(orc.zero_crossings(og))
+        got = ctx.upload(og.to_nano()).zero_crossings()
+        assert len(got) == len(want) and crossing_set(got) == want
+    assert len(ctx.upload(orc.new_level_set(1.0, 3.0).to_nano()).zero_crossings()) == 0
+
+
+def test_trim_matches(orc, ctx):
+    announce("trim")
+    for hw in (2.0, 0.5, 10.0):
+        og = orc.sphere(5.0, (0, 0, 0), 1.0, 5.0)
+        g = ctx.upload(og.to_nano())
+        orc.trim(og, hw)
+        g.trim(hw)
+        assert_nano_equal(g.download(), og.to_nano(), f"trim {hw}")
+
+
+def test_rebuild_matches_brute_force(orc, ctx):
+    announce("rebuild brute force")
+    for og in [orc.sphere(5.0, (0, 0, 0), 1.0, 3.0), orc.sphere(4.0, (1.5, -2.0, 0.25), 0.5, 3.0)]:
+        for hw in (3.0, 1.5, 5.0):
+            src = og.to_nano()
+            ref = orc.from_nano(src)
+            orc.rebuild(ref, hw)
+            g = ctx.upload(src)
+            g.rebuild(hw)
+            assert_nano_equal(g.download(), ref.to_nano(), f"rebuild hw={hw}")
+
+
+def test_rebuild_after_advect_config1(orc, ctx):
+    announce("rebuild config1")
+    sdf = config1_sphere(orc)
+    dt = config1_dt()
+    params = (0, 0, 2 * math.pi, 0.5, 0.5, 0.5)
+    adv = orc.advect_analytic(sdf, 3, params, dt, 1)
+    ref = orc.from_nano(adv.to_nano())
+    orc.rebuild_bounded(ref, 3.0)
+    g = ctx.upload(sdf.to_nano()).advect(ctx.velocity_analytic(3, params), dt)
+    g.rebuild(3.0)
+    assert_nano_equal(g.download(), ref.to_nano(), "rebuild config1")
+
+
+def test_rebuild_noop_cases(orc, ctx):
+    announce("rebuild no-op")
+    e = orc.new_level_set(1.0, 3.0)
+    g = ctx.upload(e.to_nano())
+    g.rebuild(3.0)
+    assert g.counts() == (0, 0)
+    # all-positive band: no crossings -> grid unchanged, empty leaves kept
+    p = orc.new_grid(3.0, 1.0)
+    for i in range(10):
+        p.set(i, 0, 0, 1.0 + i)
+    p.set(20, 0, 0, 1.0)
+    p.deactivate(20, 0, 0)
+    b = p.to_nano()
+    g = ctx.upload(b)
+    g.rebuild(3.0)
+    assert_nano_equal(g.download(), b, "no crossings")
+
+
+def test_rebuild_rejects_wide_band(ctx, orc, vr):
+    g = ctx.upload(orc.sphere(5.0, (0, 0, 0), 1.0, 3.0).to_nano())
+    with pytest.raises(vr.VolrusError) as ei:
+        g.rebuild(8.5)
+    assert ei.value.code == vr.VR_ERR_UNSUPPORTED
+
+
+# ---------------------------------------------------------------- marching cubes
+@pytest.mark.parametrize("case", ["r5", "config1", "config1_advected", "slotted", "offset_negative"])
+def test_marching_cubes_matches(orc, ctx, case):
+    announce(f"marching cubes {case}")
+    if case == "r5":
+        og, vs = orc.sphere(5.0, (0, 0, 0), 1.0, 3.0), 1.0
+    elif case == "config1":
+        og, vs = config1_sphere(orc), 1.0 / 128
+    elif case == "config1_advected":
+        vs = 1.0 / 128
+        og = orc.advect_analytic(config1_sphere(orc), 3, (0, 0, 2 * math.pi, 0.5, 0.5, 0.5), config1_dt())
+    elif case == "slotted":
+        og, vs = slotted_sphere(orc, 1.0 / 128), 1.0 / 128
+    else:
+        og, vs = orc.sphere(3.0, (-10.0, 20.0, -30.0), 0.5, 3.0), 0.5
+    vw, tw, wt = orc.mesh(og, 0.0)
+    vg, tg = ctx.upload(og.to_nano()).mesh(0.0)
+    mesh_equal(vg, tg, vw, tw, vs, case)
+    assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"{case}: vertices not bit-exact"
+
+
+def test_marching_cubes_empty(orc, ctx):
+    v, t = ctx.upload(orc.new_grid(3.0, 1.0).to_nano()).mesh(0.0)
+    assert len(v) == 0 and len(t) == 0
+
+
+def test_counts(orc, ctx):
+    og = config1_sphere(orc)
+    assert ctx.upload(og.to_nano()).counts() == (og.leaf_count(), og.active_count())
+
+
+# ---------------------------------------------------------------- device fixture generation
+def test_fixture_sphere_matches_reference_generator(orc, ctx, vr):
+    """vr_grid_from_primitives(sphere) == make_level_set_sphere (sphere.rs:17-59)."""
+    announce("fixture sphere")
+    for r, c, vs, hw in [(0.3, (0.5, 0.5, 0.5), 1.0 / 128, 3.0), (5.0, (0, 0, 0), 1.0, 3.0),
+                         (3.0, (10.0, -20.0, 30.0), 0.5, 2.0)]:
+        want = orc.sphere(r, c, vs, hw).to_nano()
+        got = ctx.from_primitives([(vr.PRIM_SPHERE, 0, (*c, r))], vs, hw).download()
+        assert_nano_equal(got, want, f"sphere r={r}")
+
+
+def test_fixture_scenes_match_csg_chain(orc, ctx, vr):
+    """Slotted sphere (csg_difference) and sphere unions (csg_union chain)."""
+    announce("fixture scenes")
+    vs = 1.0 / 128
+    want = slotted_sphere(orc, vs).to_nano()
+    got = ctx.from_primitives([(vr.PRIM_SPHERE, 0, (0.5, 0.75, 0.5, 0.15)),
+                               (vr.PRIM_BOX, vr.CSG_DIFFERENCE, (0.475, 0.55, 0.3, 0.525, 0.725, 0.7))],
+                              vs, 3.0).download()
+    assert_nano_equal(got, want, "slotted")
+    rng = np.random.default_rng(11)
+    prims = [(0, tuple(rng.uniform(0.2, 0.8, 3)) + (float(np.exp(rng.uniform(np.log(0.02), np.log(0.2)))),))
+             for _ in range(5)]
+    want = orc.scene([(k, p) for k, p in prims], [0] * len(prims), vs, 3.0).to_nano()
+    got = ctx.from_primitives([(k, vr.CSG_UNION, p) for k, p in prims], vs, 3.0).download()
+    assert_nano_equal(got, want, "union")
+    noisy = [(2, (0.5, 0.5, 0.5, 0.3, 0.02, 4.0, 7.0))]
+    want = orc.scene(noisy, [0], vs, 3.0).to_nano()
+    got = ctx.from_primitives([(vr.PRIM_NOISY_SPHERE, 0, noisy[0][1])], vs, 3.0).download()
+    assert_nano_equal(got, want, "noisy")
