@@ -155,6 +155,25 @@ int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, double t0, doubl
                int rebuild_every, float half_width, int mesh_every, float iso, vr_mesh* mesh,
                vr_rollout_stats* stats);
 
+/* ---- differentiable rollout -------------------------------------------- */
+/* No counterpart in the reference (it has no derivatives).  `steps` advection
+ * steps (scheme RK1 or RK3) of phi0 through a static dense velocity field,
+ * topology fixed, then loss = 1/2 sum over active voxels of
+ * (phi_steps - target.get(x))^2 and its gradient with respect to the dense
+ * field, written to grad (3 n^3 floats, the field's layout; may be NULL).
+ * Forward states are kept every `checkpoint_every` steps and the others are
+ * recomputed in the reverse pass.  The gradient is summed with float
+ * atomics, so its last bits can vary from run to run. */
+typedef struct vr_adjoint_stats {
+  double loss;
+  double forward_ms; /* device time of the forward steps */
+  double adjoint_ms; /* device time of recomputation + reverse steps */
+  uint64_t steps;
+  uint64_t active_voxels;
+} vr_adjoint_stats;
+int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_velocity* vel, double dt, int scheme, int steps,
+                       int checkpoint_every, const vr_grid* target, vr_adjoint_stats* stats, float* grad);
+
 /* ---- measurement -------------------------------------------------------- */
 /* Kernel classes for vr_ctx_timing. */
 enum { VR_TIMING_ADVECT = 0, VR_TIMING_CROSSINGS = 1, VR_TIMING_REBUILD = 2, VR_TIMING_COMPACT = 3, VR_TIMING_MC = 4 };

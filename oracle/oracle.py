@@ -66,8 +66,9 @@ _SIGS = {
     "or_mesh_watertight": (_I, [_P]),
     "or_mesh_free": (None, [_P]),
     "or_mc_tiles": (_S, [_P, _P, _S]),
-    "or_rollout_loss_f64": (_D, [_P, _I, _D, _P, _D, _I, _P, _D]),
-    "or_rollout_adjoint_f64": (_D, [_P, _I, _D, _P, _D, _I, _P, _D, _P]),
+    "or_rollout_loss_f64": (_D, [_P, _I, _D, _P, _D, _I, _I, _P]),
+    "or_rollout_adjoint_f64": (_D, [_P, _I, _D, _P, _D, _I, _I, _P, _P]),
+    "or_rollout_dense": (_P, [_P, _I, _D, _P, _D, _I, _I]),
     "or_time_advect_mc": (_D, [_P, _I, _P, _D, _F, _P]),
     "or_time_step": (_D, [_P, _I, _P, _D, _I, _P, _I, _F, _I, _F]),
 }
@@ -297,18 +298,21 @@ def mc_tiles(g):
     return out[:n]
 
 
-def rollout_loss_f64(g, u64, n, h, dt, steps, ct, rt):
+def rollout_loss_f64(g, u64, n, h, dt, scheme, steps, target):
     u = np.ascontiguousarray(u64, dtype=np.float64).reshape(-1)
-    c = _dvec(ct, 3)
-    return lib().or_rollout_loss_f64(g.h, n, h, _ptr(u), dt, steps, _ptr(c), rt)
+    return lib().or_rollout_loss_f64(g.h, n, h, _ptr(u), dt, scheme, steps, target.h)
 
 
-def rollout_adjoint_f64(g, u64, n, h, dt, steps, ct, rt):
+def rollout_adjoint_f64(g, u64, n, h, dt, scheme, steps, target):
     u = np.ascontiguousarray(u64, dtype=np.float64).reshape(-1)
-    c = _dvec(ct, 3)
     grad = np.zeros(3 * n ** 3, dtype=np.float64)
-    loss = lib().or_rollout_adjoint_f64(g.h, n, h, _ptr(u), dt, steps, _ptr(c), rt, _ptr(grad))
+    loss = lib().or_rollout_adjoint_f64(g.h, n, h, _ptr(u), dt, scheme, steps, target.h, _ptr(grad))
     return loss, grad
+
+
+def rollout_dense(g, uvw, n, h, dt, scheme, steps):
+    u = np.ascontiguousarray(uvw, dtype=np.float32).reshape(-1)
+    return OGrid(lib().or_rollout_dense(g.h, n, h, _ptr(u), dt, scheme, steps))
 
 
 def time_step(g, kind, params, dt, scheme, tf, do_rebuild, hw, do_mc, iso=0.0):

@@ -175,18 +175,6 @@ size_t or_mc_tiles(void* g, int32_t* out, size_t cap) {
 }
 
 // ---------------------------------------------------------------- adjoint (FP64)
-// Target = (float)(|x - c_t| - r_t) at each active voxel of phi0 (fixed
-// topology under advection).
-static std::unordered_map<Coord, double, CoordHash> target_map(const Grid<float>& g, const double* ct, double rt) {
-  std::unordered_map<Coord, double, CoordHash> t;
-  g.tree.for_each_active([&](Coord c, float) {
-    double w[3];
-    g.affine.index_to_world(c, w);
-    const double dx = w[0] - ct[0], dy = w[1] - ct[1], dz = w[2] - ct[2];
-    t[c] = (double)(float)(std::sqrt(dx * dx + dy * dy + dz * dz) - rt);
-  });
-  return t;
-}
 static DenseVelocityD make_dense_d(int n, double h, const double* u) {
   DenseVelocityD d;
   d.n = n;
@@ -195,18 +183,25 @@ static DenseVelocityD make_dense_d(int n, double h, const double* u) {
   for (int c = 0; c < 3; ++c) d.u[c].assign(u + c * nn, u + (c + 1) * nn);
   return d;
 }
-double or_rollout_loss_f64(void* g, int n, double h, const double* u, double dt, int steps, const double* ct,
-                           double rt) {
+double or_rollout_loss_f64(void* g, int n, double h, const double* u, double dt, int scheme, int steps, void* target) {
   const Grid<double> gd = convert_grid<double, float>(*G(g));
-  return rollout_loss_d(gd, make_dense_d(n, h, u), dt, steps, target_map(*G(g), ct, rt));
+  return rollout_loss_d(gd, make_dense_d(n, h, u), dt, scheme, steps, *G(target));
 }
-double or_rollout_adjoint_f64(void* g, int n, double h, const double* u, double dt, int steps, const double* ct,
-                              double rt, double* grad) {
+double or_rollout_adjoint_f64(void* g, int n, double h, const double* u, double dt, int scheme, int steps,
+                              void* target, double* grad) {
   const Grid<double> gd = convert_grid<double, float>(*G(g));
-  const AdjointResult r = rollout_adjoint_d(gd, make_dense_d(n, h, u), dt, steps, target_map(*G(g), ct, rt));
+  const AdjointResult r = rollout_adjoint_d(gd, make_dense_d(n, h, u), dt, scheme, steps, *G(target));
   const size_t nn = (size_t)n * n * n;
   for (int c = 0; c < 3; ++c) std::memcpy(grad + c * nn, r.grad[c].data(), nn * sizeof(double));
   return r.loss;
+}
+// f32 forward rollout (the GPU's arithmetic) and its loss against a target grid.
+void* or_rollout_dense(void* g, int n, double h, const float* u, double dt, int scheme, int steps) {
+  const double tf[3] = {1.0, 1.0, 1.0};
+  const DenseVelocity d = make_dense(n, h, u);
+  Grid<float> cur = advect_scheme(*G(g), d, dt, scheme, tf);
+  for (int s = 1; s < steps; ++s) cur = advect_scheme(cur, d, dt, scheme, tf);
+  return new Grid<float>(std::move(cur));
 }
 
 // ---------------------------------------------------------------- CPU timing

@@ -310,30 +310,15 @@ inline Grid<float> make_scene_grid(const std::vector<Primitive>& prims, const st
 }
 
 // ---------------------------------------------------------------- adjoint (FP64)
-// Builder-defined: K RK1 steps of a fixed-topology level set through a
-// static dense velocity U, loss L = 1/2 sum_active (phi_K - target)^2.
-// Returns dL/dU (3 n^3) by reverse-mode differentiation of the real-valued
-// step phi_{n+1}(x_i) = S(phi_n, x_i - dt u(x_i)).  Only active voxels are
-// variables; inactive and background values are constants.
+// Builder-defined: K steps (RK1 or RK3) of a fixed-topology level set
+// through a static dense velocity U, loss L = 1/2 sum_active (phi_K - t)^2
+// with t = target.get(x).  dL/dU (3 n^3) by reverse-mode differentiation of
+// the real-valued step phi_{n+1}(x_i) = S(phi_n, p_i(U)).  Only active voxels
+// are variables; inactive and background values are constants.
 struct AdjointResult {
   double loss = 0.0;
   std::vector<double> grad[3];
 };
-
-template <class T>
-double rollout_loss(const Grid<T>& phi0, const DenseVelocity& vel, double dt, int steps,
-                    const std::unordered_map<Coord, T, CoordHash>& target, Grid<T>* final_out = nullptr) {
-  const double tf[3] = {1.0, 1.0, 1.0};
-  Grid<T> g = advect_scheme(phi0, vel, dt, SCHEME_RK1, tf);
-  for (int s = 1; s < steps; ++s) g = advect_scheme(g, vel, dt, SCHEME_RK1, tf);
-  double loss = 0.0;
-  g.tree.for_each_active([&](Coord c, T v) {
-    const double d = (double)v - (double)target.at(c);
-    loss += 0.5 * d * d;
-  });
-  if (final_out) *final_out = std::move(g);
-  return loss;
-}
 
 // Dense velocity of a double-valued field, for FP64 forward and FD checks.
 struct DenseVelocityD {
@@ -344,15 +329,18 @@ struct DenseVelocityD {
     if (i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n) return 0.0;
     return u[c][((size_t)i * n + j) * n + k];
   }
-  void sample(const double p[3], double out[3]) const {
+  void base_frac(const double p[3], int32_t b[3], double f[3]) const {
     const double inv = 1.0 / h;
-    double f[3];
-    int32_t b[3];
     for (int a = 0; a < 3; ++a) {
       const double idx = p[a] * inv, fl = std::floor(idx);
       b[a] = f64_to_i32(fl);
       f[a] = idx - fl;
     }
+  }
+  void sample(const double p[3], double out[3]) const {
+    int32_t b[3];
+    double f[3];
+    base_frac(p, b, f);
     for (int c = 0; c < 3; ++c) {
       double acc = 0.0;
       for (int q = 0; q < 8; ++q) {
@@ -363,33 +351,63 @@ struct DenseVelocityD {
       out[c] = acc;
     }
   }
+  // grad[c][node(p) q] += coef[c] * w_q(p)
+  void scatter(const double p[3], const double coef[3], std::vector<double>* grad) const {
+    int32_t b[3];
+    double f[3];
+    base_frac(p, b, f);
+    for (int q = 0; q < 8; ++q) {
+      const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+      const int32_t i = b[0] + dx, j = b[1] + dy, k = b[2] + dz;
+      if (i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n) continue;
+      const double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
+      for (int c = 0; c < 3; ++c) grad[c][((size_t)i * n + j) * n + k] += coef[c] * w;
+    }
+  }
+  // (J^T g)_a = sum_c g_c du_c/dp_a
+  void jt(const double p[3], const double g[3], double out[3]) const {
+    int32_t b[3];
+    double f[3];
+    base_frac(p, b, f);
+    out[0] = out[1] = out[2] = 0.0;
+    for (int q = 0; q < 8; ++q) {
+      const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+      const double wx = dx ? f[0] : 1.0 - f[0], wy = dy ? f[1] : 1.0 - f[1], wz = dz ? f[2] : 1.0 - f[2];
+      double gu = 0.0;
+      for (int c = 0; c < 3; ++c) gu += g[c] * at(c, b[0] + dx, b[1] + dy, b[2] + dz);
+      out[0] += gu * (dx ? 1.0 : -1.0) * wy * wz;
+      out[1] += gu * wx * (dy ? 1.0 : -1.0) * wz;
+      out[2] += gu * wx * wy * (dz ? 1.0 : -1.0);
+    }
+    for (int a = 0; a < 3; ++a) out[a] /= h;
+  }
 };
 
-inline double rollout_loss_d(const Grid<double>& phi0, const DenseVelocityD& vel, double dt, int steps,
-                             const std::unordered_map<Coord, double, CoordHash>& target) {
+inline double rollout_loss_d(const Grid<double>& phi0, const DenseVelocityD& vel, double dt, int scheme, int steps,
+                             const Grid<float>& target) {
   const double tf[3] = {1.0, 1.0, 1.0};
-  Grid<double> g = advect_scheme(phi0, vel, dt, SCHEME_RK1, tf);
-  for (int s = 1; s < steps; ++s) g = advect_scheme(g, vel, dt, SCHEME_RK1, tf);
+  Grid<double> g = advect_scheme(phi0, vel, dt, scheme, tf);
+  for (int s = 1; s < steps; ++s) g = advect_scheme(g, vel, dt, scheme, tf);
   double loss = 0.0;
   g.tree.for_each_active([&](Coord c, double v) {
-    const double d = v - target.at(c);
+    const double d = v - (double)target.get(c);
     loss += 0.5 * d * d;
   });
   return loss;
 }
 
-inline AdjointResult rollout_adjoint_d(const Grid<double>& phi0, const DenseVelocityD& vel, double dt, int steps,
-                                       const std::unordered_map<Coord, double, CoordHash>& target) {
+inline AdjointResult rollout_adjoint_d(const Grid<double>& phi0, const DenseVelocityD& vel, double dt, int scheme,
+                                       int steps, const Grid<float>& target) {
   const double tf[3] = {1.0, 1.0, 1.0};
   std::vector<Grid<double>> states;
   states.push_back(clone_grid(phi0));
-  for (int s = 0; s < steps; ++s) states.push_back(advect_scheme(states.back(), vel, dt, SCHEME_RK1, tf));
+  for (int s = 0; s < steps; ++s) states.push_back(advect_scheme(states.back(), vel, dt, scheme, tf));
 
   AdjointResult res;
   for (int c = 0; c < 3; ++c) res.grad[c].assign((size_t)vel.n * vel.n * vel.n, 0.0);
   std::unordered_map<Coord, double, CoordHash> lam;
   states.back().tree.for_each_active([&](Coord c, double v) {
-    const double d = v - target.at(c);
+    const double d = v - (double)target.get(c);
     res.loss += 0.5 * d * d;
     lam[c] = d;
   });
@@ -401,10 +419,18 @@ inline AdjointResult rollout_adjoint_d(const Grid<double>& phi0, const DenseVelo
     g.tree.for_each_active([&](Coord c, double) {
       const double L = lam.at(c);
       if (L == 0.0) return;
-      double x[3], u[3];
+      double x[3], u[3], y1[3], y2[3], p[3];
       g.affine.index_to_world(c, x);
       vel.sample(x, u);
-      const double p[3] = {x[0] - dt * u[0], x[1] - dt * u[1], x[2] - dt * u[2]};
+      for (int a = 0; a < 3; ++a) y1[a] = x[a] - dt * u[a];
+      if (scheme == SCHEME_RK1) {
+        for (int a = 0; a < 3; ++a) p[a] = y1[a];
+      } else {
+        vel.sample(y1, u);
+        for (int a = 0; a < 3; ++a) y2[a] = 0.75 * x[a] + 0.25 * (y1[a] - dt * u[a]);
+        vel.sample(y2, u);
+        for (int a = 0; a < 3; ++a) p[a] = (1.0 / 3.0) * x[a] + (2.0 / 3.0) * (y2[a] - dt * u[a]);
+      }
       double idx[3], f[3];
       int32_t b[3];
       g.transform.world_to_index_f64(p, idx);
@@ -413,28 +439,33 @@ inline AdjointResult rollout_adjoint_d(const Grid<double>& phi0, const DenseVelo
         b[a] = f64_to_i32(fl);
         f[a] = idx[a] - fl;
       }
-      double cv[8];
-      for (int q = 0; q < 8; ++q) cv[q] = g.get(Coord(b[0] + (q & 1), b[1] + ((q >> 1) & 1), b[2] + ((q >> 2) & 1)));
-      // dS/dp (world) = dS/dfrac * inv_vs
       double dS[3] = {0, 0, 0};
       for (int q = 0; q < 8; ++q) {
         const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
-        const double wx = dx ? f[0] : 1.0 - f[0], wy = dy ? f[1] : 1.0 - f[1], wz = dz ? f[2] : 1.0 - f[2];
-        dS[0] += cv[q] * (dx ? 1.0 : -1.0) * wy * wz;
-        dS[1] += cv[q] * wx * (dy ? 1.0 : -1.0) * wz;
-        dS[2] += cv[q] * wx * wy * (dz ? 1.0 : -1.0);
         const Coord cc(b[0] + dx, b[1] + dy, b[2] + dz);
+        const double cv = g.get(cc);
+        const double wx = dx ? f[0] : 1.0 - f[0], wy = dy ? f[1] : 1.0 - f[1], wz = dz ? f[2] : 1.0 - f[2];
+        dS[0] += cv * (dx ? 1.0 : -1.0) * wy * wz;
+        dS[1] += cv * wx * (dy ? 1.0 : -1.0) * wz;
+        dS[2] += cv * wx * wy * (dz ? 1.0 : -1.0);
         if (g.tree.is_active(cc)) lam_prev[cc] += L * wx * wy * wz;
       }
-      int32_t vb[3];
-      double vw[8];
-      const DenseVelocity shim{vel.n, vel.h, {}};
-      shim.weights(x, vb, vw);
-      for (int q = 0; q < 8; ++q) {
-        const int32_t i = vb[0] + (q & 1), j = vb[1] + ((q >> 1) & 1), k = vb[2] + ((q >> 2) & 1);
-        if (i < 0 || j < 0 || k < 0 || i >= vel.n || j >= vel.n || k >= vel.n) continue;
-        const size_t id = ((size_t)i * vel.n + j) * vel.n + k;
-        for (int a = 0; a < 3; ++a) res.grad[a][id] += L * dS[a] * inv_vs * (-dt) * vw[q];
+      const double gp[3] = {L * dS[0] * inv_vs, L * dS[1] * inv_vs, L * dS[2] * inv_vs};
+      if (scheme == SCHEME_RK1) {
+        const double cf[3] = {-dt * gp[0], -dt * gp[1], -dt * gp[2]};
+        vel.scatter(x, cf, res.grad);
+      } else {
+        double jt[3], gy2[3], gy1[3];
+        vel.jt(y2, gp, jt);
+        for (int a = 0; a < 3; ++a) gy2[a] = (2.0 / 3.0) * gp[a] - (2.0 / 3.0) * dt * jt[a];
+        const double c3[3] = {-(2.0 / 3.0) * dt * gp[0], -(2.0 / 3.0) * dt * gp[1], -(2.0 / 3.0) * dt * gp[2]};
+        vel.scatter(y2, c3, res.grad);
+        vel.jt(y1, gy2, jt);
+        for (int a = 0; a < 3; ++a) gy1[a] = 0.25 * gy2[a] - 0.25 * dt * jt[a];
+        const double c2[3] = {-0.25 * dt * gy2[0], -0.25 * dt * gy2[1], -0.25 * dt * gy2[2]};
+        vel.scatter(y1, c2, res.grad);
+        const double c1[3] = {-dt * gy1[0], -dt * gy1[1], -dt * gy1[2]};
+        vel.scatter(x, c1, res.grad);
       }
     });
     lam.swap(lam_prev);

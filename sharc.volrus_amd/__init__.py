@@ -76,6 +76,8 @@ _SIGS = {
     "vr_mesh_destroy": (ctypes.c_int, [_P, _P]),
     "vr_mesh_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "vr_grid_clone": (ctypes.c_int, [_P, _P, ctypes.POINTER(_P)]),
+    "vr_adjoint_rollout": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _P, _P, _P]),
     "vr_rollout": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, _P, _P]),
     "vr_ctx_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -88,6 +90,11 @@ TIMING_CROSSINGS = 1
 TIMING_REBUILD = 2
 TIMING_COMPACT = 3
 TIMING_MC = 4
+
+
+class AdjointStats(ctypes.Structure):
+    _fields_ = [("loss", ctypes.c_double), ("forward_ms", ctypes.c_double), ("adjoint_ms", ctypes.c_double),
+                ("steps", ctypes.c_uint64), ("active_voxels", ctypes.c_uint64)]
 
 
 class RolloutStats(ctypes.Structure):
@@ -246,6 +253,14 @@ class Grid:
                                                half_width, mesh_every, iso, mesh.h if mesh else None,
                                                ctypes.byref(st)))
         return st
+
+    def adjoint_rollout(self, vel, n, dt, scheme, steps, checkpoint_every, target, want_grad=True):
+        """Differentiable rollout through a dense field of n^3 nodes; returns (AdjointStats, grad[3, n, n, n])."""
+        st = AdjointStats()
+        grad = np.zeros(3 * n ** 3, dtype=np.float32) if want_grad else None
+        self.ctx.check(self.ctx.lib.vr_adjoint_rollout(self.ctx.h, self.h, vel.h, dt, scheme, steps,
+                                                       checkpoint_every, target.h, ctypes.byref(st), _ptr(grad)))
+        return st, (grad.reshape(3, n, n, n) if want_grad else None)
 
     def zero_crossings(self):
         n = ctypes.c_size_t()

@@ -123,6 +123,19 @@ __device__ __forceinline__ void index_to_world(const double m[12], int x, int y,
   for (int r = 0; r < 3; ++r) w[r] = m[4 * r] * fx + m[4 * r + 1] * fy + m[4 * r + 2] * fz + m[4 * r + 3];
 }
 
+// Stages the 10^3 brick (local coordinates -1..8) around the leaf whose 27
+// neighbour indices are nb[] (nb[13] is the leaf itself) into LDS; missing
+// leaves read as the background.  One wave, lane in [0, 64).
+__device__ __forceinline__ void stage_brick_dev(const GridView& in, const int* nb, float* brick, int lane) {
+  for (int i = lane; i < 1000; i += 64) {
+    const int bx = i / 100 - 1, by = (i / 10) % 10 - 1, bz = i % 10 - 1;
+    const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
+              dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
+    const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+    brick[i] = src < 0 ? in.background : in.leaves[src].values[voxel_offset(bx, by, bz)];
+  }
+}
+
 // ------------------------------------------------------------------ sinpi / cospi
 // Exact reduction of x to f in [-1/4, 1/4] and the Taylor series of sin(pi f)
 // and cos(pi f) to f^19 / f^20, Horner in plain IEEE double (no fused ops),

@@ -5,6 +5,8 @@
 // wave's ballot layout (mask bit = x * 64 + lane, src/tree/leaf.rs:52-56).
 // Leaf bricks plus halo are staged in LDS; every count and offset is made by
 // wave/block prefix scans, so outputs are deterministic.
+#include <cstdlib>
+
 #include "vr_kernels.h"
 
 // ------------------------------------------------------------------ MC tables (device)
@@ -124,12 +126,38 @@ __global__ void k_scan_add(int* out, const int* sums, size_t n) {
 }
 
 // ------------------------------------------------------------------ advection
-// One wave per leaf.  The leaf plus a one-voxel halo (10^3) is staged in
-// LDS; departure-cell corners inside that brick are read from LDS, any other
-// corner goes through the root/internal lookup, so any dt is handled.
+// New value of the active voxel (ox + x, oy + y, oz + z): back-trace, then
+// trilinear sampling of the input.  Departure-cell corners inside the LDS
+// brick (leaf plus one-voxel halo, 10^3) come from LDS; any other corner goes
+// through the root/internal lookup, so any dt is handled.
 template <int VK, int SCHEME>
-__global__ __launch_bounds__(64) void k_advect(GridView in, VelView vel, NanoLeaf* out, double dt, double tf0,
-                                               double tf1, double tf2) {
+__device__ __forceinline__ float advect_voxel(const GridView& in, const VelView& vel, const float* brick, int ox,
+                                              int oy, int oz, int x, int y, int z, double dt, const double tf[3]) {
+  double w[3], p[3];
+  index_to_world(in.mat, ox + x, oy + y, oz + z, w);
+  departure<VK, SCHEME>(vel, w, dt, tf, p);
+  const double ix = p[0] * in.inv_vs, iy = p[1] * in.inv_vs, iz = p[2] * in.inv_vs;
+  const double flx = floor(ix), fly = floor(iy), flz = floor(iz);
+  const float fx = (float)(ix - flx), fy = (float)(iy - fly), fz = (float)(iz - flz);
+  const int i = f64_to_i32(flx), j = f64_to_i32(fly), k = f64_to_i32(flz);
+  const int bi = i - ox, bj = j - oy, bk = k - oz;
+  float c[8];
+  if (bi >= -1 && bi <= 7 && bj >= -1 && bj <= 7 && bk >= -1 && bk <= 7) {
+    const int b0 = (bi + 1) * 100 + (bj + 1) * 10 + (bk + 1);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = brick[b0 + (q & 1) * 100 + ((q >> 1) & 1) * 10 + ((q >> 2) & 1)];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = grid_get(in, i + (q & 1), j + ((q >> 1) & 1), k + ((q >> 2) & 1));
+  }
+  return trilinear_blend(c, fx, fy, fz);
+}
+
+// Slab variant: lane = (y, z), the wave walks the 8 x-slabs; inactive lanes
+// idle while active ones compute.
+template <int VK, int SCHEME>
+__global__ __launch_bounds__(64) void k_advect_slab(GridView in, VelView vel, NanoLeaf* out, double dt, double tf0,
+                                                    double tf1, double tf2) {
   __shared__ float brick[1000];
   __shared__ int nb[VR_NBR];
   const int lane = threadIdx.x;
@@ -139,13 +167,7 @@ __global__ __launch_bounds__(64) void k_advect(GridView in, VelView vel, NanoLea
     const NanoLeaf& L = in.leaves[li];
     if (lane < VR_NBR) nb[lane] = in.nbr[(size_t)li * VR_NBR + lane];
     __syncthreads();
-    for (int i = lane; i < 1000; i += 64) {
-      const int bx = i / 100 - 1, by = (i / 10) % 10 - 1, bz = i % 10 - 1;
-      const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
-                dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
-      const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
-      brick[i] = src < 0 ? in.background : in.leaves[src].values[voxel_offset(bx, by, bz)];
-    }
+    stage_brick_dev(in, nb, brick, lane);
     __syncthreads();
     const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
     const int y = lane >> 3, z = lane & 7;
@@ -156,28 +178,56 @@ __global__ __launch_bounds__(64) void k_advect(GridView in, VelView vel, NanoLea
     for (int x = 0; x < 8; ++x) {
       const bool act = (L.mask[x] >> lane) & 1ull;
       float res = in.background;
-      if (act) {
-        double w[3], p[3];
-        index_to_world(in.mat, ox + x, oy + y, oz + z, w);
-        departure<VK, SCHEME>(vel, w, dt, tf, p);
-        const double ix = p[0] * in.inv_vs, iy = p[1] * in.inv_vs, iz = p[2] * in.inv_vs;
-        const double flx = floor(ix), fly = floor(iy), flz = floor(iz);
-        const float fx = (float)(ix - flx), fy = (float)(iy - fly), fz = (float)(iz - flz);
-        const int i = f64_to_i32(flx), j = f64_to_i32(fly), k = f64_to_i32(flz);
-        const int bi = i - ox, bj = j - oy, bk = k - oz;
-        float c[8];
-        if (bi >= -1 && bi <= 7 && bj >= -1 && bj <= 7 && bk >= -1 && bk <= 7) {
-          const int b0 = (bi + 1) * 100 + (bj + 1) * 10 + (bk + 1);
-#pragma unroll
-          for (int q = 0; q < 8; ++q) c[q] = brick[b0 + (q & 1) * 100 + ((q >> 1) & 1) * 10 + ((q >> 2) & 1)];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) c[q] = grid_get(in, i + (q & 1), j + ((q >> 1) & 1), k + ((q >> 2) & 1));
-        }
-        res = trilinear_blend(c, fx, fy, fz);
-      }
+      if (act) res = advect_voxel<VK, SCHEME>(in, vel, brick, ox, oy, oz, x, y, z, dt, tf);
       O.values[x * 64 + lane] = res;
     }
+    __syncthreads();
+  }
+}
+
+// Compacted variant: the active voxels of the leaf are listed in LDS by a
+// ballot/popcount prefix over the 8 mask words, the wave computes them
+// densely (about 3 passes of 64 for a band leaf instead of 8 slabs), and the
+// leaf is then written back slab by slab, coalesced, with the background in
+// the inactive positions.
+template <int VK, int SCHEME>
+__global__ __launch_bounds__(64) void k_advect(GridView in, VelView vel, NanoLeaf* out, double dt, double tf0,
+                                               double tf1, double tf2) {
+  __shared__ float brick[1000];
+  __shared__ float res[512];
+  __shared__ uint16_t list[512];
+  __shared__ int nb[VR_NBR];
+  const int lane = threadIdx.x;
+  const int n = *in.count;
+  const double tf[3] = {tf0, tf1, tf2};
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int li = blockIdx.x; li < n; li += gridDim.x) {
+    const NanoLeaf& L = in.leaves[li];
+    if (lane < VR_NBR) nb[lane] = in.nbr[(size_t)li * VR_NBR + lane];
+    uint64_t m[8];
+    int total = 0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      m[x] = L.mask[x];
+      if ((m[x] >> lane) & 1ull) list[total + __popcll(m[x] & below)] = (uint16_t)(x * 64 + lane);
+      total += __popcll(m[x]);
+    }
+    __syncthreads();
+    stage_brick_dev(in, nb, brick, lane);
+    __syncthreads();
+    const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
+    for (int i = lane; i < total; i += 64) {
+      const int off = list[i];
+      res[off] = advect_voxel<VK, SCHEME>(in, vel, brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt, tf);
+    }
+    __syncthreads();
+    NanoLeaf& O = out[li];
+    if (lane < 3) O.origin[lane] = L.origin[lane];
+    if (lane == 3) O.pad = 0;
+    if (lane < 8) O.mask[lane] = m[lane];
+#pragma unroll
+    for (int x = 0; x < 8; ++x)
+      O.values[x * 64 + lane] = ((m[x] >> lane) & 1ull) ? res[x * 64 + lane] : in.background;
     __syncthreads();
   }
 }
@@ -186,7 +236,15 @@ template <int VK, int SCHEME>
 static hipError_t launch_advect_t(const GridView& in, const VelView& vel, NanoLeaf* out, int cap, double dt,
                                   const double tf[3], hipStream_t s) {
   const int blocks = cap < 1 ? 1 : (cap < 65536 ? cap : 65536);
-  hipLaunchKernelGGL((k_advect<VK, SCHEME>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1], tf[2]);
+  static const int variant = [] {
+    const char* e = getenv("VOLRUS_ADVECT_VARIANT");
+    return (e && e[0] == 's') ? 1 : 0;
+  }();
+  if (variant == 1)
+    hipLaunchKernelGGL((k_advect_slab<VK, SCHEME>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1],
+                       tf[2]);
+  else
+    hipLaunchKernelGGL((k_advect<VK, SCHEME>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1], tf[2]);
   return hipGetLastError();
 }
 

@@ -1,0 +1,68 @@
+"""Differentiable rollout on the GPU (vr_adjoint_rollout) vs the FP64 CPU
+adjoint (itself pinned to central differences in test_oracle_math.py), and
+against central differences of the GPU's own forward loss."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def setup(orc, ctx, vr, n=16, vs=1.0 / 64, seed=4):
+    rng = np.random.default_rng(seed)
+    u = rng.normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32)
+    g0 = orc.sphere(0.3, (0.5, 0.5, 0.5), vs, 3.0)
+    tg = orc.sphere(0.3, (0.6, 0.5, 0.5), vs, 3.0)
+    dt = 0.5 * vs / float(np.abs(u).max())
+    return u, g0, tg, dt
+
+
+@pytest.mark.parametrize("scheme,ckpt", [(1, 1), (1, 3), (3, 2)])
+def test_adjoint_matches_cpu_adjoint(orc, ctx, vr, scheme, ckpt):
+    announce(f"adjoint scheme={scheme} checkpoint_every={ckpt}")
+    n, steps = 16, 6
+    u, g0, tg, dt = setup(orc, ctx, vr, n)
+    vel = ctx.velocity_dense(u, n, 1.0 / n)
+    g = ctx.upload(g0.to_nano())
+    target = ctx.upload(tg.to_nano())
+    st, grad = g.adjoint_rollout(vel, n, dt, scheme, steps, ckpt, target)
+    # forward loss: the f32 rollout is the oracle's f32 rollout bit for bit
+    fin = orc.rollout_dense(g0, u, n, 1.0 / n, dt, scheme, steps)
+    from util import active_voxels
+    c, v = active_voxels(fin.to_nano())
+    t = np.array([tg.get(*xyz) for xyz in c.tolist()], dtype=np.float64)
+    loss_f32 = 0.5 * float(((v.astype(np.float64) - t) ** 2).sum())
+    assert abs(st.loss - loss_f32) <= 1e-12 * loss_f32
+    loss64, g64 = orc.rollout_adjoint_f64(g0, u.astype(np.float64), n, 1.0 / n, dt, scheme, steps, tg)
+    g64 = g64.reshape(3, n, n, n)
+    scale = np.abs(g64).max()
+    err = np.abs(grad - g64).max()
+    assert err <= 2e-3 * scale, (err, scale)
+    rel = np.linalg.norm(grad - g64) / np.linalg.norm(g64)
+    assert rel < 1e-3, rel
+    assert abs(st.loss - loss64) <= 1e-4 * loss64
+    assert st.forward_ms > 0 and st.adjoint_ms > 0
+
+
+def test_adjoint_matches_central_differences_of_gpu_forward(orc, ctx, vr):
+    announce("adjoint vs FD of the GPU forward")
+    n, steps, scheme = 16, 6, 1
+    u, g0, tg, dt = setup(orc, ctx, vr, n)
+    g = ctx.upload(g0.to_nano())
+    target = ctx.upload(tg.to_nano())
+    st, grad = g.adjoint_rollout(ctx.velocity_dense(u, n, 1.0 / n), n, dt, scheme, steps, 1, target)
+    flat = grad.reshape(-1)
+    rng = np.random.default_rng(9)
+    big = np.nonzero(np.abs(flat) > 0.05 * np.abs(flat).max())[0]
+    h = 1e-3
+    for i in rng.choice(big, 8, replace=False):
+        up, dn = u.reshape(-1).copy(), u.reshape(-1).copy()
+        up[i] += h
+        dn[i] -= h
+        lu = g.adjoint_rollout(ctx.velocity_dense(up, n, 1.0 / n), n, dt, scheme, steps, 1, target, False)[0].loss
+        ld = g.adjoint_rollout(ctx.velocity_dense(dn, n, 1.0 / n), n, dt, scheme, steps, 1, target, False)[0].loss
+        fd = (lu - ld) / (2 * h)
+        assert abs(fd - flat[i]) <= 2e-2 * abs(flat).max(), (i, fd, flat[i])

@@ -309,25 +309,26 @@ def test_rebuild_values_are_lattice_distances(orc):
 
 
 # ---------------------------------------------------------------- adjoint
-def test_adjoint_matches_central_differences(orc):
-    """FP64 adjoint of a K-step RK1 rollout through a dense trilinear velocity
-    vs central differences (h = 1e-3) on random velocity entries."""
+@pytest.mark.parametrize("scheme", [1, 3])
+def test_adjoint_matches_central_differences(orc, scheme):
+    """FP64 adjoint of a K-step RK1/RK3 rollout through a dense trilinear
+    velocity vs central differences (h = 1e-3) on random velocity entries."""
     n = 6
     rng = np.random.default_rng(7)
     u = rng.normal(0, 0.5, size=3 * n ** 3)
     vs = 1.0 / 32
     g = orc.sphere(0.3, (0.5, 0.5, 0.5), vs, 3.0)
+    target = orc.sphere(0.3, (0.6, 0.5, 0.5), vs, 3.0)
     dt = 0.5 * vs / np.abs(u).max()
     steps = 4
-    ct, rt = (0.6, 0.5, 0.5), 0.3
-    loss, grad = orc.rollout_adjoint_f64(g, u, n, 1.0 / n, dt, steps, ct, rt)
-    assert abs(loss - orc.rollout_loss_f64(g, u, n, 1.0 / n, dt, steps, ct, rt)) < 1e-12 * max(1.0, abs(loss))
+    loss, grad = orc.rollout_adjoint_f64(g, u, n, 1.0 / n, dt, scheme, steps, target)
+    assert abs(loss - orc.rollout_loss_f64(g, u, n, 1.0 / n, dt, scheme, steps, target)) < 1e-12 * max(1.0, loss)
     h = 1e-3
     idx = rng.choice(np.nonzero(np.abs(grad) > 1e-6 * np.abs(grad).max())[0], 12, replace=False)
     for i in idx:
         up, dn = u.copy(), u.copy()
         up[i] += h
         dn[i] -= h
-        fd = (orc.rollout_loss_f64(g, up, n, 1.0 / n, dt, steps, ct, rt) -
-              orc.rollout_loss_f64(g, dn, n, 1.0 / n, dt, steps, ct, rt)) / (2 * h)
+        fd = (orc.rollout_loss_f64(g, up, n, 1.0 / n, dt, scheme, steps, target) -
+              orc.rollout_loss_f64(g, dn, n, 1.0 / n, dt, scheme, steps, target)) / (2 * h)
         assert abs(fd - grad[i]) <= 1e-3 * abs(grad).max() + 1e-2 * abs(fd), (i, fd, grad[i])
