@@ -61,6 +61,94 @@ def traffic_from_profiles():
     return None
 
 
+class SplitMix64:
+    """Steele, Lea, Flood (OOPSLA 2014) generator, for the seeded scene."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.s = seed & self.MASK
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def config3_scenes(vr, count=64, seed=0):
+    """64 level sets, each the csg_union of 1..32 spheres with centres uniform
+    in [0,1]^3 and radii log-uniform in [0.01, 0.2], and a uniform velocity
+    with components uniform in [-1, 1] (SplitMix64, seed 0)."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        ns = 1 + min(31, int(rng.uniform() * 32))
+        prims = []
+        for _ in range(ns):
+            c = (rng.uniform(), rng.uniform(), rng.uniform())
+            r = math.exp(math.log(0.01) + rng.uniform() * (math.log(0.2) - math.log(0.01)))
+            prims.append((vr.PRIM_SPHERE, vr.CSG_UNION, (c[0], c[1], c[2], r)))
+        vel = tuple(2.0 * rng.uniform() - 1.0 for _ in range(3))
+        out.append((prims, vel))
+    return out
+
+
+def batched_extra(vr, ctx, reps=3):
+    """Config 3: batched persistent graph step vs the per-grid loop, one GPU."""
+    vs = 1.0 / 1024
+    t0 = time.perf_counter()
+    spec = config3_scenes(vr)
+    grids = [ctx.from_primitives(p, vs, HW) for p, _ in spec]
+    vels = [ctx.velocity_analytic(vr.VEL_UNIFORM, v) for _, v in spec]
+    dts = [0.5 * vs / max(abs(c) for c in v) for _, v in spec]
+    counts = [g.counts() for g in grids]
+    active = sum(a for _, a in counts)
+    log(f"config 3: 64 grids, {sum(l for l, _ in counts)} leaves, {active} active voxels "
+        f"(min {min(a for _, a in counts)}, max {max(a for _, a in counts)}), built in {time.perf_counter() - t0:.1f} s")
+    b = vr.Batch(ctx, grids, vels)
+    b.set_dt(dts)
+    lpg, wpc = b.autotune(dts[0], vr.RK3)
+    res = {"grids": len(grids), "active_voxels": active, "leaves_per_grab": lpg, "waves_per_cu": wpc}
+    for name, mode in [("loop", vr.BATCH_LOOP), ("graph", vr.BATCH_GRAPH)]:
+        b.step(0.0, 0.0, vr.RK3, 0.0, mode)  # warm (sizes, captures)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            b.step(0.0, 0.0, vr.RK3, 0.0, mode)
+        ctx.synchronize()
+        dtw = (time.perf_counter() - t1) / reps
+        res[f"{name}_ms"] = round(1e3 * dtw, 3)
+        res[f"{name}_updates_per_s"] = round(active / dtw, 1)
+    res["speedup_graph_vs_loop"] = round(res["loop_ms"] / res["graph_ms"], 3)
+    b.close()
+    return res
+
+
+def adjoint_extra(vr, ctx):
+    """Config 4: 20 RK1 steps of a 1/256 sphere through a dense 64^3 field
+    (seeded Gaussian, sigma 0.5), loss vs the sphere moved by (0.1, 0, 0),
+    gradient by the adjoint with a checkpoint at every step."""
+    import numpy as np
+
+    vs, n, steps = 1.0 / 256, 64, 20
+    rng = np.random.default_rng(0)
+    u = rng.normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32)
+    phi = ctx.from_primitives([(vr.PRIM_SPHERE, 0, (0.5, 0.5, 0.5, 0.3))], vs, HW)
+    target = ctx.from_primitives([(vr.PRIM_SPHERE, 0, (0.6, 0.5, 0.5, 0.3))], vs, HW)
+    vel = ctx.velocity_dense(u, n, 1.0 / n)
+    dt = 0.5 * vs / float(np.abs(u).max())
+    phi.adjoint_rollout(vel, n, dt, vr.RK1, steps, 1, target)  # warm
+    st, _ = phi.adjoint_rollout(vel, n, dt, vr.RK1, steps, 1, target)
+    return {"steps": steps, "active_voxels": st.active_voxels, "forward_ms": round(st.forward_ms, 3),
+            "adjoint_ms": round(st.adjoint_ms, 3), "adjoint_over_forward": round(st.adjoint_ms / st.forward_ms, 3),
+            "loss": st.loss}
+
+
 def cpu_baseline(seconds_budget):
     """Oracle (literal C++ restatement of the reference CPU path) on a bounded
     sample of the same workload: the first steps of the rollout, single thread."""
@@ -96,6 +184,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the config 3 (batch) and config 4 (adjoint) lines")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -148,6 +237,19 @@ def main():
                 "frac": round(adv["achieved_GBps"] / PEAK_GBPS, 4), "traffic": traffic}
     log(f"kernels: {json.dumps(kern)}")
 
+    extras = {}
+    if not args.no_extras:
+        try:
+            extras["adjoint_config4"] = adjoint_extra(vr, ctx)
+            log(f"adjoint: {extras['adjoint_config4']}")
+        except Exception as e:  # reported, not fatal for the headline line
+            extras["adjoint_config4"] = {"error": str(e)}
+        try:
+            extras["batched_config3"] = batched_extra(vr, ctx)
+            log(f"batched: {extras['batched_config3']}")
+        except Exception as e:
+            extras["batched_config3"] = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
@@ -177,6 +279,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kern,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -174,9 +174,43 @@ typedef struct vr_adjoint_stats {
 int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_velocity* vel, double dt, int scheme, int steps,
                        int checkpoint_every, const vr_grid* target, vr_adjoint_stats* stats, float* grad);
 
+/* ---- batches of level sets ----------------------------------------------- */
+/* No counterpart in the reference.  A batch advects every grid with its own
+ * velocity and meshes the results in one step: one persistent launch over a
+ * global leaf worklist (XCD-partitioned, `leaves_per_grab` leaves per atomic
+ * grab, `waves_per_cu` one-wave blocks per CU) followed by marching cubes
+ * over the concatenated tiles of all grids.  Inputs are borrowed and left
+ * unchanged; outputs live in the batch (vr_batch_output).  Modes:
+ *   VR_BATCH_LOOP  the per-grid path (one advect launch + one MC pipeline per grid),
+ *   VR_BATCH_EAGER the batched kernels launched directly,
+ *   VR_BATCH_GRAPH the batched kernels captured once into a hipGraph and replayed. */
+typedef struct vr_batch vr_batch;
+enum { VR_BATCH_LOOP = 0, VR_BATCH_EAGER = 1, VR_BATCH_GRAPH = 2 };
+int vr_batch_create(vr_ctx* ctx, vr_grid* const* grids, const vr_velocity* const* vels, int count, vr_batch** out);
+int vr_batch_set_tuning(vr_batch* b, int leaves_per_grab, int waves_per_cu);
+/* Per-grid time steps (count doubles); NULL returns to the dt of vr_batch_step. */
+int vr_batch_set_dt(vr_ctx* ctx, vr_batch* b, const double* dts);
+/* Times the batched advect launch for leaves-per-grab in {1,2,4,8,16} and
+ * waves-per-CU in {2,4,8,16,32} and keeps the fastest pair. */
+int vr_batch_autotune(vr_ctx* ctx, vr_batch* b, double dt, int scheme, int* leaves_per_grab, int* waves_per_cu);
+int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int scheme, float iso, int mode);
+/* Mesh of grid i from the last step in `mode` (LOOP keeps its own meshes). */
+int vr_batch_mesh_counts(vr_ctx* ctx, vr_batch* b, int i, int mode, uint64_t* vertex_count, uint64_t* tri_count);
+int vr_batch_mesh_copy(vr_ctx* ctx, vr_batch* b, int i, int mode, double* xyz, size_t vcap, uint32_t* tri,
+                       size_t tcap);
+int vr_batch_output(vr_batch* b, int i, const vr_grid** out);
+int vr_batch_destroy(vr_ctx* ctx, vr_batch* b);
+
 /* ---- measurement -------------------------------------------------------- */
-/* Kernel classes for vr_ctx_timing. */
-enum { VR_TIMING_ADVECT = 0, VR_TIMING_CROSSINGS = 1, VR_TIMING_REBUILD = 2, VR_TIMING_COMPACT = 3, VR_TIMING_MC = 4 };
+/* Kernel classes for vr_ctx_timing (GRAPH: whole batch graph launches). */
+enum {
+  VR_TIMING_ADVECT = 0,
+  VR_TIMING_CROSSINGS = 1,
+  VR_TIMING_REBUILD = 2,
+  VR_TIMING_COMPACT = 3,
+  VR_TIMING_MC = 4,
+  VR_TIMING_GRAPH = 5
+};
 /* Turns per-launch HIP-event timing on the context stream on or off and
  * clears the totals. */
 int vr_ctx_set_timing(vr_ctx* ctx, int enable);

@@ -76,6 +76,19 @@ _SIGS = {
     "vr_mesh_destroy": (ctypes.c_int, [_P, _P]),
     "vr_mesh_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "vr_grid_clone": (ctypes.c_int, [_P, _P, ctypes.POINTER(_P)]),
+    "vr_batch_create": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(_P)]),
+    "vr_batch_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    "vr_batch_set_dt": (ctypes.c_int, [_P, _P, _P]),
+    "vr_batch_autotune": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]),
+    "vr_batch_step": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_int]),
+    "vr_batch_mesh_counts": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                            ctypes.POINTER(ctypes.c_uint64)]),
+    "vr_batch_mesh_copy": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_size_t, _P,
+                                          ctypes.c_size_t]),
+    "vr_batch_output": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(_P)]),
+    "vr_batch_destroy": (ctypes.c_int, [_P, _P]),
     "vr_adjoint_rollout": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           _P, _P, _P]),
     "vr_rollout": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -291,6 +304,76 @@ class Grid:
     def close(self):
         if self.h:
             self.ctx.lib.vr_grid_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BATCH_LOOP = 0
+BATCH_EAGER = 1
+BATCH_GRAPH = 2
+TIMING_GRAPH = 5
+
+
+class Batch:
+    """Many level sets advected and meshed in one step (vr_batch)."""
+
+    def __init__(self, ctx, grids, vels):
+        self.ctx, self._keep = ctx, (list(grids), list(vels))
+        gh = (ctypes.c_void_p * len(grids))(*[g.h.value if hasattr(g.h, "value") else g.h for g in grids])
+        vh = (ctypes.c_void_p * len(vels))(*[v.h.value if hasattr(v.h, "value") else v.h for v in vels])
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.vr_batch_create(ctx.h, gh, vh, len(grids), ctypes.byref(h)))
+        self.h = h
+        self.n = len(grids)
+
+    def set_dt(self, dts):
+        """Per-grid time steps (None: use the dt passed to step)."""
+        if dts is None:
+            self.ctx.check(self.ctx.lib.vr_batch_set_dt(self.ctx.h, self.h, None))
+            return
+        a = np.ascontiguousarray(dts, dtype=np.float64)
+        assert a.size == self.n
+        self.ctx.check(self.ctx.lib.vr_batch_set_dt(self.ctx.h, self.h, _ptr(a)))
+
+    def set_tuning(self, leaves_per_grab, waves_per_cu):
+        self.ctx.check(self.ctx.lib.vr_batch_set_tuning(self.h, leaves_per_grab, waves_per_cu))
+
+    def autotune(self, dt, scheme):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(self.ctx.lib.vr_batch_autotune(self.ctx.h, self.h, dt, scheme, ctypes.byref(a),
+                                                      ctypes.byref(b)))
+        return a.value, b.value
+
+    def step(self, dt, t, scheme, iso=0.0, mode=BATCH_GRAPH):
+        self.ctx.check(self.ctx.lib.vr_batch_step(self.ctx.h, self.h, dt, t, scheme, iso, mode))
+
+    def mesh(self, i, mode=BATCH_GRAPH):
+        nv, nt = ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.vr_batch_mesh_counts(self.ctx.h, self.h, i, mode, ctypes.byref(nv),
+                                                         ctypes.byref(nt)))
+        v = np.empty((max(nv.value, 1), 3), dtype=np.float64)
+        t = np.empty((max(nt.value, 1), 3), dtype=np.uint32)
+        self.ctx.check(self.ctx.lib.vr_batch_mesh_copy(self.ctx.h, self.h, i, mode, _ptr(v), nv.value, _ptr(t),
+                                                       nt.value))
+        return v[: nv.value], t[: nt.value]
+
+    def output_nano(self, i):
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.vr_batch_output(self.h, i, ctypes.byref(h)))
+        g = Grid(self.ctx, h)
+        try:
+            return g.download()
+        finally:
+            g.h = None  # borrowed
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.vr_batch_destroy(self.ctx.h, self.h)
             self.h = None
 
     def __del__(self):

@@ -252,13 +252,13 @@ __global__ __launch_bounds__(64) void k_adj_terminal(GridView g, GridView target
     double part = 0.0;
     for (int x = 0; x < 8; ++x) {
       const int off = x * 64 + lane;
-      float d = 0.0f;
+      double d = 0.0;
       if ((L.mask[x] >> lane) & 1ull) {
         const float t = grid_get(target, L.origin[0] + x, L.origin[1] + y, L.origin[2] + z);
-        d = L.values[off] - t;
-        part += 0.5 * (double)d * (double)d;
+        d = (double)L.values[off] - (double)t;
+        part += 0.5 * d * d;
       }
-      lam[(size_t)li * 512 + off] = d;
+      lam[(size_t)li * 512 + off] = (float)d;
     }
     for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
     if (lane == 0) leaf_loss[li] = part;

@@ -1,0 +1,79 @@
+// Per-voxel and per-leaf advection device code shared by the single-grid
+// kernel (vr_kernels.hip) and the batched persistent kernel (vr_batch.hip).
+#pragma once
+
+#include "vr_device.h"
+
+// New value of the active voxel (ox + x, oy + y, oz + z): back-trace, then
+// trilinear sampling of the input (src/tools/advection.rs:39-53).
+// Departure-cell corners inside the LDS brick (leaf plus one-voxel halo,
+// 10^3) come from LDS; any other corner goes through the root/internal
+// lookup, so any dt is handled.
+template <int VK, int SCHEME>
+__device__ __forceinline__ float advect_voxel(const GridView& in, const VelView& vel, const float* brick, int ox,
+                                              int oy, int oz, int x, int y, int z, double dt, const double tf[3]) {
+  double w[3], p[3];
+  index_to_world(in.mat, ox + x, oy + y, oz + z, w);
+  departure<VK, SCHEME>(vel, w, dt, tf, p);
+  const double ix = p[0] * in.inv_vs, iy = p[1] * in.inv_vs, iz = p[2] * in.inv_vs;
+  const double flx = floor(ix), fly = floor(iy), flz = floor(iz);
+  const float fx = (float)(ix - flx), fy = (float)(iy - fly), fz = (float)(iz - flz);
+  const int i = f64_to_i32(flx), j = f64_to_i32(fly), k = f64_to_i32(flz);
+  const int bi = i - ox, bj = j - oy, bk = k - oz;
+  float c[8];
+  if (bi >= -1 && bi <= 7 && bj >= -1 && bj <= 7 && bk >= -1 && bk <= 7) {
+    const int b0 = (bi + 1) * 100 + (bj + 1) * 10 + (bk + 1);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = brick[b0 + (q & 1) * 100 + ((q >> 1) & 1) * 10 + ((q >> 2) & 1)];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = grid_get(in, i + (q & 1), j + ((q >> 1) & 1), k + ((q >> 2) & 1));
+  }
+  return trilinear_blend(c, fx, fy, fz);
+}
+
+// LDS of one advecting wave.
+struct AdvectLDS {
+  float brick[1000];
+  float res[512];
+  uint16_t list[512];
+  int nb[VR_NBR];
+};
+
+// Advects leaf li of `in` into out[li] with one wave (64 threads, the whole
+// block): the active voxels are listed by a ballot/popcount prefix over the
+// 8 mask words and computed densely (about 3 passes of 64 for a band leaf
+// instead of 8 slabs), then the leaf is written back slab by slab, coalesced,
+// with the background in the inactive positions.  Output origin and mask are
+// the input's (the topology does not change, src/tools/advection.rs:34-53).
+template <int VK, int SCHEME>
+__device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
+                                            AdvectLDS& s, int lane, double dt, const double tf[3]) {
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const NanoLeaf& L = in.leaves[li];
+  if (lane < VR_NBR) s.nb[lane] = in.nbr[(size_t)li * VR_NBR + lane];
+  uint64_t m[8];
+  int total = 0;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    m[x] = L.mask[x];
+    if ((m[x] >> lane) & 1ull) s.list[total + __popcll(m[x] & below)] = (uint16_t)(x * 64 + lane);
+    total += __popcll(m[x]);
+  }
+  __syncthreads();
+  stage_brick_dev(in, s.nb, s.brick, lane);
+  __syncthreads();
+  const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
+  for (int i = lane; i < total; i += 64) {
+    const int off = s.list[i];
+    s.res[off] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt, tf);
+  }
+  __syncthreads();
+  NanoLeaf& O = out[li];
+  if (lane < 3) O.origin[lane] = L.origin[lane];
+  if (lane == 3) O.pad = 0;
+  if (lane < 8) O.mask[lane] = m[lane];
+#pragma unroll
+  for (int x = 0; x < 8; ++x) O.values[x * 64 + lane] = ((m[x] >> lane) & 1ull) ? s.res[x * 64 + lane] : in.background;
+  __syncthreads();
+}

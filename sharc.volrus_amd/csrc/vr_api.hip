@@ -180,6 +180,7 @@ GridView make_view(const vr_grid* g) {
   v.nbr = g->topo->d_nbr;
   v.root_count = (int)g->topo->roots.size();
   v.background = g->background;
+  v.vs = g->voxel_size;
   v.inv_vs = 1.0 / g->voxel_size;
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 4; ++c) v.mat[4 * r + c] = g->mat[r][c];
@@ -736,94 +737,6 @@ extern "C" int vr_marching_cubes(vr_ctx* ctx, const vr_grid* g, float iso, vr_me
   int rc = mc_run(ctx, g, iso, &s, m.get());
   if (rc != VR_OK) return rc;
   *out = m.release();
-  return VR_OK;
-}
-
-// Runs the tile build, the count pass, the scans and the emit pass into
-// mesh-owned buffers (allocated to the exact sizes).
-int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* s, vr_mesh* m) {
-  hipError_t e;
-  const int n = g->topo->count;
-  const GridView v = make_view(g);
-  const size_t n7 = (size_t)n * 7;
-  DevBuf& keep = ctx->scratch[SCR_MC_KEEP];
-  DevBuf& pos = ctx->scratch[SCR_MC_POS];
-  DevBuf& sums = ctx->scratch[SCR_SUMS];
-  DevBuf& tot = ctx->scratch[SCR_TOTAL];
-  if ((e = keep.ensure(sizeof(int) * (n7 + 1))) != hipSuccess || (e = pos.ensure(sizeof(int) * (n7 + 1))) != hipSuccess ||
-      (e = sums.ensure(sizeof(int) * (vr_scan_scratch_ints(n7) + 1))) != hipSuccess ||
-      (e = tot.ensure(sizeof(int) * 4)) != hipSuccess)
-    return vr_hip_fail(ctx, e, "scratch");
-  int* ntiles_d = tot.as<int>();
-  if ((e = vr_launch_mc_keep(v, n, keep.as<int>(), ctx->stream)) != hipSuccess ||
-      (e = vr_exclusive_scan(keep.as<int>(), pos.as<int>(), n7, sums.as<int>(), ntiles_d, ctx->stream)) != hipSuccess)
-    return vr_hip_fail(ctx, e, "mc tiles");
-  int nt = 0;
-  if ((e = hipMemcpyAsync(&nt, ntiles_d, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
-      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-    return vr_hip_fail(ctx, e, "readback");
-  DevBuf& tiles = ctx->scratch[SCR_MC_TILES];
-  DevBuf& tleaf = ctx->scratch[SCR_MC_TLEAF];
-  DevBuf& ttile = ctx->scratch[SCR_MC_TTILE];
-  DevBuf& meta = ctx->scratch[SCR_MC_META];
-  DevBuf& tv = ctx->scratch[SCR_MC_TV];
-  DevBuf& tt = ctx->scratch[SCR_MC_TT];
-  DevBuf& vb = ctx->scratch[SCR_MC_VB];
-  DevBuf& tb = ctx->scratch[SCR_MC_TB];
-  DevBuf& sums2 = ctx->scratch[SCR_SUMS2];
-  if ((e = tiles.ensure(sizeof(int4) * (size_t)(nt + 1))) != hipSuccess ||
-      (e = tleaf.ensure(sizeof(int) * VR_NBR * (size_t)(nt + 1))) != hipSuccess ||
-      (e = ttile.ensure(sizeof(int) * VR_NBR * (size_t)(nt + 1))) != hipSuccess ||
-      (e = meta.ensure(sizeof(uint32_t) * 512 * (size_t)(nt + 1))) != hipSuccess ||
-      (e = tv.ensure(sizeof(int) * (size_t)(nt + 1))) != hipSuccess ||
-      (e = tt.ensure(sizeof(int) * (size_t)(nt + 1))) != hipSuccess ||
-      (e = vb.ensure(sizeof(int) * (size_t)(nt + 1))) != hipSuccess ||
-      (e = tb.ensure(sizeof(int) * (size_t)(nt + 1))) != hipSuccess ||
-      (e = sums2.ensure(sizeof(int) * (vr_scan_scratch_ints(nt) + 1))) != hipSuccess)
-    return vr_hip_fail(ctx, e, "scratch");
-  int* totals = tot.as<int>();  // [0] tiles, [1] vertices, [2] triangles
-  if ((e = vr_launch_mc_scatter(v, n, keep.as<int>(), pos.as<int>(), totals, tiles.as<int4>(), ctx->stream)) != hipSuccess ||
-      (e = vr_launch_mc_tile_nbr(v, tiles.as<int4>(), totals, nt, tleaf.as<int>(), ttile.as<int>(), ctx->stream)) !=
-          hipSuccess)
-    return vr_hip_fail(ctx, e, "mc tiles");
-  {
-    TimedLaunch tl(ctx, TC_MC, (double)VR_LEAF_BYTES * n);
-    e = vr_launch_mc_count(v, tiles.as<int4>(), totals, nt, tleaf.as<int>(), ttile.as<int>(), iso, meta.as<uint32_t>(),
-                           tv.as<int>(), tt.as<int>(), ctx->stream);
-  }
-  if (e != hipSuccess ||
-      (e = vr_exclusive_scan(tv.as<int>(), vb.as<int>(), nt, sums2.as<int>(), totals + 1, ctx->stream)) != hipSuccess ||
-      (e = vr_exclusive_scan(tt.as<int>(), tb.as<int>(), nt, sums2.as<int>(), totals + 2, ctx->stream)) != hipSuccess)
-    return vr_hip_fail(ctx, e, "mc count");
-  int th[3] = {0, 0, 0};
-  if ((e = hipMemcpyAsync(th, totals, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
-      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-    return vr_hip_fail(ctx, e, "readback");
-  m->nv = (uint64_t)th[1];
-  m->nt = (uint64_t)th[2];
-  if (m->cap_v < m->nv + 1) {
-    if (m->d_v) (void)hipFree(m->d_v);
-    m->d_v = nullptr;
-    m->cap_v = 0;
-    if ((e = hipMalloc(&m->d_v, sizeof(double) * 3 * (size_t)(th[1] + 1))) != hipSuccess)
-      return vr_hip_fail(ctx, e, "hipMalloc mesh");
-    m->cap_v = (uint64_t)th[1] + 1;
-  }
-  if (m->cap_t < m->nt + 1) {
-    if (m->d_t) (void)hipFree(m->d_t);
-    m->d_t = nullptr;
-    m->cap_t = 0;
-    if ((e = hipMalloc(&m->d_t, sizeof(uint32_t) * 3 * (size_t)(th[2] + 1))) != hipSuccess)
-      return vr_hip_fail(ctx, e, "hipMalloc mesh");
-    m->cap_t = (uint64_t)th[2] + 1;
-  }
-  {
-    TimedLaunch tl(ctx, TC_MC, 24.0 * th[1] + 12.0 * th[2]);
-    e = vr_launch_mc_emit(v, tiles.as<int4>(), totals, nt, tleaf.as<int>(), ttile.as<int>(), iso, g->voxel_size,
-                          meta.as<uint32_t>(), vb.as<int>(), tb.as<int>(), m->d_v, m->d_t, ctx->stream);
-  }
-  if (e != hipSuccess) return vr_hip_fail(ctx, e, "mc emit");
-  (void)s;
   return VR_OK;
 }
 

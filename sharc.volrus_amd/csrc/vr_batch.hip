@@ -1,0 +1,428 @@
+// Batched advection + marching cubes over many level sets.
+//
+// No reference counterpart (the reference handles one grid per call).  The
+// leaves of all grids form one global worklist (grid g owns global leaves
+// [leaf_base[g], leaf_base[g+1])).  One persistent launch of
+// `waves_per_cu` one-wave blocks per CU advects them: the worklist is cut
+// into 8 contiguous ranges, one per XCD (blocks are dispatched to the 8 XCDs
+// round-robin by block index, so block b serves range b % 8), and waves grab
+// `leaves_per_grab` consecutive leaves at a time from their range's atomic
+// counter, so neighbouring leaves, whose halos overlap, stay in one XCD's L2.
+// Marching cubes then runs over the concatenated tiles of all grids
+// (vr_mc.hip).  The whole step can be captured once into a hipGraph and
+// replayed; `vr_batch_autotune` picks leaves-per-grab and waves-per-CU by
+// timing the advect launch.
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "vr_advect.h"
+#include "vr_internal.h"
+
+struct vr_batch {
+  int G = 0;
+  std::vector<vr_grid*> in;
+  std::vector<const vr_velocity*> vel;
+  std::vector<std::unique_ptr<vr_grid>> out;  // advected grids (own leaves, share topology)
+  std::vector<int> leaf_base;                 // G + 1
+  long long ltot = 0;
+  int lpg = 4, wpc = 8;                       // leaves per grab, waves per CU
+  int cus = 256;
+  DevBuf d_in, d_out_views, d_vel, d_outs, d_leaf_base, d_counters, d_ranges, d_dts;
+  bool has_dts = false;
+  std::vector<double> dts;
+  MCMulti mc;
+  int tcap = 0;
+  long long vcap = 0, tricap = 0;
+  double* d_verts = nullptr;
+  uint32_t* d_tris = nullptr;
+  std::vector<int> ranges_host;  // 4 per grid after a step
+  bool sized = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double g_dt = 0, g_t = 0;
+  int g_scheme = 0, g_lpg = 0, g_wpc = 0;
+  float g_iso = 0;
+  std::vector<std::unique_ptr<vr_mesh>> loop_mesh;  // per-grid meshes of the loop mode
+  ~vr_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (d_verts) (void)hipFree(d_verts);
+    if (d_tris) (void)hipFree(d_tris);
+  }
+};
+
+// ------------------------------------------------------------------ persistent advect
+template <int SCHEME>
+__global__ __launch_bounds__(64) void k_batch_advect(const GridView* ins, const VelView* vels, NanoLeaf* const* outs,
+                                                     const int* leaf_base, int G, int* counters, const int* ranges,
+                                                     int lpg, const double* dts, double dt, double tf0, double tf1,
+                                                     double tf2) {
+  __shared__ AdvectLDS s;
+  __shared__ int s_item;
+  const int lane = threadIdx.x;
+  const int xcd = blockIdx.x & 7;
+  const int lo = ranges[xcd], hi = ranges[xcd + 1];
+  const double tf[3] = {tf0, tf1, tf2};
+  for (;;) {
+    if (lane == 0) s_item = atomicAdd(&counters[xcd], lpg);
+    __syncthreads();
+    const int start = lo + s_item;
+    __syncthreads();
+    if (start >= hi) break;
+    const int end = start + lpg < hi ? start + lpg : hi;
+    for (int gl = start; gl < end; ++gl) {
+      int a = 0, b = G;  // grid of global leaf gl
+      while (b - a > 1) {
+        const int mid = (a + b) >> 1;
+        if (leaf_base[mid] <= gl) a = mid; else b = mid;
+      }
+      const int li = gl - leaf_base[a];
+      const GridView& in = ins[a];
+      const VelView& v = vels[a];
+      const double gdt = dts ? dts[a] : dt;
+      switch (v.kind) {
+        case VK_UNIFORM: advect_leaf<VK_UNIFORM, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        case VK_RIGID: advect_leaf<VK_RIGID, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        case VK_VORTEX: advect_leaf<VK_VORTEX, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        case VK_DENSE: advect_leaf<VK_DENSE, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        default: advect_leaf<VK_GRIDS3, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
+      }
+    }
+  }
+}
+
+__global__ void k_batch_reset(int* counters) {
+  if (threadIdx.x < 8) counters[threadIdx.x] = 0;
+}
+
+// ------------------------------------------------------------------ API
+extern "C" int vr_batch_create(vr_ctx* ctx, vr_grid* const* grids, const vr_velocity* const* vels, int count,
+                               vr_batch** out) {
+  if (!ctx || !grids || !vels || count <= 0 || !out) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "bad batch");
+  *out = nullptr;
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
+  std::unique_ptr<vr_batch> b(new vr_batch());
+  b->G = count;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+    b->cus = prop.multiProcessorCount;
+  b->leaf_base.assign((size_t)count + 1, 0);
+  std::vector<GridView> vin, vout;
+  std::vector<VelView> vv;
+  std::vector<NanoLeaf*> outs;
+  for (int i = 0; i < count; ++i) {
+    vr_grid* g = grids[i];
+    const vr_velocity* v = vels[i];
+    if (!g || !v) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "null grid or velocity %d", i);
+    if (g->topo->has_empty) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "grid %d has leaves without active voxels", i);
+    b->in.push_back(g);
+    b->vel.push_back(v);
+    std::unique_ptr<vr_grid> o(new vr_grid());
+    copy_header_into(o.get(), g);
+    o->topo = g->topo;
+    o->cap = g->topo->count;
+    if ((e = hipMalloc(&o->d_leaves, sizeof(NanoLeaf) * (size_t)(o->cap ? o->cap : 1))) != hipSuccess)
+      return vr_hip_fail(ctx, e, "hipMalloc");
+    vin.push_back(make_view(g));
+    vout.push_back(make_view(o.get()));
+    vv.push_back(make_vel_view(v));
+    outs.push_back(o->d_leaves);
+    b->out.push_back(std::move(o));
+    b->leaf_base[i + 1] = b->leaf_base[i] + g->topo->count;
+  }
+  b->ltot = b->leaf_base[count];
+  if (b->ltot > 0x7fffffff / 7) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "batch too large");
+  std::vector<int> ranges(9);
+  for (int x = 0; x <= 8; ++x) ranges[x] = (int)((b->ltot * x) / 8);
+  if ((e = b->d_in.ensure(sizeof(GridView) * count)) != hipSuccess ||
+      (e = b->d_out_views.ensure(sizeof(GridView) * count)) != hipSuccess ||
+      (e = b->d_vel.ensure(sizeof(VelView) * count)) != hipSuccess ||
+      (e = b->d_outs.ensure(sizeof(NanoLeaf*) * count)) != hipSuccess ||
+      (e = b->d_leaf_base.ensure(sizeof(int) * (count + 1))) != hipSuccess ||
+      (e = b->d_counters.ensure(sizeof(int) * 8)) != hipSuccess || (e = b->d_ranges.ensure(sizeof(int) * 9)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMalloc");
+  if ((e = hipMemcpyAsync(b->d_in.p, vin.data(), sizeof(GridView) * count, hipMemcpyHostToDevice, ctx->stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(b->d_out_views.p, vout.data(), sizeof(GridView) * count, hipMemcpyHostToDevice,
+                          ctx->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->d_vel.p, vv.data(), sizeof(VelView) * count, hipMemcpyHostToDevice, ctx->stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(b->d_outs.p, outs.data(), sizeof(NanoLeaf*) * count, hipMemcpyHostToDevice, ctx->stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(b->d_leaf_base.p, b->leaf_base.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice,
+                          ctx->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->d_ranges.p, ranges.data(), sizeof(int) * 9, hipMemcpyHostToDevice, ctx->stream)) !=
+          hipSuccess ||
+      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMemcpy");
+  *out = b.release();
+  return VR_OK;
+}
+
+extern "C" int vr_batch_set_dt(vr_ctx* ctx, vr_batch* b, const double* dts) {
+  if (!ctx || !b) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "null argument");
+  if (!dts) {
+    b->has_dts = false;
+    return VR_OK;
+  }
+  hipError_t e;
+  b->dts.assign(dts, dts + b->G);
+  if ((e = b->d_dts.ensure(sizeof(double) * b->G)) != hipSuccess ||
+      (e = hipMemcpyAsync(b->d_dts.p, dts, sizeof(double) * b->G, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMemcpy");
+  b->has_dts = true;
+  if (b->exec) {  // captured graphs hold the old choice
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  return VR_OK;
+}
+
+extern "C" int vr_batch_set_tuning(vr_batch* b, int leaves_per_grab, int waves_per_cu) {
+  if (!b || leaves_per_grab < 1 || waves_per_cu < 1 || waves_per_cu > 64) return VR_ERR_INVALID_ARGUMENT;
+  b->lpg = leaves_per_grab;
+  b->wpc = waves_per_cu;
+  return VR_OK;
+}
+
+static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int scheme) {
+  double tf[3];
+  // one field per grid; time factors from the first field's period (uniform fields use none)
+  vr_stage_factors(b->vel[0]->period, t, dt, scheme, tf);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, ctx->stream, b->d_counters.as<int>());
+  const int blocks = b->cus * b->wpc;
+  double bytes = 2.0 * VR_LEAF_BYTES * (double)b->ltot;
+  {
+    TimedLaunch tl(ctx, TC_ADVECT, bytes);
+    const double* dts = b->has_dts ? b->d_dts.as<double>() : nullptr;
+    if (scheme == VR_SCHEME_RK3)
+      hipLaunchKernelGGL(k_batch_advect<3>, dim3(blocks), dim3(64), 0, ctx->stream, b->d_in.as<GridView>(),
+                         b->d_vel.as<VelView>(), b->d_outs.as<NanoLeaf*>(), b->d_leaf_base.as<int>(), b->G,
+                         b->d_counters.as<int>(), b->d_ranges.as<int>(), b->lpg, dts, dt, tf[0], tf[1], tf[2]);
+    else
+      hipLaunchKernelGGL(k_batch_advect<1>, dim3(blocks), dim3(64), 0, ctx->stream, b->d_in.as<GridView>(),
+                         b->d_vel.as<VelView>(), b->d_outs.as<NanoLeaf*>(), b->d_leaf_base.as<int>(), b->G,
+                         b->d_counters.as<int>(), b->d_ranges.as<int>(), b->lpg, dts, dt, tf[0], tf[1], tf[2]);
+  }
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "batch advect");
+}
+
+// Batched MC over the advected grids into the batch's mesh buffers (sizes
+// fixed by the first, host-synchronised run).
+static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
+  int rc = mc_tiles_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->d_leaf_base.as<int>(), b->G, b->ltot);
+  if (rc != VR_OK) return rc;
+  hipError_t e;
+  if (sizing) {
+    int nt = 0;
+    if ((e = hipMemcpyAsync(&nt, b->mc.totals.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+      return vr_hip_fail(ctx, e, "readback");
+    b->tcap = nt;
+  }
+  rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->d_leaf_base.as<int>(), b->G, b->ltot, b->tcap,
+                      iso, (double)VR_LEAF_BYTES * b->ltot);
+  if (rc != VR_OK) return rc;
+  if (sizing) {
+    int th[3];
+    if ((e = hipMemcpyAsync(th, b->mc.totals.p, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+      return vr_hip_fail(ctx, e, "readback");
+    if (th[1] + 1 > b->vcap) {
+      if (b->d_verts) (void)hipFree(b->d_verts);
+      b->d_verts = nullptr;
+      if ((e = hipMalloc(&b->d_verts, sizeof(double) * 3 * (size_t)(th[1] + 1))) != hipSuccess)
+        return vr_hip_fail(ctx, e, "hipMalloc");
+      b->vcap = th[1] + 1;
+    }
+    if (th[2] + 1 > b->tricap) {
+      if (b->d_tris) (void)hipFree(b->d_tris);
+      b->d_tris = nullptr;
+      if ((e = hipMalloc(&b->d_tris, sizeof(uint32_t) * 3 * (size_t)(th[2] + 1))) != hipSuccess)
+        return vr_hip_fail(ctx, e, "hipMalloc");
+      b->tricap = th[2] + 1;
+    }
+  }
+  return mc_emit_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->tcap, iso, b->d_verts, b->d_tris, 0.0);
+}
+
+static int batch_read_ranges(vr_ctx* ctx, vr_batch* b) {
+  b->ranges_host.assign((size_t)4 * b->G, 0);
+  hipError_t e;
+  if ((e = hipMemcpyAsync(b->ranges_host.data(), b->mc.ranges.p, sizeof(int) * 4 * b->G, hipMemcpyDeviceToHost,
+                          ctx->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "readback");
+  return VR_OK;
+}
+
+extern "C" int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int scheme, float iso, int mode) {
+  if (!ctx || !b) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "null argument");
+  if (scheme != VR_SCHEME_RK1 && scheme != VR_SCHEME_RK3)
+    return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "scheme must be 1 or 3");
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
+  if (mode == VR_BATCH_LOOP) {
+    // The per-grid path: one advect launch and one marching-cubes pipeline per grid.
+    if (b->loop_mesh.size() != (size_t)b->G) {
+      b->loop_mesh.clear();
+      for (int i = 0; i < b->G; ++i) b->loop_mesh.emplace_back(new vr_mesh());
+    }
+    for (int i = 0; i < b->G; ++i) {
+      int rc = advect_into(ctx, b->in[i], b->vel[i], b->has_dts ? b->dts[i] : dt, t, scheme, b->out[i].get());
+      if (rc != VR_OK) return rc;
+      if (b->out[i]->topo->count > 0) {
+        MCScratch sc;
+        rc = mc_run(ctx, b->out[i].get(), iso, &sc, b->loop_mesh[i].get());
+        if (rc != VR_OK) return rc;
+      }
+    }
+    return (e = hipStreamSynchronize(ctx->stream)) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "sync");
+  }
+  if (!b->sized || mode == VR_BATCH_EAGER) {
+    int rc = batch_advect(ctx, b, dt, t, scheme);
+    if (rc == VR_OK) rc = batch_mc(ctx, b, iso, !b->sized);
+    if (rc == VR_OK) rc = batch_read_ranges(ctx, b);
+    if (rc != VR_OK) return rc;
+    b->sized = true;
+    if (mode == VR_BATCH_EAGER) return VR_OK;
+  }
+  // VR_BATCH_GRAPH: capture once per parameter set, then replay.
+  if (!b->exec || b->g_dt != dt || b->g_t != t || b->g_scheme != scheme || b->g_iso != iso || b->g_lpg != b->lpg ||
+      b->g_wpc != b->wpc) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    const bool timing = ctx->timing;
+    ctx->timing = false;  // no event records inside the captured graph
+    if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+      ctx->timing = timing;
+      return vr_hip_fail(ctx, e, "hipStreamBeginCapture");
+    }
+    int rc = batch_advect(ctx, b, dt, t, scheme);
+    if (rc == VR_OK) rc = batch_mc(ctx, b, iso, false);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->timing = timing;
+    if (rc != VR_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return vr_hip_fail(ctx, e, "hipStreamEndCapture");
+    b->graph = g;
+    if ((e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0)) != hipSuccess)
+      return vr_hip_fail(ctx, e, "hipGraphInstantiate");
+    b->g_dt = dt;
+    b->g_t = t;
+    b->g_scheme = scheme;
+    b->g_iso = iso;
+    b->g_lpg = b->lpg;
+    b->g_wpc = b->wpc;
+  }
+  {
+    TimedLaunch tl(ctx, TC_GRAPH, 2.0 * VR_LEAF_BYTES * (double)b->ltot);
+    if ((e = hipGraphLaunch(b->exec, ctx->stream)) != hipSuccess) return vr_hip_fail(ctx, e, "hipGraphLaunch");
+  }
+  return VR_OK;
+}
+
+extern "C" int vr_batch_autotune(vr_ctx* ctx, vr_batch* b, double dt, int scheme, int* best_lpg, int* best_wpc) {
+  if (!ctx || !b) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "null argument");
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
+  hipEvent_t a, z;
+  if ((e = hipEventCreate(&a)) != hipSuccess || (e = hipEventCreate(&z)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "hipEventCreate");
+  const int lpgs[] = {1, 2, 4, 8, 16};
+  const int wpcs[] = {2, 4, 8, 16, 32};
+  float best = 1e30f;
+  int bl = b->lpg, bw = b->wpc;
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  int rc = VR_OK;
+  for (int lp : lpgs)
+    for (int wp : wpcs) {
+      b->lpg = lp;
+      b->wpc = wp;
+      rc = batch_advect(ctx, b, dt, 0.0, scheme);  // warm
+      (void)hipEventRecord(a, ctx->stream);
+      for (int r = 0; r < 3 && rc == VR_OK; ++r) rc = batch_advect(ctx, b, dt, 0.0, scheme);
+      (void)hipEventRecord(z, ctx->stream);
+      if (rc != VR_OK || (e = hipEventSynchronize(z)) != hipSuccess) break;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, a, z);
+      if (ms < best) {
+        best = ms;
+        bl = lp;
+        bw = wp;
+      }
+    }
+  ctx->timing = timing;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(z);
+  b->lpg = bl;
+  b->wpc = bw;
+  if (best_lpg) *best_lpg = bl;
+  if (best_wpc) *best_wpc = bw;
+  return rc;
+}
+
+extern "C" int vr_batch_mesh_counts(vr_ctx* ctx, vr_batch* b, int i, int mode, uint64_t* nv, uint64_t* nt) {
+  if (!b || i < 0 || i >= b->G) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "bad grid index");
+  if (mode == VR_BATCH_LOOP) {
+    if (b->loop_mesh.size() != (size_t)b->G) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "no loop step yet");
+    if (nv) *nv = b->loop_mesh[i]->nv;
+    if (nt) *nt = b->loop_mesh[i]->nt;
+    return VR_OK;
+  }
+  if (!b->sized) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "no batched step yet");
+  if (mode == VR_BATCH_GRAPH) {
+    int rc = batch_read_ranges(ctx, b);
+    if (rc != VR_OK) return rc;
+  }
+  if (nv) *nv = (uint64_t)b->ranges_host[4 * i + 1];
+  if (nt) *nt = (uint64_t)b->ranges_host[4 * i + 3];
+  return VR_OK;
+}
+
+extern "C" int vr_batch_mesh_copy(vr_ctx* ctx, vr_batch* b, int i, int mode, double* xyz, size_t vcap, uint32_t* tri,
+                                  size_t tcap) {
+  if (!ctx || !b || i < 0 || i >= b->G) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "bad grid index");
+  if (mode == VR_BATCH_LOOP) {
+    if (b->loop_mesh.size() != (size_t)b->G) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "no loop step yet");
+    return vr_mesh_copy(ctx, b->loop_mesh[i].get(), xyz, vcap, tri, tcap);
+  }
+  uint64_t nv, nt;
+  int rc = vr_batch_mesh_counts(ctx, b, i, mode, &nv, &nt);
+  if (rc != VR_OK) return rc;
+  if ((xyz && vcap < nv) || (tri && tcap < nt)) return vr_fail(ctx, VR_ERR_CAPACITY, "mesh buffers too small");
+  hipError_t e;
+  const size_t v0 = (size_t)b->ranges_host[4 * i], t0 = (size_t)b->ranges_host[4 * i + 2];
+  if (xyz && nv &&
+      (e = hipMemcpyAsync(xyz, b->d_verts + 3 * v0, sizeof(double) * 3 * nv, hipMemcpyDeviceToHost, ctx->stream)) !=
+          hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMemcpyAsync");
+  if (tri && nt &&
+      (e = hipMemcpyAsync(tri, b->d_tris + 3 * t0, sizeof(uint32_t) * 3 * nt, hipMemcpyDeviceToHost, ctx->stream)) !=
+          hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMemcpyAsync");
+  return (e = hipStreamSynchronize(ctx->stream)) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "sync");
+}
+
+extern "C" int vr_batch_output(vr_batch* b, int i, const vr_grid** out) {
+  if (!b || !out || i < 0 || i >= b->G) return VR_ERR_INVALID_ARGUMENT;
+  *out = b->out[i].get();
+  return VR_OK;
+}
+
+extern "C" int vr_batch_destroy(vr_ctx* ctx, vr_batch* b) {
+  if (!b) return VR_ERR_INVALID_ARGUMENT;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  delete b;
+  return VR_OK;
+}

@@ -40,7 +40,8 @@ struct GridView {
   const int* nbr;
   int root_count;
   float background;
-  double inv_vs;    // 1.0 / voxel_size of the uniform transform (grid.rs:114-121)
+  double vs;        // voxel_size of the uniform transform
+  double inv_vs;    // 1.0 / voxel_size (grid.rs:114-121)
   double mat[12];   // forward affine rows 0..2 (grid.rs:248-257)
 };
 

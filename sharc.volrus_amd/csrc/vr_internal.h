@@ -50,8 +50,13 @@ enum ScratchSlot {
   SCR_COUNT
 };
 
+// Device scratch of the marching-cubes passes (reused across calls).
+struct MCMulti {
+  DevBuf views, leaf_base, keep, pos, sums, sums2, totals, tile_base, tiles, tleaf, ttile, meta, tv, tt, vb, tb, ranges;
+};
+
 // Kernel classes timed with HIP events when timing is on.
-enum TimingClass { TC_ADVECT = 0, TC_CROSSINGS, TC_REBUILD, TC_COMPACT, TC_MC, TC_COUNT };
+enum TimingClass { TC_ADVECT = 0, TC_CROSSINGS, TC_REBUILD, TC_COMPACT, TC_MC, TC_GRAPH, TC_COUNT };
 
 struct TimingRec {
   hipEvent_t a, b;
@@ -64,12 +69,13 @@ struct vr_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   DevBuf scratch[SCR_COUNT];
+  MCMulti mc;
   bool timing = false;
   std::vector<TimingRec> pending;
   std::vector<hipEvent_t> event_pool;
-  double t_ms[TC_COUNT] = {0, 0, 0, 0, 0};
-  double t_bytes[TC_COUNT] = {0, 0, 0, 0, 0};
-  uint64_t t_n[TC_COUNT] = {0, 0, 0, 0, 0};
+  double t_ms[TC_COUNT] = {0, 0, 0, 0, 0, 0};
+  double t_bytes[TC_COUNT] = {0, 0, 0, 0, 0, 0};
+  uint64_t t_n[TC_COUNT] = {0, 0, 0, 0, 0, 0};
   ~vr_ctx();
 };
 
@@ -135,6 +141,11 @@ int advect_into(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel, double d
                 vr_grid* o);
 int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any);
 int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* s, vr_mesh* m);
+int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot);
+int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
+                   int tcap, float iso, double algorithmic_bytes);
+int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, float iso, double* verts, uint32_t* tris,
+                  double algorithmic_bytes);
 int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp);
 void copy_header_into(vr_grid* dst, const vr_grid* src);
 double vr_time_factor(double period, double t);

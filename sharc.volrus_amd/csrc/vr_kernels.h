@@ -28,14 +28,4 @@ hipError_t vr_launch_popcount_words(const uint64_t* w, const int* count, int cap
 hipError_t vr_launch_crossing_coords(const int4* origins, const int* count, int cap, const uint64_t* cross,
                                      const int* pos, int* xyz, hipStream_t s);
 
-hipError_t vr_launch_mc_keep(const GridView& g, int cap, int* keep, hipStream_t s);
-hipError_t vr_launch_mc_scatter(const GridView& g, int cap, const int* keep, const int* pos, const int* ntiles,
-                                int4* tiles, hipStream_t s);
-hipError_t vr_launch_mc_tile_nbr(const GridView& g, const int4* tiles, const int* ntiles, int tcap, int* tleaf,
-                                 int* ttile, hipStream_t s);
-hipError_t vr_launch_mc_count(const GridView& g, const int4* tiles, const int* ntiles, int tcap, const int* tleaf,
-                              const int* ttile, float iso, uint32_t* meta, int* tv, int* tt, hipStream_t s);
-hipError_t vr_launch_mc_emit(const GridView& g, const int4* tiles, const int* ntiles, int tcap, const int* tleaf,
-                             const int* ttile, float iso, double vs, const uint32_t* meta, const int* vbase,
-                             const int* tbase, double* verts, uint32_t* tris, hipStream_t s);
 hipError_t vr_launch_leaf_popcount(const NanoLeaf* leaves, const int* count, int cap, int* out, hipStream_t s);
