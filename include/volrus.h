@@ -73,7 +73,7 @@ int vr_grid_download_nano(vr_ctx* ctx, const vr_grid* g, uint8_t* out, size_t ca
 /* Grid::leaf_count / active_voxel_count (src/grid.rs:505-513). */
 int vr_grid_counts(vr_ctx* ctx, const vr_grid* g, uint64_t* leaf_count, uint64_t* active_count);
 int vr_grid_destroy(vr_ctx* ctx, vr_grid* g);
-/* Device-side deep copy of a grid's leaves (lookup arrays are shared). */
+/* Device-side deep copy of a grid (leaves and lookup arrays). */
 int vr_grid_clone(vr_ctx* ctx, const vr_grid* g, vr_grid** out);
 
 /* ---- synthetic level sets (fixture generation on the device) ----------- */

@@ -29,35 +29,30 @@ struct SplitMix64 {
 
 // ---------------------------------------------------------------- sin/cos of pi*x
 // Deterministic sinpi/cospi: exact reduction to f in [-1/4, 1/4], then the
-// Taylor series of sin(pi f) and cos(pi f) to f^19 / f^20 (truncation below
-// 1.3e-22), evaluated by Horner in plain IEEE double with no fused
-// operations, so any IEEE machine gives the same bits.
+// Taylor series of sin(pi f) and cos(pi f) to f^15 / f^14 (truncation below
+// 4e-16 and 1e-15), Horner with explicit fused multiply-adds (one IEEE
+// rounding each, std::fma), so any IEEE machine gives the same bits.
 inline void sincospi_det(double x, double* s_out, double* c_out) {
   const double n = std::nearbyint(2.0 * x);
   const double f = x - 0.5 * n;
   const double f2 = f * f;
-  double s = -2.2948428997269873e-08;
-  s = s * f2 + 7.952054001475513e-07;
-  s = s * f2 + -2.1915353447830217e-05;
-  s = s * f2 + 0.00046630280576761255;
-  s = s * f2 + -0.0073704309457143504;
-  s = s * f2 + 0.08214588661112823;
-  s = s * f2 + -0.5992645293207921;
-  s = s * f2 + 2.5501640398773455;
-  s = s * f2 + -5.16771278004997;
-  s = s * f2 + 3.141592653589793;
+  double s = -2.1915353447830217e-05;
+  s = std::fma(s, f2, 0.00046630280576761255);
+  s = std::fma(s, f2, -0.0073704309457143504);
+  s = std::fma(s, f2, 0.08214588661112823);
+  s = std::fma(s, f2, -0.5992645293207921);
+  s = std::fma(s, f2, 2.5501640398773455);
+  s = std::fma(s, f2, -5.16771278004997);
+  s = std::fma(s, f2, 3.141592653589793);
   s = s * f;
-  double c = 3.604730797462501e-09;
-  c = c * f2 + -1.3878952462213771e-07;
-  c = c * f2 + 4.303069587032947e-06;
-  c = c * f2 + -0.0001046381049248457;
-  c = c * f2 + 0.0019295743094039231;
-  c = c * f2 + -0.02580689139001406;
-  c = c * f2 + 0.2353306303588932;
-  c = c * f2 + -1.3352627688545895;
-  c = c * f2 + 4.0587121264167685;
-  c = c * f2 + -4.934802200544679;
-  c = c * f2 + 1.0;
+  double c = -0.0001046381049248457;
+  c = std::fma(c, f2, 0.0019295743094039231);
+  c = std::fma(c, f2, -0.02580689139001406);
+  c = std::fma(c, f2, 0.2353306303588932);
+  c = std::fma(c, f2, -1.3352627688545895);
+  c = std::fma(c, f2, 4.0587121264167685);
+  c = std::fma(c, f2, -4.934802200544679);
+  c = std::fma(c, f2, 1.0);
   const long q = ((long)n) & 3;
   switch (q) {
     case 0: *s_out = s; *c_out = c; break;

@@ -32,7 +32,7 @@ __device__ __forceinline__ float advect_voxel(const GridView& in, const VelView&
   return trilinear_blend(c, fx, fy, fz);
 }
 
-// LDS of one advecting wave.
+// LDS of one advecting block.
 struct AdvectLDS {
   float brick[1000];
   float res[512];
@@ -40,40 +40,54 @@ struct AdvectLDS {
   int nb[VR_NBR];
 };
 
-// Advects leaf li of `in` into out[li] with one wave (64 threads, the whole
-// block): the active voxels are listed by a ballot/popcount prefix over the
-// 8 mask words and computed densely (about 3 passes of 64 for a band leaf
-// instead of 8 slabs), then the leaf is written back slab by slab, coalesced,
-// with the background in the inactive positions.  Output origin and mask are
-// the input's (the topology does not change, src/tools/advection.rs:34-53).
-template <int VK, int SCHEME>
+// Advects leaf li of `in` into out[li] with NW waves (the whole block,
+// tid in [0, 64 NW)): the active voxels are listed by a ballot/popcount
+// prefix over the 8 mask words and spread densely over the block's lanes
+// (one pass for a band leaf with 4 waves), then the leaf is written back
+// slab by slab, coalesced, with the background in the inactive positions.
+// Output origin and mask are the input's (the topology does not change,
+// src/tools/advection.rs:34-53).
+template <int VK, int SCHEME, int NW>
 __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
-                                            AdvectLDS& s, int lane, double dt, const double tf[3]) {
+                                            AdvectLDS& s, int tid, double dt, const double tf[3]) {
+  constexpr int NT = 64 * NW;
+  const int lane = tid & 63, wave = tid >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const NanoLeaf& L = in.leaves[li];
-  if (lane < VR_NBR) s.nb[lane] = in.nbr[(size_t)li * VR_NBR + lane];
+  if (tid < VR_NBR) s.nb[tid] = in.nbr[(size_t)li * VR_NBR + tid];
   uint64_t m[8];
+  int pre[8];
   int total = 0;
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
     m[x] = L.mask[x];
-    if ((m[x] >> lane) & 1ull) s.list[total + __popcll(m[x] & below)] = (uint16_t)(x * 64 + lane);
+    pre[x] = total;
     total += __popcll(m[x]);
   }
+#pragma unroll
+  for (int x = 0; x < 8; ++x)
+    if ((x % NW) == wave && ((m[x] >> lane) & 1ull)) s.list[pre[x] + __popcll(m[x] & below)] = (uint16_t)(x * 64 + lane);
   __syncthreads();
-  stage_brick_dev(in, s.nb, s.brick, lane);
+  for (int i = tid; i < 1000; i += NT) {
+    const int bx = i / 100 - 1, by = (i / 10) % 10 - 1, bz = i % 10 - 1;
+    const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
+              dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
+    const int src = s.nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+    s.brick[i] = src < 0 ? in.background : in.leaves[src].values[voxel_offset(bx, by, bz)];
+  }
   __syncthreads();
   const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
-  for (int i = lane; i < total; i += 64) {
+  for (int i = tid; i < total; i += NT) {
     const int off = s.list[i];
     s.res[off] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt, tf);
   }
   __syncthreads();
   NanoLeaf& O = out[li];
-  if (lane < 3) O.origin[lane] = L.origin[lane];
-  if (lane == 3) O.pad = 0;
-  if (lane < 8) O.mask[lane] = m[lane];
+  if (tid < 3) O.origin[tid] = L.origin[tid];
+  if (tid == 3) O.pad = 0;
+  if (tid < 8) O.mask[tid] = m[tid];
 #pragma unroll
-  for (int x = 0; x < 8; ++x) O.values[x * 64 + lane] = ((m[x] >> lane) & 1ull) ? s.res[x * 64 + lane] : in.background;
+  for (int x = 0; x < 8; ++x)
+    if ((x % NW) == wave) O.values[x * 64 + lane] = ((m[x] >> lane) & 1ull) ? s.res[x * 64 + lane] : in.background;
   __syncthreads();
 }

@@ -699,9 +699,13 @@ int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp) {
     e = vr_launch_rebuild(make_view(g), g->topo->cap, cross, any, tmp, keep.as<int>(), R, vs, band, ctx->stream);
   }
   if (e != hipSuccess) return vr_hip_fail(ctx, e, "rebuild");
-  int anyh = 0;
-  if ((e = hipMemcpyAsync(&anyh, any, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
-      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+  // Leaves without active voxels survive only when the grid has no crossing
+  // at all (every leaf is then kept as it was); the flag is read back only
+  // when the grid can hold such leaves.
+  int anyh = 1;
+  if (g->topo->has_empty &&
+      ((e = hipMemcpyAsync(&anyh, any, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess))
     return vr_hip_fail(ctx, e, "readback");
   return compact_into(ctx, g, tmp, keep.as<int>(), anyh ? false : g->topo->has_empty);
 }

@@ -82,11 +82,11 @@ __global__ __launch_bounds__(64) void k_batch_advect(const GridView* ins, const 
       const VelView& v = vels[a];
       const double gdt = dts ? dts[a] : dt;
       switch (v.kind) {
-        case VK_UNIFORM: advect_leaf<VK_UNIFORM, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
-        case VK_RIGID: advect_leaf<VK_RIGID, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
-        case VK_VORTEX: advect_leaf<VK_VORTEX, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
-        case VK_DENSE: advect_leaf<VK_DENSE, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
-        default: advect_leaf<VK_GRIDS3, SCHEME>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        case VK_UNIFORM: advect_leaf<VK_UNIFORM, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        case VK_RIGID: advect_leaf<VK_RIGID, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        case VK_VORTEX: advect_leaf<VK_VORTEX, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        case VK_DENSE: advect_leaf<VK_DENSE, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        default: advect_leaf<VK_GRIDS3, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
       }
     }
   }

@@ -46,7 +46,7 @@ struct Topo {
 enum ScratchSlot {
   SCR_POS = 0, SCR_SUMS, SCR_SUMS2, SCR_TOTAL, SCR_KEEP, SCR_CROSS, SCR_ANY, SCR_WORDS, SCR_COORDS,
   SCR_MC_KEEP, SCR_MC_POS, SCR_MC_TILES, SCR_MC_TLEAF, SCR_MC_TTILE, SCR_MC_META, SCR_MC_TV, SCR_MC_TT,
-  SCR_MC_VB, SCR_MC_TB, SCR_NEWCOUNT, SCR_TMP_LEAVES,
+  SCR_MC_VB, SCR_MC_TB, SCR_NEWCOUNT, SCR_TMP_LEAVES, SCR_ACTIVE, SCR_PING,
   SCR_COUNT
 };
 
@@ -147,6 +147,7 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, float iso, double* verts, uint32_t* tris,
                   double algorithmic_bytes);
 int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp);
+int count_active_into(vr_ctx* ctx, const vr_grid* g, unsigned long long* slot);
 void copy_header_into(vr_grid* dst, const vr_grid* src);
 double vr_time_factor(double period, double t);
 void vr_stage_factors(double period, double t, double dt, int scheme, double tf[3]);

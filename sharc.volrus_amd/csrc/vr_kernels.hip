@@ -138,18 +138,21 @@ __global__ __launch_bounds__(64) void k_advect_slab(GridView in, VelView vel, Na
   }
 }
 
-// Compacted variant: the active voxels of the leaf are listed in LDS by a
-// ballot/popcount prefix over the 8 mask words, the wave computes them
-// densely (about 3 passes of 64 for a band leaf instead of 8 slabs), and the
-// leaf is then written back slab by slab, coalesced, with the background in
-// the inactive positions.
+// Compacted variant: one block of VR_ADVECT_WAVES waves per leaf; the active
+// voxels are listed in LDS by a ballot/popcount prefix over the 8 mask words
+// and spread over the block's lanes, and the leaf is then written back slab
+// by slab, coalesced, with the background in the inactive positions.
+#ifndef VR_ADVECT_WAVES
+#define VR_ADVECT_WAVES 4
+#endif
 template <int VK, int SCHEME>
-__global__ __launch_bounds__(64) void k_advect(GridView in, VelView vel, NanoLeaf* out, double dt, double tf0,
-                                               double tf1, double tf2) {
+__global__ __launch_bounds__(64 * VR_ADVECT_WAVES) void k_advect(GridView in, VelView vel, NanoLeaf* out, double dt,
+                                                                 double tf0, double tf1, double tf2) {
   __shared__ AdvectLDS s;
   const int n = *in.count;
   const double tf[3] = {tf0, tf1, tf2};
-  for (int li = blockIdx.x; li < n; li += gridDim.x) advect_leaf<VK, SCHEME>(in, vel, out, li, s, threadIdx.x, dt, tf);
+  for (int li = blockIdx.x; li < n; li += gridDim.x)
+    advect_leaf<VK, SCHEME, VR_ADVECT_WAVES>(in, vel, out, li, s, threadIdx.x, dt, tf);
 }
 
 template <int VK, int SCHEME>
@@ -164,7 +167,8 @@ static hipError_t launch_advect_t(const GridView& in, const VelView& vel, NanoLe
     hipLaunchKernelGGL((k_advect_slab<VK, SCHEME>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1],
                        tf[2]);
   else
-    hipLaunchKernelGGL((k_advect<VK, SCHEME>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1], tf[2]);
+    hipLaunchKernelGGL((k_advect<VK, SCHEME>), dim3(blocks), dim3(64 * VR_ADVECT_WAVES), 0, s, in, vel, out, dt, tf[0],
+                       tf[1], tf[2]);
   return hipGetLastError();
 }
 

@@ -144,26 +144,20 @@ __global__ void k_mc_tile_nbr(const GridView* views, const int4* tiles, const in
 // Owner of the lattice edge carried by local edge e of the cell at local
 // (lx, ly, lz): the candidate j (0..3) with the smallest visiting key
 // (tile origin, lx, ly, lz) among the cells of processed tiles.
+//
+// The key is one integer: the neighbour-tile index (tx+1)*9 + (ty+1)*3 +
+// (tz+1), whose order is the lexicographic order of tile origins around the
+// cell's tile, times 512, plus the local offset lx*64 + ly*8 + lz.
 __device__ __forceinline__ int mc_owner(int e, int lx, int ly, int lz, uint32_t processed) {
-  int best = -1;
-  int bk[6] = {0, 0, 0, 0, 0, 0};
+  int best = -1, bkey = 0x7fffffff;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int cx = lx + c_mc_cand[e][j][0], cy = ly + c_mc_cand[e][j][1], cz = lz + c_mc_cand[e][j][2];
-    const int tx = cx >> 3, ty = cy >> 3, tz = cz >> 3;  // tile offset -1, 0 or 1
-    if (!((processed >> ((tx + 1) * 9 + (ty + 1) * 3 + (tz + 1))) & 1u)) continue;
-    const int k[6] = {tx, ty, tz, cx & 7, cy & 7, cz & 7};
-    bool less = best < 0;
-    if (!less) {
-      int a = 0;
-      while (a < 6 && k[a] == bk[a]) ++a;
-      less = a < 6 && k[a] < bk[a];
-    }
-    if (less) {
-      best = j;
-#pragma unroll
-      for (int a = 0; a < 6; ++a) bk[a] = k[a];
-    }
+    const int t = ((cx >> 3) + 1) * 9 + ((cy >> 3) + 1) * 3 + ((cz >> 3) + 1);
+    const int key = (t << 9) + ((cx & 7) << 6) + ((cy & 7) << 3) + (cz & 7);
+    const bool ok = (processed >> t) & 1u;
+    best = (ok && key < bkey) ? j : best;
+    bkey = (ok && key < bkey) ? key : bkey;
   }
   return best;
 }
@@ -184,10 +178,43 @@ __device__ __forceinline__ void mc_stage(const GridView& g, const int* tleaf_t, 
   }
 }
 
+// ------------------------------------------------------------------ shared per-cell pass
+// Index of a cell among the 169 cells of a tile with a local coordinate 7
+// (-1 for the other 343).  An owner in another tile always has one: its tile
+// precedes the cell's, so its first differing tile offset is -1 and its local
+// coordinate on that axis is 7.
+__device__ __forceinline__ int mc_face_index(int lx, int ly, int lz) {
+  if (lx == 7) return ly * 8 + lz;
+  if (ly == 7) return 64 + lx * 8 + lz;
+  if (lz == 7) return 120 + lx * 7 + ly;
+  return -1;
+}
+#define MC_FACE 169
+
+__device__ __forceinline__ uint32_t mc_case(const float* brick, int x, int y, int z, float iso, float vals[8]) {
+  uint32_t cube = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    vals[c] = brick[(x + c_mc_corner[c][0]) * 81 + (y + c_mc_corner[c][1]) * 9 + (z + c_mc_corner[c][2])];
+    if (vals[c] < iso) cube |= 1u << c;
+  }
+  return cube;
+}
+
+__device__ __forceinline__ uint32_t mc_owned_mask(uint32_t em, int x, int y, int z, uint32_t processed) {
+  uint32_t owned = 0;
+  for (int e = 0; e < 12; ++e) {
+    if (!((em >> e) & 1u)) continue;
+    if (mc_owner(e, x, y, z, processed) == mc_self_cand(e)) owned |= 1u << e;
+  }
+  return owned;
+}
+
 // ------------------------------------------------------------------ pass 1
-// Per cell: case, owned crossing edges, triangle count.  meta = (vertex
-// prefix inside the tile << 12) | owned-edge mask, for cells with crossing
-// edges; per-tile vertex and triangle totals.
+// Per cell: case, owned crossing edges, triangle count.  The face cells
+// (mc_face_index >= 0) with crossing edges store meta = (vertex prefix inside
+// the tile << 12) | owned-edge mask for the emit pass of later tiles; per-tile
+// vertex and triangle totals.
 __global__ __launch_bounds__(64) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
                                                  const int* tleaf, const int* ttile, float iso, uint32_t* meta,
                                                  int* tv, int* tt) {
@@ -210,22 +237,15 @@ __global__ __launch_bounds__(64) void k_mc_count(const GridView* views, const in
     __syncthreads();
     int vcarry = 0, tcarry = 0;
     for (int x = 0; x < 8; ++x) {
-      uint32_t cube = 0;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const float v = brick[(x + c_mc_corner[c][0]) * 81 + (y + c_mc_corner[c][1]) * 9 + (z + c_mc_corner[c][2])];
-        if (v < iso) cube |= 1u << c;
-      }
+      float vals[8];
+      const uint32_t cube = mc_case(brick, x, y, z, iso, vals);
       const uint32_t em = c_mc_edge_mask[cube];
-      uint32_t owned = 0;
-      for (int e = 0; e < 12; ++e) {
-        if (!((em >> e) & 1u)) continue;
-        if (mc_owner(e, x, y, z, processed) == mc_self_cand(e)) owned |= 1u << e;
-      }
+      const uint32_t owned = em ? mc_owned_mask(em, x, y, z, processed) : 0u;
       const int nv = __popc(owned), ntr = c_mc_ntri[cube];
       const int vin = wave_incl_scan(nv);
       const int tin = wave_incl_scan(ntr);
-      if (em) meta[(size_t)t * 512 + x * 64 + lane] = ((uint32_t)(vcarry + vin - nv) << 12) | owned;
+      const int fi = mc_face_index(x, y, z);
+      if (em && fi >= 0) meta[(size_t)t * MC_FACE + fi] = ((uint32_t)(vcarry + vin - nv) << 12) | owned;
       vcarry += __shfl(vin, 63, 64);
       tcarry += __shfl(tin, 63, 64);
     }
@@ -240,12 +260,14 @@ __global__ __launch_bounds__(64) void k_mc_count(const GridView* views, const in
 // ------------------------------------------------------------------ pass 2
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
-// Vertex indices are relative to the tile's grid.
+// The tile's own cell meta is rebuilt in LDS; owners in earlier tiles are
+// read from their face meta.  Vertex indices are relative to the tile's grid.
 __global__ __launch_bounds__(64) void k_mc_emit(const GridView* views, const int4* tiles, const int* ntiles,
                                                 const int* tleaf, const int* ttile, const int* tile_base, float iso,
                                                 const uint32_t* meta, const int* vbase, const int* tbase,
                                                 const int* totals, double* verts, uint32_t* tris) {
   __shared__ float brick[729];
+  __shared__ uint32_t s_meta[512];
   __shared__ int s_tleaf[VR_NBR];
   __shared__ int s_ttile[VR_NBR];
   const int lane = threadIdx.x;
@@ -266,22 +288,30 @@ __global__ __launch_bounds__(64) void k_mc_emit(const GridView* views, const int
     for (int d = 0; d < VR_NBR; ++d) processed |= (s_ttile[d] >= 0 ? 1u : 0u) << d;
     mc_stage(g, s_tleaf, brick);
     __syncthreads();
+    {
+      int vcarry = 0;
+      for (int x = 0; x < 8; ++x) {
+        float vals[8];
+        const uint32_t em = c_mc_edge_mask[mc_case(brick, x, y, z, iso, vals)];
+        const uint32_t owned = em ? mc_owned_mask(em, x, y, z, processed) : 0u;
+        const int nv = __popc(owned);
+        const int vin = wave_incl_scan(nv);
+        s_meta[x * 64 + lane] = ((uint32_t)(vcarry + vin - nv) << 12) | owned;
+        vcarry += __shfl(vin, 63, 64);
+      }
+    }
+    __syncthreads();
     const int vb = vbase[t];
     const int vmax = totals[1], tmax = totals[2];
     int tcarry = tbase[t];
     for (int x = 0; x < 8; ++x) {
-      uint32_t cube = 0;
       float vals[8];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        vals[c] = brick[(x + c_mc_corner[c][0]) * 81 + (y + c_mc_corner[c][1]) * 9 + (z + c_mc_corner[c][2])];
-        if (vals[c] < iso) cube |= 1u << c;
-      }
+      const uint32_t cube = mc_case(brick, x, y, z, iso, vals);
       const uint32_t em = c_mc_edge_mask[cube];
       const int ntr = c_mc_ntri[cube];
       const int tin = wave_incl_scan(ntr);
       if (em) {
-        const uint32_t m = meta[(size_t)t * 512 + x * 64 + lane];
+        const uint32_t m = s_meta[x * 64 + lane];
         const uint32_t owned = m & 0xFFFu;
         int vi = vb + (int)(m >> 12);
         for (int e = 0; e < 12; ++e) {
@@ -312,10 +342,19 @@ __global__ __launch_bounds__(64) void k_mc_emit(const GridView* views, const int
           const int cx = x + c_mc_cand[e][j][0], cy = y + c_mc_cand[e][j][1], cz = z + c_mc_cand[e][j][2];
           const int ep = c_mc_cand[e][j][3];
           const int d = ((cx >> 3) + 1) * 9 + ((cy >> 3) + 1) * 3 + ((cz >> 3) + 1);
-          const int ot = s_ttile[d];
-          if (ot < 0) continue;
-          const uint32_t om = meta[(size_t)ot * 512 + ((cx & 7) << 6) + ((cy & 7) << 3) + (cz & 7)];
-          ev[e] = vbase[ot] - vrel + (int)(om >> 12) + __popc((om & 0xFFFu) & ((1u << ep) - 1u));
+          uint32_t om;
+          int obase;
+          if (d == 13) {
+            om = s_meta[(cx << 6) + (cy << 3) + cz];
+            obase = vb;
+          } else {
+            const int ot = s_ttile[d];
+            const int fi = mc_face_index(cx & 7, cy & 7, cz & 7);
+            if (ot < 0 || fi < 0) continue;
+            om = meta[(size_t)ot * MC_FACE + fi];
+            obase = vbase[ot];
+          }
+          ev[e] = obase - vrel + (int)(om >> 12) + __popc((om & 0xFFFu) & ((1u << ep) - 1u));
         }
         int ti = tcarry + tin - ntr;
         for (int q = 0; q < ntr && ti < tmax; ++q) {
@@ -382,7 +421,7 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
   const size_t tc = (size_t)(tcap + 1);
   if ((e = s.tiles.ensure(sizeof(int4) * tc)) != hipSuccess || (e = s.tleaf.ensure(sizeof(int) * VR_NBR * tc)) != hipSuccess ||
       (e = s.ttile.ensure(sizeof(int) * VR_NBR * tc)) != hipSuccess ||
-      (e = s.meta.ensure(sizeof(uint32_t) * 512 * tc)) != hipSuccess || (e = s.tv.ensure(sizeof(int) * tc)) != hipSuccess ||
+      (e = s.meta.ensure(sizeof(uint32_t) * MC_FACE * tc)) != hipSuccess || (e = s.tv.ensure(sizeof(int) * tc)) != hipSuccess ||
       (e = s.tt.ensure(sizeof(int) * tc)) != hipSuccess || (e = s.vb.ensure(sizeof(int) * tc)) != hipSuccess ||
       (e = s.tb.ensure(sizeof(int) * tc)) != hipSuccess ||
       (e = s.sums2.ensure(sizeof(int) * (vr_scan_scratch_ints(tc) + 1))) != hipSuccess ||

@@ -77,9 +77,68 @@ int active_count(vr_ctx* ctx, const vr_grid* g, uint64_t* out) {
   return vr_grid_counts(ctx, g, &lc, out);
 }
 
+__global__ void k_zero_u64(unsigned long long* p) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = 0ull;
+}
+
+// *out += popcount of the active masks of leaves [0, *count).
+__global__ __launch_bounds__(256) void k_active_sum(const NanoLeaf* __restrict__ leaves, const int* __restrict__ count,
+                                                    unsigned long long* out) {
+  const int n = *count * 8;
+  unsigned int c = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    c += (unsigned int)__popcll(leaves[i >> 3].mask[i & 7]);
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  __shared__ unsigned int part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
+// Device-side active-voxel count of g into *slot (no host synchronisation).
+int count_active_into(vr_ctx* ctx, const vr_grid* g, unsigned long long* slot) {
+  const int cap = g->topo->cap < 1 ? 1 : g->topo->cap;
+  size_t blocks = ((size_t)cap * 8 + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, slot);
+  hipLaunchKernelGGL(k_active_sum, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, g->d_leaves, g->topo->d_count,
+                     slot);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "active count");
+}
+
 __global__ void k_copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
+}
+
+// Device copy of a grid's lookup arrays, so the copy can be compacted in place.
+static int clone_topology(vr_ctx* ctx, const Topo& s, std::shared_ptr<Topo>* out) {
+  hipError_t e;
+  std::shared_ptr<Topo> t(new Topo());
+  t->cap = s.cap;
+  t->count = s.count;
+  t->has_empty = s.has_empty;
+  t->roots = s.roots;
+  const size_t cap = (size_t)(s.cap < 1 ? 1 : s.cap), R = s.roots.size() ? s.roots.size() : 1;
+  if ((e = hipMalloc(&t->d_count, sizeof(int))) != hipSuccess ||
+      (e = hipMalloc(&t->d_origins, sizeof(int4) * cap)) != hipSuccess ||
+      (e = hipMalloc(&t->d_roots, sizeof(int4) * R)) != hipSuccess ||
+      (e = hipMalloc(&t->d_child, sizeof(int) * 4096 * R)) != hipSuccess ||
+      (e = hipMalloc(&t->d_nbr, sizeof(int) * VR_NBR * cap)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMalloc");
+  if ((e = hipMemcpyAsync(t->d_count, s.d_count, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(t->d_origins, s.d_origins, sizeof(int4) * cap, hipMemcpyDeviceToDevice, ctx->stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(t->d_roots, s.d_roots, sizeof(int4) * R, hipMemcpyDeviceToDevice, ctx->stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(t->d_child, s.d_child, sizeof(int) * 4096 * R, hipMemcpyDeviceToDevice, ctx->stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(t->d_nbr, s.d_nbr, sizeof(int) * VR_NBR * cap, hipMemcpyDeviceToDevice, ctx->stream)) !=
+          hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMemcpyAsync");
+  *out = t;
+  return VR_OK;
 }
 
 extern "C" int vr_grid_clone(vr_ctx* ctx, const vr_grid* g, vr_grid** out) {
@@ -89,7 +148,8 @@ extern "C" int vr_grid_clone(vr_ctx* ctx, const vr_grid* g, vr_grid** out) {
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
   std::unique_ptr<vr_grid> o(new vr_grid());
   copy_header_into(o.get(), g);
-  o->topo = g->topo;
+  int rc = clone_topology(ctx, *g->topo, &o->topo);
+  if (rc != VR_OK) return rc;
   o->cap = g->topo->count;
   const size_t n = (size_t)(o->cap ? o->cap : 1);
   if ((e = hipMalloc(&o->d_leaves, sizeof(NanoLeaf) * n)) != hipSuccess) return vr_hip_fail(ctx, e, "hipMalloc");
@@ -122,13 +182,32 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
   vr_rollout_stats st;
   std::memset(&st, 0, sizeof(st));
-  uint64_t active = 0;
-  int rc = active_count(ctx, g, &active);
+  if (g->topo.use_count() > 1) {
+    // Own the lookup arrays so every compaction below runs in place and
+    // never reallocates the leaf buffers.
+    std::shared_ptr<Topo> t;
+    int rc0 = clone_topology(ctx, *g->topo, &t);
+    if (rc0 != VR_OK) return rc0;
+    g->topo = t;
+  }
+  // Active voxels of each topology segment (between rebuilds) are counted on
+  // the device into one slot per segment and read back once at the end.
+  const int nseg = 1 + (rebuild_every > 0 ? steps / rebuild_every : 0);
+  std::vector<int> seg_steps((size_t)nseg, 0);
+  DevBuf& slots = ctx->scratch[SCR_ACTIVE];
+  if ((e = slots.ensure(sizeof(unsigned long long) * (size_t)nseg)) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
+  int seg = 0;
+  int rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg);
   if (rc != VR_OK) return rc;
-  // Second leaf buffer; the advected grid shares g's topology.
-  NanoLeaf* other = nullptr;
+  // Second leaf buffer (context scratch; ownership is swapped back at the end);
+  // the advected grid shares g's topology.
   const int cap = g->topo->cap < 1 ? 1 : g->topo->cap;
-  if ((e = hipMalloc(&other, sizeof(NanoLeaf) * (size_t)cap)) != hipSuccess) return vr_hip_fail(ctx, e, "hipMalloc");
+  DevBuf& ping = ctx->scratch[SCR_PING];
+  if ((e = ping.ensure(sizeof(NanoLeaf) * (size_t)cap)) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
+  NanoLeaf* const pbuf = ping.as<NanoLeaf>();
+  const size_t own_bytes = sizeof(NanoLeaf) * (size_t)(g->cap < 1 ? 1 : g->cap);
+  const size_t pbuf_bytes = ping.bytes;
+  NanoLeaf* other = pbuf;
   std::unique_ptr<vr_mesh> own_mesh;
   if (!mesh && mesh_every > 0) {
     own_mesh.reset(new vr_mesh());
@@ -136,15 +215,19 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
   }
   for (int s = 0; s < steps && rc == VR_OK; ++s) {
     const double t = t0 + (double)s * dt;
-    vr_grid out_view;
-    copy_header_into(&out_view, g);
-    out_view.topo = g->topo;
-    out_view.d_leaves = other;
-    rc = advect_into(ctx, g, vel, dt, t, scheme, &out_view);
-    out_view.d_leaves = nullptr;  // not owned by the view
+    {
+      // The view shares g's topology only for the launch, so the
+      // compactions below see it unshared and run in place.
+      vr_grid out_view;
+      copy_header_into(&out_view, g);
+      out_view.topo = g->topo;
+      out_view.d_leaves = other;
+      rc = advect_into(ctx, g, vel, dt, t, scheme, &out_view);
+      out_view.d_leaves = nullptr;  // not owned by the view
+    }
     if (rc != VR_OK) break;
     std::swap(g->d_leaves, other);
-    st.voxel_updates += active;
+    seg_steps[seg] += 1;
     st.advect_steps += 1;
     if (g->topo->has_empty) {
       // Inputs with leaves that hold no active voxel: the first step drops them.
@@ -165,9 +248,12 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
     if (rebuild_every > 0 && (s + 1) % rebuild_every == 0) {
       rc = rebuild_impl(ctx, g, half_width, other);
       if (rc != VR_OK) break;
-      rc = active_count(ctx, g, &active);
       st.rebuilds += 1;
-      if (rc != VR_OK) break;
+      if (seg + 1 < nseg) {
+        ++seg;
+        rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg);
+        if (rc != VR_OK) break;
+      }
     }
     if (mesh_every > 0 && (s + 1) % mesh_every == 0) {
       if (g->topo->count > 0) {
@@ -180,15 +266,25 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
       st.meshes += 1;
     }
   }
-  (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(other);
+  // The grid keeps whichever buffer holds its data; the scratch slot takes
+  // the spare one (`other` is never freed by the steps above).
+  ping.p = other;
+  ping.bytes = other == pbuf ? pbuf_bytes : own_bytes;
+  if (g->d_leaves == pbuf) g->cap = (int)(pbuf_bytes / sizeof(NanoLeaf));
+  std::vector<unsigned long long> act((size_t)nseg, 0);
+  if (rc == VR_OK &&
+      ((e = hipMemcpyAsync(act.data(), slots.p, sizeof(unsigned long long) * nseg, hipMemcpyDeviceToHost,
+                           ctx->stream)) != hipSuccess ||
+       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess))
+    rc = vr_hip_fail(ctx, e, "readback");
   if (rc != VR_OK) return rc;
+  for (int k = 0; k <= seg; ++k) st.voxel_updates += act[k] * (uint64_t)seg_steps[k];
   if (mesh) {
     st.last_vertex_count = mesh->nv;
     st.last_triangle_count = mesh->nt;
   }
   st.final_leaf_count = (uint64_t)g->topo->count;
-  st.final_active_count = active;
+  st.final_active_count = act[seg];
   if (stats) *stats = st;
   return VR_OK;
 }

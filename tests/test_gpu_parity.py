@@ -265,3 +265,50 @@ def test_fixture_scenes_match_csg_chain(orc, ctx, vr):
     want = orc.scene(noisy, [0], vs, 3.0).to_nano()
     got = ctx.from_primitives([(vr.PRIM_NOISY_SPHERE, 0, noisy[0][1])], vs, 3.0).download()
     assert_nano_equal(got, want, "noisy")
+
+
+# ---------------------------------------------------------------- rollout
+def oracle_rollout(orc, g, kind, params, period, t0, dt, scheme, steps, rebuild_every, hw, mesh_every):
+    """advect -> rebuild every k -> marching cubes every m, composed from the oracle's tools."""
+    updates, mesh = 0, None
+    for s in range(steps):
+        tf = orc.stage_factors(period, t0 + s * dt, dt, scheme)
+        updates += g.active_count()
+        g = orc.advect_analytic(g, kind, params, dt, scheme, tf)
+        if rebuild_every and (s + 1) % rebuild_every == 0:
+            orc.rebuild(g, hw)
+        if mesh_every and (s + 1) % mesh_every == 0:
+            mesh = orc.mesh(g, 0.0)
+    return g, updates, mesh
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_rollout_matches_oracle_composition(orc, ctx, vr, shared):
+    """vr_rollout == the oracle's advect/rebuild/MC chain, over repeated calls
+    on one context (reused ping buffer) and on a grid whose lookup arrays are
+    shared with another grid (an advect output)."""
+    announce(f"rollout shared={shared}")
+    vs, hw, period = 1.0 / 64, 3.0, 8.0
+    dt = 1.0 * vs
+    steps, rb, me = 9, 3, 4
+    og = slotted_sphere(orc, vs)
+    vel = ctx.velocity_analytic(vr.VEL_VORTEX, (), period)
+    src = ctx.upload(og.to_nano())
+    base, t0 = src, 0.0
+    if shared:
+        base = src.advect(vel, dt, 0.0, 3)  # shares src's lookup arrays
+        og = orc.advect_analytic(og, 4, (), dt, 3, orc.stage_factors(period, 0.0, dt, 3))
+        t0 = dt
+    want, updates, wmesh = oracle_rollout(orc, og, 4, (), period, t0, dt, 3, steps, rb, hw, me)
+    want_b = want.to_nano()
+    mesh = vr.Mesh(ctx)
+    for rep in range(3):
+        g = base.clone() if rep < 2 else base
+        st = g.rollout(vel, t0, dt, vr.RK3, steps, rb, hw, me, 0.0, mesh)
+        assert_nano_equal(g.download(), want_b, f"rollout rep {rep}")
+        assert st.voxel_updates == updates
+        assert (st.advect_steps, st.rebuilds, st.meshes) == (steps, steps // rb, steps // me)
+        assert (st.final_leaf_count, st.final_active_count) == (want.leaf_count(), want.active_count())
+        vg, tg = mesh.arrays()
+        mesh_equal(vg, tg, wmesh[0], wmesh[1], vs, f"rollout mesh rep {rep}")
+        assert (st.last_vertex_count, st.last_triangle_count) == (len(wmesh[0]), len(wmesh[1]))
