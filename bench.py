@@ -80,7 +80,7 @@ class SplitMix64:
         return (self.next() >> 11) * (1.0 / 9007199254740992.0)
 
 
-def config3_scenes(vr, count=64, seed=0):
+def batched_sphere_scenes(vr, count=64, seed=0):
     """64 level sets, each the csg_union of 1..32 spheres with centres uniform
     in [0,1]^3 and radii log-uniform in [0.01, 0.2], and a uniform velocity
     with components uniform in [-1, 1] (SplitMix64, seed 0)."""
@@ -99,16 +99,17 @@ def config3_scenes(vr, count=64, seed=0):
 
 
 def batched_extra(vr, ctx, reps=3):
-    """Config 3: batched persistent graph step vs the per-grid loop, one GPU."""
+    """64 sphere unions at 1/1024, one RK3 step + MC each: batched persistent graph step vs the
+    per-grid loop, one GPU."""
     vs = 1.0 / 1024
     t0 = time.perf_counter()
-    spec = config3_scenes(vr)
+    spec = batched_sphere_scenes(vr)
     grids = [ctx.from_primitives(p, vs, HW) for p, _ in spec]
     vels = [ctx.velocity_analytic(vr.VEL_UNIFORM, v) for _, v in spec]
     dts = [0.5 * vs / max(abs(c) for c in v) for _, v in spec]
     counts = [g.counts() for g in grids]
     active = sum(a for _, a in counts)
-    log(f"config 3: 64 grids, {sum(l for l, _ in counts)} leaves, {active} active voxels "
+    log(f"batched sphere unions: 64 grids, {sum(l for l, _ in counts)} leaves, {active} active voxels "
         f"(min {min(a for _, a in counts)}, max {max(a for _, a in counts)}), built in {time.perf_counter() - t0:.1f} s")
     b = vr.Batch(ctx, grids, vels)
     b.set_dt(dts)
@@ -130,7 +131,7 @@ def batched_extra(vr, ctx, reps=3):
 
 
 def adjoint_extra(vr, ctx):
-    """Config 4: 20 RK1 steps of a 1/256 sphere through a dense 64^3 field
+    """Differentiable rollout: 20 RK1 steps of a 1/256 sphere through a dense 64^3 field
     (seeded Gaussian, sigma 0.5), loss vs the sphere moved by (0.1, 0, 0),
     gradient by the adjoint with a checkpoint at every step."""
     import numpy as np
@@ -184,7 +185,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the config 3 (batch) and config 4 (adjoint) lines")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint lines")
+    ap.add_argument("--only", choices=["batched", "adjoint"], help="run one side workload alone (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -192,6 +194,11 @@ def main():
 
     vr = load_package()
     ctx = vr.Context(0)
+    if args.only:
+        res = batched_extra(vr, ctx) if args.only == "batched" else adjoint_extra(vr, ctx)
+        print(json.dumps({args.only: res}), flush=True)
+        ctx.close()
+        return
     prims = [(vr.PRIM_SPHERE, 0, SPHERE), (vr.PRIM_BOX, vr.CSG_DIFFERENCE, SLOT)]
     base = ctx.from_primitives(prims, VS, HW)
     leaves0, active0 = base.counts()
@@ -240,15 +247,15 @@ def main():
     extras = {}
     if not args.no_extras:
         try:
-            extras["adjoint_config4"] = adjoint_extra(vr, ctx)
-            log(f"adjoint: {extras['adjoint_config4']}")
+            extras["adjoint_sphere256_dense64"] = adjoint_extra(vr, ctx)
+            log(f"adjoint: {extras['adjoint_sphere256_dense64']}")
         except Exception as e:  # reported, not fatal for the headline line
-            extras["adjoint_config4"] = {"error": str(e)}
+            extras["adjoint_sphere256_dense64"] = {"error": str(e)}
         try:
-            extras["batched_config3"] = batched_extra(vr, ctx)
-            log(f"batched: {extras['batched_config3']}")
+            extras["batched_64_sphere_unions_1024"] = batched_extra(vr, ctx)
+            log(f"batched: {extras['batched_64_sphere_unions_1024']}")
         except Exception as e:
-            extras["batched_config3"] = {"error": str(e)}
+            extras["batched_64_sphere_unions_1024"] = {"error": str(e)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -271,7 +278,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (slotted sphere generated on the device)",
         "config": {
-            "workload": "config 2: Zalesak slotted sphere r=0.15, vs=1/512, band 3, single vortex T=8, "
+            "workload": "Zalesak slotted sphere r=0.15, vs=1/512, band 3, single vortex T=8, "
                         "100 RK3 steps at CFL 0.5, rebuild every 5, marching cubes every 10; one step = one rollout",
             "leaves": leaves0, "active_voxels": active0, "voxel_updates_per_rollout": stats.voxel_updates,
             "final_triangles": stats.last_triangle_count, "parallelism": "single GPU",

@@ -176,10 +176,13 @@ int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_velocity* vel,
 
 /* ---- batches of level sets ----------------------------------------------- */
 /* No counterpart in the reference.  A batch advects every grid with its own
- * velocity and meshes the results in one step: one persistent launch over a
- * global leaf worklist (XCD-partitioned, `leaves_per_grab` leaves per atomic
- * grab, `waves_per_cu` one-wave blocks per CU) followed by marching cubes
- * over the concatenated tiles of all grids.  Inputs are borrowed and left
+ * velocity and meshes the results in one step: one launch over a global leaf
+ * worklist cut into chunks of `leaves_per_grab` leaves of one grid and
+ * partitioned over the 8 XCDs -- persistent, `waves_per_cu` waves per CU
+ * taking chunks from per-XCD atomic counters, or, with waves_per_cu = 0, one
+ * block per chunk -- followed by marching cubes over the concatenated tiles
+ * of all grids.  The advect code is specialised for the velocity kind when
+ * all grids share one.  Inputs are borrowed and left
  * unchanged; outputs live in the batch (vr_batch_output).  Modes:
  *   VR_BATCH_LOOP  the per-grid path (one advect launch + one MC pipeline per grid),
  *   VR_BATCH_EAGER the batched kernels launched directly,
@@ -191,7 +194,8 @@ int vr_batch_set_tuning(vr_batch* b, int leaves_per_grab, int waves_per_cu);
 /* Per-grid time steps (count doubles); NULL returns to the dt of vr_batch_step. */
 int vr_batch_set_dt(vr_ctx* ctx, vr_batch* b, const double* dts);
 /* Times the batched advect launch for leaves-per-grab in {1,2,4,8,16} and
- * waves-per-CU in {2,4,8,16,32} and keeps the fastest pair. */
+ * waves-per-CU in {0 (one block per chunk),4,8,16,32} and keeps the fastest
+ * pair. */
 int vr_batch_autotune(vr_ctx* ctx, vr_batch* b, double dt, int scheme, int* leaves_per_grab, int* waves_per_cu);
 int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int scheme, float iso, int mode);
 /* Mesh of grid i from the last step in `mode` (LOOP keeps its own meshes). */

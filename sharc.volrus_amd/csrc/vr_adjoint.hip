@@ -6,12 +6,12 @@
 // (src/tools/interpolation.rs:11-48) and p_i the RK1 or RK3 departure point
 // through the dense field U.  Only active voxels are variables; inactive and
 // background values are constants.  Per backward step and per leaf (one
-// wave):
+// block of 4 waves):
 //   - the adjoint lambda_{n+1} of each active voxel is scattered with the
-//     trilinear weights into an LDS brick (leaf + one-voxel halo); the brick
-//     goes to an outbox and a merge kernel sums, for each voxel, the entries
-//     of the (up to 8) leaves whose brick covers it in a fixed order, so
-//     lambda_n is deterministic;
+//     trilinear weights into per-wave LDS bricks (leaf + one-voxel halo),
+//     summed in wave order into an outbox; a merge kernel sums, for each
+//     voxel, the entries of the (up to 8) leaves whose brick covers it in a
+//     fixed order, so lambda_n is deterministic;
 //   - lambda * dS/dp is pulled back through the departure point (and, for
 //     RK3, through the stage positions with the field's Jacobian) onto the
 //     velocity nodes, pre-reduced in an LDS window and added to dL/dU with
@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 
+#include "vr_advect.h"
 #include "vr_internal.h"
 
 #define VW 6  // velocity nodes per axis in a leaf's LDS window
@@ -46,8 +47,17 @@ __device__ __forceinline__ void dense_weights(const VelView& v, const double p[3
   }
 }
 
+// LDS float add on a pointer into __shared__ memory, always as a DS
+// instruction (a generic-pointer atomic that may also target global memory
+// would compile to a slower FLAT atomic).
+__device__ __forceinline__ void lds_add(float* p, float v) {
+  __builtin_amdgcn_ds_faddf((__attribute__((address_space(3))) float*)p, v, 0, 0, false);
+}
+
 // Adds coef[c] * w_q(p) to node q of component c: LDS window when inside it,
-// else global.
+// else global.  Lanes visit the 8 nodes in a lane-rotated order: voxels of
+// one wave mostly share a velocity cell, and the rotation makes the lanes of
+// one atomic instruction hit different nodes instead of serialising on one.
 __device__ __forceinline__ void vel_scatter(const AdjStep& P, float* vwin, const int wb[3], const double p[3],
                                             const double coef[3]) {
   int b[3];
@@ -55,8 +65,10 @@ __device__ __forceinline__ void vel_scatter(const AdjStep& P, float* vwin, const
   dense_weights(P.vel, p, b, f);
   const int n = P.vel.n;
   const size_t n3 = (size_t)n * n * n;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
+  const int lane = threadIdx.x & 63;
+  const int rot = lane + (lane >> 3);
+  for (int s = 0; s < 8; ++s) {
+    const int q = (s + rot) & 7;
     const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
     const int i = b[0] + dx, j = b[1] + dy, k = b[2] + dz;
     if ((unsigned)i >= (unsigned)n || (unsigned)j >= (unsigned)n || (unsigned)k >= (unsigned)n) continue;
@@ -67,7 +79,7 @@ __device__ __forceinline__ void vel_scatter(const AdjStep& P, float* vwin, const
     for (int c = 0; c < 3; ++c) {
       const float val = (float)(coef[c] * w);
       if (in)
-        atomicAdd(&vwin[c * VW3 + (wi * VW + wj) * VW + wk], val);
+        lds_add(&vwin[c * VW3 + (wi * VW + wj) * VW + wk], val);
       else
         atomicAdd(&P.grad[c * n3 + ((size_t)i * n + j) * n + k], val);
     }
@@ -95,30 +107,37 @@ __device__ __forceinline__ void dense_jt(const VelView& v, const double p[3], co
   for (int a = 0; a < 3; ++a) out[a] *= v.inv_h;
 }
 
+#define ADJ_WAVES 4
+#define ADJ_THREADS (64 * ADJ_WAVES)
+
+// One backward step, one leaf per block of ADJ_WAVES waves.  Each wave
+// scatters its voxels' lambda into its own LDS brick (a wave's LDS atomics
+// apply in a fixed order), and the outbox entry is the sum of the wave
+// bricks in wave order, so lambda_n is the same on every run.
 template <int SCHEME>
-__global__ __launch_bounds__(64) void k_adj_step(AdjStep P) {
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(AdjStep P) {
   __shared__ float brick[1000];
-  __shared__ float acc[1000];
+  __shared__ float acc[ADJ_WAVES][1000];
   __shared__ float vwin[3 * VW3];
   __shared__ uint16_t list[512];
   __shared__ int nb[VR_NBR];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const GridView& g = P.phi;
   const int n = *g.count;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double tf[3] = {P.tf0, P.tf1, P.tf2};
   for (int li = blockIdx.x; li < n; li += gridDim.x) {
     const NanoLeaf& L = g.leaves[li];
-    if (lane < VR_NBR) nb[lane] = g.nbr[(size_t)li * VR_NBR + lane];
+    if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
     int total = 0;
 #pragma unroll
     for (int x = 0; x < 8; ++x) {
       const uint64_t m = L.mask[x];
-      if ((m >> lane) & 1ull) list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
+      if ((x % ADJ_WAVES) == wave && ((m >> lane) & 1ull)) list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
       total += __popcll(m);
     }
-    for (int i = lane; i < 1000; i += 64) acc[i] = 0.0f;
-    for (int i = lane; i < 3 * VW3; i += 64) vwin[i] = 0.0f;
+    for (int i = tid; i < ADJ_WAVES * 1000; i += ADJ_THREADS) (&acc[0][0])[i] = 0.0f;
+    for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin[i] = 0.0f;
     const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
     int wb[3];
     {
@@ -126,10 +145,12 @@ __global__ __launch_bounds__(64) void k_adj_step(AdjStep P) {
       index_to_world(g.mat, ox, oy, oz, ow);
       for (int a = 0; a < 3; ++a) wb[a] = f64_to_i32(floor(ow[a] * P.vel.inv_h)) - 2;
     }
+    stage_interior<ADJ_THREADS>(g, li, brick, tid);
     __syncthreads();
-    stage_brick_dev(g, nb, brick, lane);
+    stage_halo<ADJ_THREADS>(g, nb, brick, tid);
     __syncthreads();
-    for (int ii = lane; ii < total; ii += 64) {
+    float* const wacc = acc[wave];
+    for (int ii = tid; ii < total; ii += ADJ_THREADS) {
       const int off = list[ii];
       const float lam = P.lam_next[(size_t)li * 512 + off];
       if (lam == 0.0f) continue;
@@ -160,7 +181,7 @@ __global__ __launch_bounds__(64) void k_adj_step(AdjStep P) {
         if (inb) {
           const int bidx = (bi + 1 + dx) * 100 + (bj + 1 + dy) * 10 + (bk + 1 + dz);
           c[q] = (double)brick[bidx];
-          atomicAdd(&acc[bidx], (float)((double)lam * w));
+          lds_add(&wacc[bidx], (float)((double)lam * w));
         } else {
           const int cx = i0 + dx, cy = j0 + dy, cz = k0 + dz;
           const int tl = find_leaf(g, cx, cy, cz);
@@ -196,10 +217,15 @@ __global__ __launch_bounds__(64) void k_adj_step(AdjStep P) {
     }
     __syncthreads();
     float* ob = P.outbox + (size_t)li * 1000;
-    for (int i = lane; i < 1000; i += 64) ob[i] = acc[i];
+    for (int i = tid; i < 1000; i += ADJ_THREADS) {
+      float v = acc[0][i];
+#pragma unroll
+      for (int w = 1; w < ADJ_WAVES; ++w) v += acc[w][i];
+      ob[i] = v;
+    }
     const int nv = P.vel.n;
     const size_t n3 = (size_t)nv * nv * nv;
-    for (int i = lane; i < 3 * VW3; i += 64) {
+    for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) {
       const float v = vwin[i];
       if (v == 0.0f) continue;
       const int c = i / VW3, r = i % VW3;
@@ -211,28 +237,30 @@ __global__ __launch_bounds__(64) void k_adj_step(AdjStep P) {
 }
 
 // lambda_n(y) for each active voxel y: the outbox entries of the leaves whose
-// brick covers y, summed in neighbour order, plus the far-corner buffer.
-__global__ __launch_bounds__(64) void k_adj_merge(GridView g, const float* outbox, float* lam_far, float* lam) {
+// brick covers y, summed in neighbour order, plus the far-corner buffer.  A
+// neighbour at offset (dx, dy, dz) covers y only where y's local coordinate
+// is 0 (for d = -1) or 7 (for d = +1) on every axis with d != 0, so at most 8
+// bricks are visited, in ascending neighbour order.
+__global__ __launch_bounds__(256) void k_adj_merge(GridView g, const float* outbox, float* lam_far, float* lam) {
   __shared__ int nb[VR_NBR];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const int n = *g.count;
-  const int y = lane >> 3, z = lane & 7;
   for (int li = blockIdx.x; li < n; li += gridDim.x) {
-    if (lane < VR_NBR) nb[lane] = g.nbr[(size_t)li * VR_NBR + lane];
+    if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
     __syncthreads();
     const NanoLeaf& L = g.leaves[li];
-    for (int x = 0; x < 8; ++x) {
-      const int off = x * 64 + lane;
+    for (int off = tid; off < 512; off += 256) {
+      const int x = off >> 6, y = (off >> 3) & 7, z = off & 7;
       float s = 0.0f;
-      if ((L.mask[x] >> lane) & 1ull) {
-        for (int d = 0; d < VR_NBR; ++d) {
-          const int src = nb[d];
-          if (src < 0) continue;
-          const int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
-          const int lx = x - 8 * dx, ly = y - 8 * dy, lz = z - 8 * dz;
-          if (lx < -1 || lx > 8 || ly < -1 || ly > 8 || lz < -1 || lz > 8) continue;
-          s += outbox[(size_t)src * 1000 + (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1)];
-        }
+      if ((L.mask[x] >> (off & 63)) & 1ull) {
+        for (int dx = (x == 0 ? -1 : 0); dx <= (x == 7 ? 1 : 0); ++dx)
+          for (int dy = (y == 0 ? -1 : 0); dy <= (y == 7 ? 1 : 0); ++dy)
+            for (int dz = (z == 0 ? -1 : 0); dz <= (z == 7 ? 1 : 0); ++dz) {
+              const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+              if (src < 0) continue;
+              const int lx = x - 8 * dx, ly = y - 8 * dy, lz = z - 8 * dz;
+              s += outbox[(size_t)src * 1000 + (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1)];
+            }
         s += lam_far[(size_t)li * 512 + off];
       }
       lam[(size_t)li * 512 + off] = s;
@@ -340,7 +368,8 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   auto step_forward = [&](NanoLeaf* src, NanoLeaf* dst) {
     if (rc != VR_OK) return;
     view.d_leaves = src;
-    if ((e = vr_launch_advect(make_view(&view), vv, dst, L, dt, tf, scheme, ctx->stream)) != hipSuccess)
+    if ((e = vr_launch_advect(make_view(&view), vv, dst, L, dt, tf, scheme, vr_advect_waves(L, ctx->cus),
+                              ctx->stream)) != hipSuccess)
       rc = vr_hip_fail(ctx, e, "advect");
   };
   if (rc == VR_OK) {
@@ -394,10 +423,10 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
       P.dt = dt;
       P.tf0 = P.tf1 = P.tf2 = 1.0;
       if (scheme == VR_SCHEME_RK3)
-        hipLaunchKernelGGL(k_adj_step<3>, dim3(grid_blocks(L, 1)), dim3(64), 0, ctx->stream, P);
+        hipLaunchKernelGGL(k_adj_step<3>, dim3(grid_blocks(L, 1)), dim3(ADJ_THREADS), 0, ctx->stream, P);
       else
-        hipLaunchKernelGGL(k_adj_step<1>, dim3(grid_blocks(L, 1)), dim3(64), 0, ctx->stream, P);
-      hipLaunchKernelGGL(k_adj_merge, dim3(grid_blocks(L, 1)), dim3(64), 0, ctx->stream, P.phi, outbox, far, lam_cur);
+        hipLaunchKernelGGL(k_adj_step<1>, dim3(grid_blocks(L, 1)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+      hipLaunchKernelGGL(k_adj_merge, dim3(grid_blocks(L, 1)), dim3(256), 0, ctx->stream, P.phi, outbox, far, lam_cur);
       if ((e = hipGetLastError()) != hipSuccess) rc = vr_hip_fail(ctx, e, "adjoint step");
       std::swap(lam_next, lam_cur);
     }

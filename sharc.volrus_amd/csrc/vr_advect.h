@@ -32,6 +32,56 @@ __device__ __forceinline__ float advect_voxel(const GridView& in, const VelView&
   return trilinear_blend(c, fx, fy, fz);
 }
 
+// Stages the 10^3 brick of leaf li (leaf plus one-voxel halo) with NT
+// threads in two parts: the leaf's own 512 values straight from its record
+// (no dependence on the neighbour list, so they load while the list does),
+// then the 488 halo values through nb[] once the caller has loaded it into
+// LDS and synchronised.
+template <int NT>
+__device__ __forceinline__ void stage_interior(const GridView& in, int li, float* brick, int tid) {
+  const float4* own = reinterpret_cast<const float4*>(in.leaves[li].values);
+  for (int q = tid; q < 128; q += NT) {
+    const float4 v = own[q];
+    const int x = q >> 4, y = (q >> 1) & 7, z = (q & 1) * 4;
+    float* b = brick + (x + 1) * 100 + (y + 1) * 10 + (z + 1);
+    b[0] = v.x;
+    b[1] = v.y;
+    b[2] = v.z;
+    b[3] = v.w;
+  }
+}
+
+// Halo: the 1000 - 512 brick positions with some local coordinate -1 or 8.
+template <int NT>
+__device__ __forceinline__ void stage_halo(const GridView& in, const int* nb, float* brick, int tid) {
+  for (int h = tid; h < 488; h += NT) {
+    int bx, by, bz;
+    if (h < 200) {  // x = -1 or x = 8 planes (10 x 10 each)
+      bx = h < 100 ? -1 : 8;
+      const int r = h % 100;
+      by = r / 10 - 1;
+      bz = r % 10 - 1;
+    } else if (h < 360) {  // y = -1 or y = 8 rows, x in 0..7 (8 x 10 each)
+      const int r = h - 200;
+      by = r < 80 ? -1 : 8;
+      const int u = r % 80;
+      bx = u / 10;
+      bz = u % 10 - 1;
+    } else {  // z = -1 or z = 8, x, y in 0..7 (8 x 8 each)
+      const int r = h - 360;
+      bz = r < 64 ? -1 : 8;
+      const int u = r & 63;
+      bx = u >> 3;
+      by = u & 7;
+    }
+    const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
+              dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
+    const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+    brick[(bx + 1) * 100 + (by + 1) * 10 + (bz + 1)] =
+        src < 0 ? in.background : in.leaves[src].values[voxel_offset(bx, by, bz)];
+  }
+}
+
 // LDS of one advecting block.
 struct AdvectLDS {
   float brick[1000];
@@ -67,14 +117,9 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
 #pragma unroll
   for (int x = 0; x < 8; ++x)
     if ((x % NW) == wave && ((m[x] >> lane) & 1ull)) s.list[pre[x] + __popcll(m[x] & below)] = (uint16_t)(x * 64 + lane);
+  stage_interior<NT>(in, li, s.brick, tid);
   __syncthreads();
-  for (int i = tid; i < 1000; i += NT) {
-    const int bx = i / 100 - 1, by = (i / 10) % 10 - 1, bz = i % 10 - 1;
-    const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
-              dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
-    const int src = s.nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
-    s.brick[i] = src < 0 ? in.background : in.leaves[src].values[voxel_offset(bx, by, bz)];
-  }
+  stage_halo<NT>(in, s.nb, s.brick, tid);
   __syncthreads();
   const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
   for (int i = tid; i < total; i += NT) {
@@ -86,8 +131,14 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   if (tid < 3) O.origin[tid] = L.origin[tid];
   if (tid == 3) O.pad = 0;
   if (tid < 8) O.mask[tid] = m[tid];
+  const float bg = in.background;
 #pragma unroll
   for (int x = 0; x < 8; ++x)
-    if ((x % NW) == wave) O.values[x * 64 + lane] = ((m[x] >> lane) & 1ull) ? s.res[x * 64 + lane] : in.background;
+    if ((x % NW) == wave) {
+      // Both operands are read, so the select is of values, not of an LDS
+      // and a global address (which would need a FLAT load).
+      const float r = s.res[x * 64 + lane];
+      O.values[x * 64 + lane] = ((m[x] >> lane) & 1ull) ? r : bg;
+    }
   __syncthreads();
 }

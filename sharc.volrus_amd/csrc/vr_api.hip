@@ -138,6 +138,8 @@ extern "C" int vr_ctx_create(int device, vr_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return VR_ERR_HIP;
   std::unique_ptr<vr_ctx> c(new vr_ctx());
   c->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) c->cus = cus;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return VR_ERR_HIP;
   if (upload_tables_once(device) != hipSuccess) {
     (void)hipStreamDestroy(c->stream);
@@ -172,12 +174,12 @@ extern "C" void* vr_ctx_stream(vr_ctx* ctx) { return ctx ? (void*)ctx->stream : 
 // ------------------------------------------------------------------ topology
 GridView make_view(const vr_grid* g) {
   GridView v;
-  v.leaves = g->d_leaves;
-  v.count = g->topo->d_count;
-  v.origins = g->topo->d_origins;
-  v.roots = g->topo->d_roots;
-  v.child = g->topo->d_child;
-  v.nbr = g->topo->d_nbr;
+  v.leaves = (decltype(v.leaves))g->d_leaves;
+  v.count = (decltype(v.count))g->topo->d_count;
+  v.origins = (decltype(v.origins))g->topo->d_origins;
+  v.roots = (decltype(v.roots))g->topo->d_roots;
+  v.child = (decltype(v.child))g->topo->d_child;
+  v.nbr = (decltype(v.nbr))g->topo->d_nbr;
   v.root_count = (int)g->topo->roots.size();
   v.background = g->background;
   v.vs = g->voxel_size;
@@ -525,7 +527,7 @@ VelView make_vel_view(const vr_velocity* v) {
   for (int i = 0; i < 6; ++i) vv.a[i] = v->a[i];
   if (v->kind == VK_GRIDS3)
     for (int c = 0; c < 3; ++c) vv.g[c] = make_view(v->g[c]);
-  vv.dense = v->d_dense;
+  vv.dense = (decltype(vv.dense))v->d_dense;
   vv.n = v->n;
   vv.inv_h = v->kind == VK_DENSE ? 1.0 / v->h : 0.0;
   return vv;
@@ -558,7 +560,8 @@ int advect_into(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel, double d
   hipError_t e;
   {
     TimedLaunch tl(ctx, TC_ADVECT, bytes);
-    e = vr_launch_advect(make_view(in), make_vel_view(vel), o->d_leaves, in->topo->cap, dt, tf, scheme, ctx->stream);
+    e = vr_launch_advect(make_view(in), make_vel_view(vel), o->d_leaves, in->topo->cap, dt, tf, scheme,
+                         vr_advect_waves(in->topo->count, ctx->cus), ctx->stream);
   }
   return e == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "advect");
 }

@@ -1,23 +1,27 @@
 // Batched advection + marching cubes over many level sets.
 //
 // No reference counterpart (the reference handles one grid per call).  The
-// leaves of all grids form one global worklist (grid g owns global leaves
-// [leaf_base[g], leaf_base[g+1])).  One persistent launch of
-// `waves_per_cu` one-wave blocks per CU advects them: the worklist is cut
-// into 8 contiguous ranges, one per XCD (blocks are dispatched to the 8 XCDs
-// round-robin by block index, so block b serves range b % 8), and waves grab
-// `leaves_per_grab` consecutive leaves at a time from their range's atomic
-// counter, so neighbouring leaves, whose halos overlap, stay in one XCD's L2.
-// Marching cubes then runs over the concatenated tiles of all grids
-// (vr_mc.hip).  The whole step can be captured once into a hipGraph and
-// replayed; `vr_batch_autotune` picks leaves-per-grab and waves-per-CU by
-// timing the advect launch.
+// leaves of all grids form one global worklist, cut into chunks of up to
+// `leaves_per_grab` consecutive leaves of one grid.  One persistent launch
+// of `waves_per_cu` waves per CU advects them: the chunk list is cut into 8
+// contiguous ranges, one per XCD (blocks are dispatched to the 8 XCDs
+// round-robin by block index, so block b serves range b % 8), and blocks
+// take chunks from their range's atomic counter, so neighbouring leaves,
+// whose halos overlap, stay in one XCD's L2.  The leaf code is specialised
+// for the velocity kind the grids share, with one wave per leaf once the
+// batch has enough leaves to fill the device.  Marching cubes then runs over the
+// concatenated tiles of all grids (vr_mc.hip).  The whole step can be
+// captured once into a hipGraph and replayed; `vr_batch_autotune` picks
+// leaves-per-grab and waves-per-CU by timing the advect launch.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <vector>
 
 #include "vr_advect.h"
 #include "vr_internal.h"
+
+#define VK_MIXED (-1)  // grids of one batch with different velocity kinds
 
 struct vr_batch {
   int G = 0;
@@ -27,8 +31,11 @@ struct vr_batch {
   std::vector<int> leaf_base;                 // G + 1
   long long ltot = 0;
   int lpg = 4, wpc = 8;                       // leaves per grab, waves per CU
+  int vk = VK_MIXED;                          // velocity kind shared by all grids
   int cus = 256;
-  DevBuf d_in, d_out_views, d_vel, d_outs, d_leaf_base, d_counters, d_ranges, d_dts;
+  DevBuf d_in, d_out_views, d_vel, d_outs, d_leaf_base, d_counters, d_ranges, d_dts, d_chunks;
+  int chunk_lpg = 0;  // leaves per chunk the chunk table was built for
+  int max_range = 0;  // most chunks in one XCD range
   bool has_dts = false;
   std::vector<double> dts;
   MCMulti mc;
@@ -53,43 +60,99 @@ struct vr_batch {
 };
 
 // ------------------------------------------------------------------ persistent advect
-template <int SCHEME>
-__global__ __launch_bounds__(64) void k_batch_advect(const GridView* ins, const VelView* vels, NanoLeaf* const* outs,
-                                                     const int* leaf_base, int G, int* counters, const int* ranges,
-                                                     int lpg, const double* dts, double dt, double tf0, double tf1,
-                                                     double tf2) {
-  __shared__ AdvectLDS s;
-  __shared__ int s_item;
-  const int lane = threadIdx.x;
-  const int xcd = blockIdx.x & 7;
-  const int lo = ranges[xcd], hi = ranges[xcd + 1];
-  const double tf[3] = {tf0, tf1, tf2};
-  for (;;) {
-    if (lane == 0) s_item = atomicAdd(&counters[xcd], lpg);
-    __syncthreads();
-    const int start = lo + s_item;
-    __syncthreads();
-    if (start >= hi) break;
-    const int end = start + lpg < hi ? start + lpg : hi;
-    for (int gl = start; gl < end; ++gl) {
-      int a = 0, b = G;  // grid of global leaf gl
-      while (b - a > 1) {
-        const int mid = (a + b) >> 1;
-        if (leaf_base[mid] <= gl) a = mid; else b = mid;
-      }
-      const int li = gl - leaf_base[a];
-      const GridView& in = ins[a];
-      const VelView& v = vels[a];
-      const double gdt = dts ? dts[a] : dt;
+// VK = the velocity kind shared by every grid of the batch (the leaf code is
+// specialised for it), or VK_MIXED when kinds differ (a per-leaf switch);
+// NW waves per leaf over its compacted active list.
+
+template <int VK, int SCHEME, int NW, class LDS>
+__device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* vels, NanoLeaf* const* outs,
+                                            const int4& ck, LDS& s, int tid, const double* dts, double dt,
+                                            const double tf[3]) {
+  const GridView& in = ins[ck.x];
+  const VelView& v = vels[ck.x];
+  // Typed global where loaded (kernel parameter types stay unqualified: the
+  // host pass has no address spaces and must name the same kernel).
+  NanoLeaf* const out = (VR_GLOBAL NanoLeaf*)reinterpret_cast<const uintptr_t*>(outs)[ck.x];
+  const double gdt = dts ? dts[ck.x] : dt;
+  for (int li = ck.y; li < ck.y + ck.z; ++li) {
+    if constexpr (VK != VK_MIXED) {
+      advect_leaf<VK, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf);
+    } else {
       switch (v.kind) {
-        case VK_UNIFORM: advect_leaf<VK_UNIFORM, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
-        case VK_RIGID: advect_leaf<VK_RIGID, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
-        case VK_VORTEX: advect_leaf<VK_VORTEX, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
-        case VK_DENSE: advect_leaf<VK_DENSE, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
-        default: advect_leaf<VK_GRIDS3, SCHEME, 1>(in, v, outs[a], li, s, lane, gdt, tf); break;
+        case VK_UNIFORM: advect_leaf<VK_UNIFORM, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
+        case VK_RIGID: advect_leaf<VK_RIGID, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
+        case VK_VORTEX: advect_leaf<VK_VORTEX, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
+        case VK_DENSE: advect_leaf<VK_DENSE, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
+        default: advect_leaf<VK_GRIDS3, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
       }
     }
   }
+}
+
+// persistent != 0: `gridDim.x` blocks loop, taking chunks from their XCD's
+// counter.  persistent == 0: one chunk per block, block b taking chunk
+// b / 8 of XCD b % 8's range (the grid covers 8 x the longest range).
+template <int VK, int SCHEME, int NW>
+__global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, const VelView* vels,
+                                                          NanoLeaf* const* outs, const int4* chunks, int* counters,
+                                                          const int* ranges, int persistent, const double* dts,
+                                                          double dt, double tf0, double tf1, double tf2) {
+  __shared__ AdvectLDS s;
+  __shared__ int s_item;
+  const int tid = threadIdx.x;
+  const int xcd = blockIdx.x & 7;
+  const int lo = ranges[xcd], hi = ranges[xcd + 1];
+  const double tf[3] = {tf0, tf1, tf2};
+  if (!persistent) {
+    const int c = lo + (int)(blockIdx.x >> 3);
+    if (c < hi) batch_chunk<VK, SCHEME, NW>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf);
+    return;
+  }
+  // Chunk indices read from LDS are block-uniform; readfirstlane tells the
+  // compiler so, and the chunk and grid descriptors load as scalars.
+  if (tid == 0) s_item = atomicAdd(&counters[xcd], 1);
+  __syncthreads();
+  int c = __builtin_amdgcn_readfirstlane(lo + s_item);
+  __syncthreads();
+  while (c < hi) {
+    // Grab the next chunk now; its latency overlaps this chunk's loads.
+    int next = 0;
+    if (tid == 0) next = atomicAdd(&counters[xcd], 1);
+    batch_chunk<VK, SCHEME, NW>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf);
+    if (tid == 0) s_item = next;
+    __syncthreads();
+    c = __builtin_amdgcn_readfirstlane(lo + s_item);
+    __syncthreads();
+  }
+}
+
+template <int VK, int SCHEME>
+static void launch_batch_t(int nw, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
+                           NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges, int persistent,
+                           const double* dts, double dt, const double tf[3]) {
+  if (nw == 1)
+    hipLaunchKernelGGL((k_batch_advect<VK, SCHEME, 1>), dim3(blocks), dim3(64), 0, st, ins, vels, outs, chunks,
+                       counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2]);
+  else
+    hipLaunchKernelGGL((k_batch_advect<VK, SCHEME, 4>), dim3(blocks), dim3(256), 0, st, ins, vels, outs, chunks,
+                       counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2]);
+}
+
+template <int SCHEME>
+static void launch_batch_s(int vk, int nw, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
+                           NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges,
+                           int persistent, const double* dts, double dt, const double tf[3]) {
+#define VR_BK(K) \
+  launch_batch_t<K, SCHEME>(nw, blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf)
+  switch (vk) {
+    case VK_UNIFORM: VR_BK(VK_UNIFORM); break;
+    case VK_RIGID: VR_BK(VK_RIGID); break;
+    case VK_VORTEX: VR_BK(VK_VORTEX); break;
+    case VK_DENSE: VR_BK(VK_DENSE); break;
+    case VK_GRIDS3: VR_BK(VK_GRIDS3); break;
+    default: VR_BK(VK_MIXED); break;
+  }
+#undef VR_BK
 }
 
 __global__ void k_batch_reset(int* counters) {
@@ -97,6 +160,35 @@ __global__ void k_batch_reset(int* counters) {
 }
 
 // ------------------------------------------------------------------ API
+// The global worklist as chunks of up to `lpg` consecutive leaves of one
+// grid, (grid, first leaf, leaf count, 0), in grid order; cut into 8
+// contiguous ranges of chunks, one per XCD.  Rebuilt when lpg changes
+// (never inside a graph capture: the step builds it before capturing).
+static int batch_chunks(vr_ctx* ctx, vr_batch* b) {
+  if (b->chunk_lpg == b->lpg && b->d_chunks.p) return VR_OK;
+  std::vector<int4> ch;
+  for (int g = 0; g < b->G; ++g) {
+    const int n = b->leaf_base[g + 1] - b->leaf_base[g];
+    for (int f = 0; f < n; f += b->lpg) ch.push_back(make_int4(g, f, n - f < b->lpg ? n - f : b->lpg, 0));
+  }
+  const long long nc = (long long)ch.size();
+  int ranges[9];
+  for (int x = 0; x <= 8; ++x) ranges[x] = (int)((nc * x) / 8);
+  b->max_range = 0;
+  for (int x = 0; x < 8; ++x) b->max_range = std::max(b->max_range, ranges[x + 1] - ranges[x]);
+  hipError_t e;
+  if ((e = b->d_chunks.ensure(sizeof(int4) * (ch.size() ? ch.size() : 1))) != hipSuccess ||
+      (e = b->d_ranges.ensure(sizeof(int) * 9)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMalloc");
+  if ((ch.size() && (e = hipMemcpyAsync(b->d_chunks.p, ch.data(), sizeof(int4) * ch.size(), hipMemcpyHostToDevice,
+                                        ctx->stream)) != hipSuccess) ||
+      (e = hipMemcpyAsync(b->d_ranges.p, ranges, sizeof(ranges), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "hipMemcpy");
+  b->chunk_lpg = b->lpg;
+  return VR_OK;
+}
+
 extern "C" int vr_batch_create(vr_ctx* ctx, vr_grid* const* grids, const vr_velocity* const* vels, int count,
                                vr_batch** out) {
   if (!ctx || !grids || !vels || count <= 0 || !out) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "bad batch");
@@ -133,15 +225,16 @@ extern "C" int vr_batch_create(vr_ctx* ctx, vr_grid* const* grids, const vr_velo
     b->leaf_base[i + 1] = b->leaf_base[i] + g->topo->count;
   }
   b->ltot = b->leaf_base[count];
+  b->vk = vels[0]->kind;
+  for (int i = 1; i < count; ++i)
+    if (vels[i]->kind != b->vk) b->vk = VK_MIXED;
   if (b->ltot > 0x7fffffff / 7) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "batch too large");
-  std::vector<int> ranges(9);
-  for (int x = 0; x <= 8; ++x) ranges[x] = (int)((b->ltot * x) / 8);
   if ((e = b->d_in.ensure(sizeof(GridView) * count)) != hipSuccess ||
       (e = b->d_out_views.ensure(sizeof(GridView) * count)) != hipSuccess ||
       (e = b->d_vel.ensure(sizeof(VelView) * count)) != hipSuccess ||
       (e = b->d_outs.ensure(sizeof(NanoLeaf*) * count)) != hipSuccess ||
       (e = b->d_leaf_base.ensure(sizeof(int) * (count + 1))) != hipSuccess ||
-      (e = b->d_counters.ensure(sizeof(int) * 8)) != hipSuccess || (e = b->d_ranges.ensure(sizeof(int) * 9)) != hipSuccess)
+      (e = b->d_counters.ensure(sizeof(int) * 8)) != hipSuccess)
     return vr_hip_fail(ctx, e, "hipMalloc");
   if ((e = hipMemcpyAsync(b->d_in.p, vin.data(), sizeof(GridView) * count, hipMemcpyHostToDevice, ctx->stream)) !=
           hipSuccess ||
@@ -153,10 +246,10 @@ extern "C" int vr_batch_create(vr_ctx* ctx, vr_grid* const* grids, const vr_velo
           hipSuccess ||
       (e = hipMemcpyAsync(b->d_leaf_base.p, b->leaf_base.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice,
                           ctx->stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(b->d_ranges.p, ranges.data(), sizeof(int) * 9, hipMemcpyHostToDevice, ctx->stream)) !=
-          hipSuccess ||
       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
     return vr_hip_fail(ctx, e, "hipMemcpy");
+  int rc = batch_chunks(ctx, b.get());
+  if (rc != VR_OK) return rc;
   *out = b.release();
   return VR_OK;
 }
@@ -182,30 +275,35 @@ extern "C" int vr_batch_set_dt(vr_ctx* ctx, vr_batch* b, const double* dts) {
 }
 
 extern "C" int vr_batch_set_tuning(vr_batch* b, int leaves_per_grab, int waves_per_cu) {
-  if (!b || leaves_per_grab < 1 || waves_per_cu < 1 || waves_per_cu > 64) return VR_ERR_INVALID_ARGUMENT;
+  if (!b || leaves_per_grab < 1 || waves_per_cu < 0 || waves_per_cu > 64) return VR_ERR_INVALID_ARGUMENT;
   b->lpg = leaves_per_grab;
   b->wpc = waves_per_cu;
   return VR_OK;
 }
 
 static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int scheme) {
+  if (b->chunk_lpg != b->lpg) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "chunk table out of date");
   double tf[3];
   // one field per grid; time factors from the first field's period (uniform fields use none)
   vr_stage_factors(b->vel[0]->period, t, dt, scheme, tf);
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, ctx->stream, b->d_counters.as<int>());
-  const int blocks = b->cus * b->wpc;
+  // wpc > 0: persistent, wpc waves per CU; wpc == 0: one block per chunk.
+  const int nw = vr_advect_waves(b->ltot, b->cus);
+  int blocks = b->wpc > 0 ? b->cus * b->wpc / nw : 8 * b->max_range;
+  if (blocks < 8) blocks = 8;
+  const int persistent = b->wpc > 0 ? 1 : 0;
   double bytes = 2.0 * VR_LEAF_BYTES * (double)b->ltot;
   {
     TimedLaunch tl(ctx, TC_ADVECT, bytes);
     const double* dts = b->has_dts ? b->d_dts.as<double>() : nullptr;
     if (scheme == VR_SCHEME_RK3)
-      hipLaunchKernelGGL(k_batch_advect<3>, dim3(blocks), dim3(64), 0, ctx->stream, b->d_in.as<GridView>(),
-                         b->d_vel.as<VelView>(), b->d_outs.as<NanoLeaf*>(), b->d_leaf_base.as<int>(), b->G,
-                         b->d_counters.as<int>(), b->d_ranges.as<int>(), b->lpg, dts, dt, tf[0], tf[1], tf[2]);
+      launch_batch_s<3>(b->vk, nw, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
+                        b->d_outs.as<NanoLeaf*>(), b->d_chunks.as<int4>(), b->d_counters.as<int>(),
+                        b->d_ranges.as<int>(), persistent, dts, dt, tf);
     else
-      hipLaunchKernelGGL(k_batch_advect<1>, dim3(blocks), dim3(64), 0, ctx->stream, b->d_in.as<GridView>(),
-                         b->d_vel.as<VelView>(), b->d_outs.as<NanoLeaf*>(), b->d_leaf_base.as<int>(), b->G,
-                         b->d_counters.as<int>(), b->d_ranges.as<int>(), b->lpg, dts, dt, tf[0], tf[1], tf[2]);
+      launch_batch_s<1>(b->vk, nw, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
+                        b->d_outs.as<NanoLeaf*>(), b->d_chunks.as<int4>(), b->d_counters.as<int>(),
+                        b->d_ranges.as<int>(), persistent, dts, dt, tf);
   }
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "batch advect");
@@ -266,6 +364,10 @@ extern "C" int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int 
     return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "scheme must be 1 or 3");
   hipError_t e;
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
+  if (mode != VR_BATCH_LOOP) {
+    const int rc = batch_chunks(ctx, b);
+    if (rc != VR_OK) return rc;
+  }
   if (mode == VR_BATCH_LOOP) {
     // The per-grid path: one advect launch and one marching-cubes pipeline per grid.
     if (b->loop_mesh.size() != (size_t)b->G) {
@@ -339,7 +441,7 @@ extern "C" int vr_batch_autotune(vr_ctx* ctx, vr_batch* b, double dt, int scheme
   if ((e = hipEventCreate(&a)) != hipSuccess || (e = hipEventCreate(&z)) != hipSuccess)
     return vr_hip_fail(ctx, e, "hipEventCreate");
   const int lpgs[] = {1, 2, 4, 8, 16};
-  const int wpcs[] = {2, 4, 8, 16, 32};
+  const int wpcs[] = {0, 4, 8, 16, 32};  // 0: one block per chunk
   float best = 1e30f;
   int bl = b->lpg, bw = b->wpc;
   const bool timing = ctx->timing;
@@ -349,7 +451,8 @@ extern "C" int vr_batch_autotune(vr_ctx* ctx, vr_batch* b, double dt, int scheme
     for (int wp : wpcs) {
       b->lpg = lp;
       b->wpc = wp;
-      rc = batch_advect(ctx, b, dt, 0.0, scheme);  // warm
+      rc = batch_chunks(ctx, b);
+      if (rc == VR_OK) rc = batch_advect(ctx, b, dt, 0.0, scheme);  // warm
       (void)hipEventRecord(a, ctx->stream);
       for (int r = 0; r < 3 && rc == VR_OK; ++r) rc = batch_advect(ctx, b, dt, 0.0, scheme);
       (void)hipEventRecord(z, ctx->stream);
@@ -367,6 +470,7 @@ extern "C" int vr_batch_autotune(vr_ctx* ctx, vr_batch* b, double dt, int scheme
   (void)hipEventDestroy(z);
   b->lpg = bl;
   b->wpc = bw;
+  if (rc == VR_OK) rc = batch_chunks(ctx, b);
   if (best_lpg) *best_lpg = bl;
   if (best_wpc) *best_wpc = bw;
   return rc;

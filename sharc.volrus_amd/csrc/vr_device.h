@@ -31,13 +31,24 @@ struct __attribute__((aligned(8))) NanoLeaf {
 static_assert(sizeof(NanoLeaf) == VR_LEAF_BYTES, "NanoLeaf must be the 2128-byte NanoGrid record");
 
 // Read-only view of a grid passed to kernels by value.
+// Grid arrays always live in global memory.  Views are also read from
+// device memory (batches, multi-grid marching cubes), where a plain pointer
+// would compile to FLAT accesses (which also wait on the LDS counter); in the
+// device pass the pointers are typed global.  The host pass sees plain
+// pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define VR_GLOBAL __attribute__((address_space(1)))
+#else
+#define VR_GLOBAL
+#endif
+
 struct GridView {
-  const NanoLeaf* leaves;
-  const int* count;
-  const int4* origins;
-  const int4* roots;
-  const int* child;
-  const int* nbr;
+  const VR_GLOBAL NanoLeaf* leaves;
+  const VR_GLOBAL int* count;
+  const VR_GLOBAL int4* origins;
+  const VR_GLOBAL int4* roots;
+  const VR_GLOBAL int* child;
+  const VR_GLOBAL int* nbr;
   int root_count;
   float background;
   double vs;        // voxel_size of the uniform transform
@@ -177,7 +188,7 @@ struct VelView {
   int kind;
   double a[6];           // analytic parameters
   GridView g[3];         // VK_GRIDS3: vx, vy, vz
-  const float* dense;    // VK_DENSE: 3 * n^3, component-major, (i * n + j) * n + k
+  const VR_GLOBAL float* dense;  // VK_DENSE: 3 * n^3, component-major, (i * n + j) * n + k
   int n;
   double inv_h;          // 1.0 / node spacing
 };

@@ -66,6 +66,7 @@ struct TimingRec {
 
 struct vr_ctx {
   int device = 0;
+  int cus = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   std::string err;
   DevBuf scratch[SCR_COUNT];

@@ -14,8 +14,13 @@ hipError_t vr_build_nbr(const GridView& g, int cap, int* nbr, hipStream_t s);
 size_t vr_scan_scratch_ints(size_t n);
 hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, int* total, hipStream_t s);
 
+// nw = waves per leaf (1 or 4), see vr_advect_waves.
 hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* out, int cap, double dt,
-                            const double tf[3], int scheme, hipStream_t s);
+                            const double tf[3], int scheme, int nw, hipStream_t s);
+// Waves per leaf for an advect over `leaves` leaves on `cus` compute units:
+// one (fewest instructions per leaf) once there are enough leaves to give
+// every CU 16 waves, else four (a leaf finishes sooner).
+inline int vr_advect_waves(long long leaves, int cus) { return leaves >= 16LL * (cus > 0 ? cus : 256) ? 1 : 4; }
 hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, hipStream_t s);
 hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, NanoLeaf* out,
                              int* keep, int R, float vs, float band, hipStream_t s);

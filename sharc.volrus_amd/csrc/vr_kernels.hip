@@ -65,6 +65,35 @@ __device__ __forceinline__ int block_excl_scan_256(int v, int* s_wave, int* tota
   return off + incl - v;
 }
 
+// Small inputs (n <= VR_SCAN_SINGLE_MAX) in one launch: one block of 1024
+// threads, each scanning a contiguous run of ceil(n / 1024) elements.
+#define VR_SCAN_SINGLE_MAX 32768
+__global__ __launch_bounds__(1024) void k_scan_single(const int* in, int* out, int n, int* total) {
+  __shared__ int s_wave[16];
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  const int C = (n + 1023) >> 10;
+  const int beg = threadIdx.x * C;
+  const int end = beg + C < n ? beg + C : n;
+  int sum = 0;
+  for (int i = beg; i < end; ++i) sum += in[i];
+  const int incl = wave_incl_scan(sum);
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  int off = incl - sum, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 16; ++w) {
+    const int ws = s_wave[w];
+    if (w < wave) off += ws;
+    tot += ws;
+  }
+  for (int i = beg; i < end; ++i) {
+    const int v = in[i];
+    out[i] = off;
+    off += v;
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
 __global__ __launch_bounds__(256) void k_scan_blocks(const int* in, int* out, int* block_sums, size_t n) {
   __shared__ int s_wave[4];
   const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x * 4;
@@ -106,78 +135,39 @@ __global__ void k_scan_add(int* out, const int* sums, size_t n) {
 }
 
 // ------------------------------------------------------------------ advection
-// Slab variant: lane = (y, z), the wave walks the 8 x-slabs; inactive lanes
-// idle while active ones compute.
-template <int VK, int SCHEME>
-__global__ __launch_bounds__(64) void k_advect_slab(GridView in, VelView vel, NanoLeaf* out, double dt, double tf0,
+// One leaf per block of NW waves: the active voxels are listed in LDS by a
+// ballot/popcount prefix over the 8 mask words and spread over the block's
+// lanes, and the leaf is then written back slab by slab, coalesced, with the
+// background in the inactive positions.  NW = 1 issues the fewest
+// instructions per leaf (dense passes of 64 active voxels); NW = 4 finishes
+// a leaf sooner, for grids with too few leaves to fill the device.
+template <int VK, int SCHEME, int NW>
+__global__ __launch_bounds__(64 * NW) void k_advect(GridView in, VelView vel, NanoLeaf* out, double dt, double tf0,
                                                     double tf1, double tf2) {
-  __shared__ float brick[1000];
-  __shared__ int nb[VR_NBR];
-  const int lane = threadIdx.x;
-  const int n = *in.count;
-  const double tf[3] = {tf0, tf1, tf2};
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
-    const NanoLeaf& L = in.leaves[li];
-    if (lane < VR_NBR) nb[lane] = in.nbr[(size_t)li * VR_NBR + lane];
-    __syncthreads();
-    stage_brick_dev(in, nb, brick, lane);
-    __syncthreads();
-    const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
-    const int y = lane >> 3, z = lane & 7;
-    NanoLeaf& O = out[li];
-    if (lane < 3) O.origin[lane] = L.origin[lane];
-    if (lane == 3) O.pad = 0;
-    if (lane < 8) O.mask[lane] = L.mask[lane];
-    for (int x = 0; x < 8; ++x) {
-      const bool act = (L.mask[x] >> lane) & 1ull;
-      float res = in.background;
-      if (act) res = advect_voxel<VK, SCHEME>(in, vel, brick, ox, oy, oz, x, y, z, dt, tf);
-      O.values[x * 64 + lane] = res;
-    }
-    __syncthreads();
-  }
-}
-
-// Compacted variant: one block of VR_ADVECT_WAVES waves per leaf; the active
-// voxels are listed in LDS by a ballot/popcount prefix over the 8 mask words
-// and spread over the block's lanes, and the leaf is then written back slab
-// by slab, coalesced, with the background in the inactive positions.
-#ifndef VR_ADVECT_WAVES
-#define VR_ADVECT_WAVES 4
-#endif
-template <int VK, int SCHEME>
-__global__ __launch_bounds__(64 * VR_ADVECT_WAVES) void k_advect(GridView in, VelView vel, NanoLeaf* out, double dt,
-                                                                 double tf0, double tf1, double tf2) {
   __shared__ AdvectLDS s;
   const int n = *in.count;
   const double tf[3] = {tf0, tf1, tf2};
   for (int li = blockIdx.x; li < n; li += gridDim.x)
-    advect_leaf<VK, SCHEME, VR_ADVECT_WAVES>(in, vel, out, li, s, threadIdx.x, dt, tf);
+    advect_leaf<VK, SCHEME, NW>(in, vel, out, li, s, threadIdx.x, dt, tf);
 }
 
 template <int VK, int SCHEME>
 static hipError_t launch_advect_t(const GridView& in, const VelView& vel, NanoLeaf* out, int cap, double dt,
-                                  const double tf[3], hipStream_t s) {
+                                  const double tf[3], int nw, hipStream_t s) {
   const int blocks = cap < 1 ? 1 : (cap < 65536 ? cap : 65536);
-  static const int variant = [] {
-    const char* e = getenv("VOLRUS_ADVECT_VARIANT");
-    return (e && e[0] == 's') ? 1 : 0;
-  }();
-  if (variant == 1)
-    hipLaunchKernelGGL((k_advect_slab<VK, SCHEME>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1],
-                       tf[2]);
+  if (nw == 1)
+    hipLaunchKernelGGL((k_advect<VK, SCHEME, 1>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1], tf[2]);
   else
-    hipLaunchKernelGGL((k_advect<VK, SCHEME>), dim3(blocks), dim3(64 * VR_ADVECT_WAVES), 0, s, in, vel, out, dt, tf[0],
-                       tf[1], tf[2]);
+    hipLaunchKernelGGL((k_advect<VK, SCHEME, 4>), dim3(blocks), dim3(256), 0, s, in, vel, out, dt, tf[0], tf[1], tf[2]);
   return hipGetLastError();
 }
 
 hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* out, int cap, double dt,
-                            const double tf[3], int scheme, hipStream_t s) {
+                            const double tf[3], int scheme, int nw, hipStream_t s) {
 #define VR_ADV(VK)                                                                         \
   case VK:                                                                                 \
-    return scheme == 3 ? launch_advect_t<VK, 3>(in, vel, out, cap, dt, tf, s)               \
-                       : launch_advect_t<VK, 1>(in, vel, out, cap, dt, tf, s);
+    return scheme == 3 ? launch_advect_t<VK, 3>(in, vel, out, cap, dt, tf, nw, s)           \
+                       : launch_advect_t<VK, 1>(in, vel, out, cap, dt, tf, nw, s);
   switch (vel.kind) {
     VR_ADV(VK_GRIDS3)
     VR_ADV(VK_DENSE)
@@ -427,6 +417,10 @@ size_t vr_scan_scratch_ints(size_t n) { return (n + 1023) / 1024 + 1; }
 hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, int* total, hipStream_t s) {
   const int nb = (int)((n + 1023) / 1024);
   if (nb == 0) return total ? vr_fill_int(total, 0, 1, s) : hipSuccess;
+  if (n <= VR_SCAN_SINGLE_MAX) {
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, in, out, (int)n, total);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, in, out, scratch, n);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, scratch, nb, total);
   hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, scratch, n);

@@ -24,25 +24,83 @@
 #include "vr_internal.h"
 
 // ------------------------------------------------------------------ tables
-__constant__ uint16_t c_mc_edge_mask[256];
-__constant__ uint8_t c_mc_ntri[256];
-__constant__ int8_t c_mc_tri[256][16];
-__constant__ int8_t c_mc_corner[8][3];
-__constant__ int8_t c_mc_edge[12][2];
+// Per case: crossing-edge mask | triangle count << 12.
+__constant__ uint16_t c_mc_emn[256];
+// Per case: TRI_TABLE row, 16 int8 (-1 padded), as 4 words.
+__constant__ uint32_t c_mc_tri4[256 * 4];
 // For local edge e and candidate j (0..3): offset (dx, dy, dz) of a cell that
-// contains the same lattice edge, and that edge's local number there.
-__constant__ int8_t c_mc_cand[12][4][4];
+// contains the same lattice edge and that edge's local number there, packed
+// as 4 int8 (dx, dy, dz, e').
+__constant__ uint32_t c_mc_cand4[12 * 4];
+// Per local edge: corner a | corner b << 3 | a's offset bits << 6 | b's << 9
+// (offset bits x | y << 1 | z << 2), the edge's reference orientation.
+__constant__ uint32_t c_mc_edgeinfo[12];
+// Candidate of each local edge that is the cell itself.
+__constant__ int c_mc_self[12];
+// Per cell class (27: per axis local 0 / 1..6 / 7) and local edge: the four
+// candidates in ascending visiting-key order, one byte each: j << 5 | the
+// neighbour-tile index (0..26) that candidate lies in.
+__constant__ uint32_t c_mc_ownsort[27 * 12];
 __constant__ int c_dir7[7][3] = {{0, 0, 0}, {-8, 0, 0}, {8, 0, 0}, {0, -8, 0}, {0, 8, 0}, {0, 0, -8}, {0, 0, 8}};
 
+// Uploads the tables derived from the reference's EDGE_TABLE / TRI_TABLE
+// (volume_to_mesh.rs:6-44) and its corner/edge numbering.
 hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, const int8_t* tri,
                                const int8_t* corner, const int8_t* edge, const int8_t* cand) {
+  uint16_t emn[256];
+  uint32_t tri4[256 * 4];
+  for (int c = 0; c < 256; ++c) {
+    emn[c] = (uint16_t)(edge_mask[c] | (ntri[c] << 12));
+    for (int w = 0; w < 4; ++w) {
+      uint32_t v = 0;
+      for (int k = 0; k < 4; ++k) v |= (uint32_t)(uint8_t)tri[c * 16 + 4 * w + k] << (8 * k);
+      tri4[c * 4 + w] = v;
+    }
+  }
+  uint32_t cand4[48], einfo[12], ownsort[27 * 12];
+  int self[12];
+  for (int e = 0; e < 12; ++e) {
+    self[e] = 0;
+    for (int j = 0; j < 4; ++j) {
+      const int8_t* q = cand + (e * 4 + j) * 4;
+      uint32_t v = 0;
+      for (int k = 0; k < 4; ++k) v |= (uint32_t)(uint8_t)q[k] << (8 * k);
+      cand4[e * 4 + j] = v;
+      if (q[0] == 0 && q[1] == 0 && q[2] == 0) self[e] = j;
+    }
+    const int ca = edge[2 * e], cb = edge[2 * e + 1];
+    const uint32_t ab = (uint32_t)(corner[3 * ca] | (corner[3 * ca + 1] << 1) | (corner[3 * ca + 2] << 2));
+    const uint32_t bb = (uint32_t)(corner[3 * cb] | (corner[3 * cb + 1] << 1) | (corner[3 * cb + 2] << 2));
+    einfo[e] = (uint32_t)ca | ((uint32_t)cb << 3) | (ab << 6) | (bb << 9);
+  }
+  const int rep[3] = {0, 3, 7};
+  for (int c = 0; c < 27; ++c)
+    for (int e = 0; e < 12; ++e) {
+      const int l[3] = {rep[c / 9], rep[(c / 3) % 3], rep[c % 3]};
+      int key[4], tile[4], ord[4] = {0, 1, 2, 3};
+      for (int j = 0; j < 4; ++j) {
+        const int8_t* q = cand + (e * 4 + j) * 4;
+        const int cx = l[0] + q[0], cy = l[1] + q[1], cz = l[2] + q[2];
+        tile[j] = ((cx >> 3) + 1) * 9 + ((cy >> 3) + 1) * 3 + ((cz >> 3) + 1);
+        key[j] = (tile[j] << 9) + ((cx & 7) << 6) + ((cy & 7) << 3) + (cz & 7);
+      }
+      for (int i = 1; i < 4; ++i)
+        for (int k = i; k > 0 && key[ord[k]] < key[ord[k - 1]]; --k) {
+          const int tmp = ord[k];
+          ord[k] = ord[k - 1];
+          ord[k - 1] = tmp;
+        }
+      uint32_t v = 0;
+      for (int k = 0; k < 4; ++k) v |= (uint32_t)((ord[k] << 5) | tile[ord[k]]) << (8 * k);
+      ownsort[c * 12 + e] = v;
+    }
   hipError_t e;
-  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_edge_mask), edge_mask, 256 * sizeof(uint16_t))) != hipSuccess) return e;
-  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_ntri), ntri, 256)) != hipSuccess) return e;
-  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_tri), tri, 256 * 16)) != hipSuccess) return e;
-  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_corner), corner, 24)) != hipSuccess) return e;
-  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_edge), edge, 24)) != hipSuccess) return e;
-  return hipMemcpyToSymbol(HIP_SYMBOL(c_mc_cand), cand, 12 * 4 * 4);
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_emn), emn, sizeof(emn))) != hipSuccess) return e;
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_tri4), tri4, sizeof(tri4))) != hipSuccess) return e;
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_cand4), cand4, sizeof(cand4))) != hipSuccess) return e;
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_edgeinfo), einfo, sizeof(einfo))) != hipSuccess) return e;
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_self), self, sizeof(self))) != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_mc_ownsort), ownsort, sizeof(ownsort));
 }
 
 // ------------------------------------------------------------------ helpers
@@ -141,44 +199,37 @@ __global__ void k_mc_tile_nbr(const GridView* views, const int4* tiles, const in
 }
 
 // ------------------------------------------------------------------ per-cell rules
-// Owner of the lattice edge carried by local edge e of the cell at local
-// (lx, ly, lz): the candidate j (0..3) with the smallest visiting key
-// (tile origin, lx, ly, lz) among the cells of processed tiles.
-//
-// The key is one integer: the neighbour-tile index (tx+1)*9 + (ty+1)*3 +
-// (tz+1), whose order is the lexicographic order of tile origins around the
-// cell's tile, times 512, plus the local offset lx*64 + ly*8 + lz.
-__device__ __forceinline__ int mc_owner(int e, int lx, int ly, int lz, uint32_t processed) {
-  int best = -1, bkey = 0x7fffffff;
+// Owner of a lattice edge: among the four cells that contain it, the one of a
+// processed tile with the smallest visiting key (tile origin, lx, ly, lz).
+// Within a tile the key order of the candidates of (class, edge) is fixed,
+// so per tile the owner of every (class, edge) is the first candidate in
+// c_mc_ownsort whose tile is processed.
+__device__ __forceinline__ int mc_cls(int l) { return l == 0 ? 0 : (l == 7 ? 2 : 1); }
+__device__ __forceinline__ int mc_cell_class(int x, int y, int z) { return mc_cls(x) * 9 + mc_cls(y) * 3 + mc_cls(z); }
+
+__device__ __forceinline__ int mc_first_processed(uint32_t w, uint32_t processed) {
+  int j = -1;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int cx = lx + c_mc_cand[e][j][0], cy = ly + c_mc_cand[e][j][1], cz = lz + c_mc_cand[e][j][2];
-    const int t = ((cx >> 3) + 1) * 9 + ((cy >> 3) + 1) * 3 + ((cz >> 3) + 1);
-    const int key = (t << 9) + ((cx & 7) << 6) + ((cy & 7) << 3) + (cz & 7);
-    const bool ok = (processed >> t) & 1u;
-    best = (ok && key < bkey) ? j : best;
-    bkey = (ok && key < bkey) ? key : bkey;
+  for (int k = 3; k >= 0; --k) {
+    const uint32_t b = (w >> (8 * k)) & 0xFFu;
+    if ((processed >> (b & 31u)) & 1u) j = (int)(b >> 5);
   }
-  return best;
+  return j;
 }
 
-__device__ __forceinline__ int mc_self_cand(int e) {
-  for (int j = 0; j < 4; ++j)
-    if (c_mc_cand[e][j][0] == 0 && c_mc_cand[e][j][1] == 0 && c_mc_cand[e][j][2] == 0) return j;
-  return 0;
-}
-
-// Stage the 9^3 corner values of a tile (local 0..8) into LDS.
-__device__ __forceinline__ void mc_stage(const GridView& g, const int* tleaf_t, float* brick) {
-  for (int i = threadIdx.x; i < 729; i += 64) {
-    const int bx = i / 81, by = (i / 9) % 9, bz = i % 9;
-    const int d = ((bx >> 3) + 1) * 9 + ((by >> 3) + 1) * 3 + ((bz >> 3) + 1);
-    const int src = tleaf_t[d];
-    brick[i] = src < 0 ? g.background : g.leaves[src].values[voxel_offset(bx, by, bz)];
+// s_self[c] = edges that a cell of class c owns itself (tile-specific);
+// s_ownsort is c_mc_ownsort staged in LDS.
+__device__ __forceinline__ void mc_build_self(uint16_t* s_self, const uint32_t* s_ownsort, uint32_t processed,
+                                              int lane) {
+  if (lane < 27) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int e = 0; e < 12; ++e)
+      if (mc_first_processed(s_ownsort[lane * 12 + e], processed) == c_mc_self[e]) m |= 1u << e;
+    s_self[lane] = (uint16_t)m;
   }
 }
 
-// ------------------------------------------------------------------ shared per-cell pass
 // Index of a cell among the 169 cells of a tile with a local coordinate 7
 // (-1 for the other 343).  An owner in another tile always has one: its tile
 // precedes the cell's, so its first differing tile offset is -1 and its local
@@ -191,65 +242,193 @@ __device__ __forceinline__ int mc_face_index(int lx, int ly, int lz) {
 }
 #define MC_FACE 169
 
-__device__ __forceinline__ uint32_t mc_case(const float* brick, int x, int y, int z, float iso, float vals[8]) {
-  uint32_t cube = 0;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    vals[c] = brick[(x + c_mc_corner[c][0]) * 81 + (y + c_mc_corner[c][1]) * 9 + (z + c_mc_corner[c][2])];
-    if (vals[c] < iso) cube |= 1u << c;
-  }
-  return cube;
+// Cube index of cell (x, y, z): bit c set when corner c is below iso
+// (volume_to_mesh.rs:115-123).
+__device__ __forceinline__ uint32_t mc_case(const float* brick, int x, int y, int z, float iso) {
+  const int b = x * 81 + y * 9 + z;
+  // corners in the reference's order: (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1)
+  const float v0 = brick[b], v1 = brick[b + 81], v2 = brick[b + 90], v3 = brick[b + 9];
+  const float v4 = brick[b + 1], v5 = brick[b + 82], v6 = brick[b + 91], v7 = brick[b + 10];
+  return (uint32_t)(v0 < iso) | ((uint32_t)(v1 < iso) << 1) | ((uint32_t)(v2 < iso) << 2) |
+         ((uint32_t)(v3 < iso) << 3) | ((uint32_t)(v4 < iso) << 4) | ((uint32_t)(v5 < iso) << 5) |
+         ((uint32_t)(v6 < iso) << 6) | ((uint32_t)(v7 < iso) << 7);
 }
 
-__device__ __forceinline__ uint32_t mc_owned_mask(uint32_t em, int x, int y, int z, uint32_t processed) {
-  uint32_t owned = 0;
-  for (int e = 0; e < 12; ++e) {
-    if (!((em >> e) & 1u)) continue;
-    if (mc_owner(e, x, y, z, processed) == mc_self_cand(e)) owned |= 1u << e;
+// ------------------------------------------------------------------ per-tile block helpers
+// Both passes run one tile per block of MC_THREADS threads (4 waves); the
+// 512 cells of a tile are visited in two halves of 256 (cell = half * 256 +
+// tid, i.e. x = wave + 4 half, y = lane / 8, z = lane % 8), so prefix sums in
+// visiting order are block scans.
+#define MC_THREADS 256
+
+// Exclusive block scan of v over the 256 threads (visiting order = tid).
+__device__ __forceinline__ int mc_block_scan(int v, int* s_w, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int incl = wave_incl_scan(v);
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  const int w0 = s_w[0], w1 = s_w[1], w2 = s_w[2], w3 = s_w[3];
+  __syncthreads();
+  *total = w0 + w1 + w2 + w3;
+  const int before = (wave > 0 ? w0 : 0) + (wave > 1 ? w1 : 0) + (wave > 2 ? w2 : 0);
+  return before + incl - v;
+}
+
+// Neighbour lists of a tile, held in registers of wave 0 (lanes < 27) so the
+// next tile's lists load while the current tile is processed.
+struct MCNext {
+  int tl = -1, tt = -1;
+};
+
+__device__ __forceinline__ MCNext mc_fetch(const int* tleaf, const int* ttile, int t, int nt) {
+  MCNext n;
+  if (threadIdx.x < VR_NBR && t < nt) {
+    n.tl = tleaf[(size_t)t * VR_NBR + threadIdx.x];
+    n.tt = ttile[(size_t)t * VR_NBR + threadIdx.x];
   }
-  return owned;
+  return n;
+}
+
+// Publishes fetched lists into LDS and the processed-neighbour mask into
+// *s_proc (wave 0).  Caller synchronises.
+__device__ __forceinline__ void mc_publish(const MCNext& n, int* s_tleaf, int* s_ttile, uint32_t* s_proc) {
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    if (tid < VR_NBR) {
+      s_tleaf[tid] = n.tl;
+      s_ttile[tid] = n.tt;
+    }
+    const uint32_t pm = (uint32_t)__ballot(tid < VR_NBR && n.tt >= 0);
+    if (tid == 0) *s_proc = pm;
+  }
+}
+
+// Stages the 9^3 corner values of a tile (local 0..8) into LDS and records
+// whether they lie on both sides of iso (some value < iso and some not): only
+// then can a cell of the tile have a case other than 0 or 255.  s_side[0]
+// and s_side[1] must be 0 on entry; read them after a synchronisation.
+__device__ __forceinline__ void mc_stage(const GridView& g, const int* s_tleaf, float* brick, float iso,
+                                         int* s_side) {
+  bool lt = false, ge = false;
+  for (int i = threadIdx.x; i < 729; i += MC_THREADS) {
+    const int bx = i / 81, by = (i / 9) % 9, bz = i % 9;
+    const int d = ((bx >> 3) + 1) * 9 + ((by >> 3) + 1) * 3 + ((bz >> 3) + 1);
+    const int src = s_tleaf[d];
+    const float v = src < 0 ? g.background : g.leaves[src].values[voxel_offset(bx, by, bz)];
+    brick[i] = v;
+    if (v < iso) lt = true; else ge = true;
+  }
+  if (__any(lt) && (threadIdx.x & 63) == 0) s_side[0] = 1;
+  if (__any(ge) && (threadIdx.x & 63) == 0) s_side[1] = 1;
+}
+
+// Global index of the vertex on local edge e of cell (x, y, z) of tile t,
+// relative to the tile's grid: its owner's meta gives the owner cell's first
+// vertex and owned-edge mask; -1 if no owner (not reached for edges of a
+// crossing cell of a processed tile).
+__device__ __forceinline__ int mc_edge_vertex(int x, int y, int z, int e, int cls, const int8_t* s_own,
+                                              const uint32_t* s_cand, const uint32_t* s_meta, const int* s_ttile,
+                                              const uint32_t* meta, const int* vbase, int vb, int vrel) {
+  const int j = s_own[cls * 12 + e];
+  if (j < 0) return 0;
+  const uint32_t cw = s_cand[e * 4 + j];
+  const int nx = x + (int)(int8_t)(cw & 0xFFu), ny = y + (int)(int8_t)((cw >> 8) & 0xFFu),
+            nz = z + (int)(int8_t)((cw >> 16) & 0xFFu);
+  const int ep = (int)(int8_t)(cw >> 24);
+  const int d = ((nx >> 3) + 1) * 9 + ((ny >> 3) + 1) * 3 + ((nz >> 3) + 1);
+  uint32_t om;
+  int obase;
+  if (d == 13) {
+    om = s_meta[(nx << 6) + (ny << 3) + nz];
+    obase = vb;
+  } else {
+    const int ot = s_ttile[d];
+    const int fi = mc_face_index(nx & 7, ny & 7, nz & 7);
+    if (ot < 0 || fi < 0) return 0;
+    om = meta[(size_t)ot * MC_FACE + fi];
+    obase = vbase[ot];
+  }
+  return obase - vrel + (int)(om >> 12) + __popc((om & 0xFFFu) & ((1u << ep) - 1u));
+}
+
+// Vertex on owned local edge e of cell (x, y, z) (volume_to_mesh.rs:162-186):
+// t = (iso - va) / (vb - va) in f32 (0.5 when |vb - va| <= 1e-10), position
+// pa + t (pb - pa) in f64 from the corners' world positions (index * vs).
+__device__ __forceinline__ void mc_write_vertex(const float* brick, int x, int y, int z, int e, const int4& to,
+                                                double vs, float iso, const uint32_t* s_einfo, double* out) {
+  const uint32_t ei = s_einfo[e];
+  const int ao = (int)((ei >> 6) & 7u), bo = (int)((ei >> 9) & 7u);
+  const int bb = x * 81 + y * 9 + z;
+  const float va = brick[bb + (ao & 1) * 81 + ((ao >> 1) & 1) * 9 + (ao >> 2)];
+  const float vbv = brick[bb + (bo & 1) * 81 + ((bo >> 1) & 1) * 9 + (bo >> 2)];
+  double tpar;
+  if (fabsf(vbv - va) > 1e-10f)
+    tpar = (double)((iso - va) / (vbv - va));
+  else
+    tpar = 0.5;
+  const double pax = (double)(to.x + x + (ao & 1)) * vs, pay = (double)(to.y + y + ((ao >> 1) & 1)) * vs,
+               paz = (double)(to.z + z + (ao >> 2)) * vs;
+  const double pbx = (double)(to.x + x + (bo & 1)) * vs, pby = (double)(to.y + y + ((bo >> 1) & 1)) * vs,
+               pbz = (double)(to.z + z + (bo >> 2)) * vs;
+  out[0] = pax + tpar * (pbx - pax);
+  out[1] = pay + tpar * (pby - pay);
+  out[2] = paz + tpar * (pbz - paz);
+}
+
+// Edge k (0..14) of a case's TRI_TABLE row.
+__device__ __forceinline__ int mc_tri_edge(uint32_t cube, int k) {
+  return (int)(int8_t)((c_mc_tri4[cube * 4 + (k >> 2)] >> (8 * (k & 3))) & 0xFFu);
 }
 
 // ------------------------------------------------------------------ pass 1
 // Per cell: case, owned crossing edges, triangle count.  The face cells
 // (mc_face_index >= 0) with crossing edges store meta = (vertex prefix inside
 // the tile << 12) | owned-edge mask for the emit pass of later tiles; per-tile
-// vertex and triangle totals.
-__global__ __launch_bounds__(64) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
-                                                 const int* tleaf, const int* ttile, float iso, uint32_t* meta,
-                                                 int* tv, int* tt) {
+// vertex and triangle totals.  Blocks loop over tiles.
+__global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
+                                                         const int* tleaf, const int* ttile, float iso,
+                                                         uint32_t* meta, int* tv, int* tt) {
   __shared__ float brick[729];
+  __shared__ uint16_t s_emn[256];
+  __shared__ uint16_t s_self[27];
   __shared__ int s_tleaf[VR_NBR];
   __shared__ int s_ttile[VR_NBR];
-  const int lane = threadIdx.x;
+  __shared__ uint32_t s_proc;
+  __shared__ int s_side[2];
+  __shared__ int s_w[4];
+  __shared__ uint32_t s_ownsort[27 * 12];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
+  for (int i = tid; i < 27 * 12; i += MC_THREADS) s_ownsort[i] = c_mc_ownsort[i];
   const int nt = *ntiles;
-  const int y = lane >> 3, z = lane & 7;
+  const int wave = tid >> 6, y = (tid >> 3) & 7, z = tid & 7;
+  MCNext nx = mc_fetch(tleaf, ttile, blockIdx.x, nt);
   for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-    if (lane < VR_NBR) {
-      s_tleaf[lane] = tleaf[(size_t)t * VR_NBR + lane];
-      s_ttile[lane] = ttile[(size_t)t * VR_NBR + lane];
-    }
+    mc_publish(nx, s_tleaf, s_ttile, &s_proc);
+    nx = mc_fetch(tleaf, ttile, t + gridDim.x, nt);
+    if (tid < 2) s_side[tid] = 0;
     __syncthreads();
+    mc_build_self(s_self, s_ownsort, s_proc, tid);
     const GridView& g = views[tiles[t].w];
-    uint32_t processed = 0;
-    for (int d = 0; d < VR_NBR; ++d) processed |= (s_ttile[d] >= 0 ? 1u : 0u) << d;
-    mc_stage(g, s_tleaf, brick);
+    mc_stage(g, s_tleaf, brick, iso, s_side);
     __syncthreads();
     int vcarry = 0, tcarry = 0;
-    for (int x = 0; x < 8; ++x) {
-      float vals[8];
-      const uint32_t cube = mc_case(brick, x, y, z, iso, vals);
-      const uint32_t em = c_mc_edge_mask[cube];
-      const uint32_t owned = em ? mc_owned_mask(em, x, y, z, processed) : 0u;
-      const int nv = __popc(owned), ntr = c_mc_ntri[cube];
-      const int vin = wave_incl_scan(nv);
-      const int tin = wave_incl_scan(ntr);
-      const int fi = mc_face_index(x, y, z);
-      if (em && fi >= 0) meta[(size_t)t * MC_FACE + fi] = ((uint32_t)(vcarry + vin - nv) << 12) | owned;
-      vcarry += __shfl(vin, 63, 64);
-      tcarry += __shfl(tin, 63, 64);
+    if (s_side[0] && s_side[1]) {
+      for (int h = 0; h < 2; ++h) {
+        const int x = wave + 4 * h;
+        const uint32_t emn = s_emn[mc_case(brick, x, y, z, iso)];
+        const uint32_t em = emn & 0xFFFu;
+        const uint32_t owned = em & s_self[mc_cell_class(x, y, z)];
+        const int packed = __popc(owned) | (int)((emn >> 12) << 16);
+        int tot;
+        const int excl = mc_block_scan(packed, s_w, &tot);
+        const int fi = mc_face_index(x, y, z);
+        if (em && fi >= 0) meta[(size_t)t * MC_FACE + fi] = ((uint32_t)(vcarry + (excl & 0xFFFF)) << 12) | owned;
+        vcarry += tot & 0xFFFF;
+        tcarry += tot >> 16;
+      }
     }
-    if (lane == 0) {
+    if (tid == 0) {
       tv[t] = vcarry;
       tt[t] = tcarry;
     }
@@ -260,111 +439,124 @@ __global__ __launch_bounds__(64) void k_mc_count(const GridView* views, const in
 // ------------------------------------------------------------------ pass 2
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
-// The tile's own cell meta is rebuilt in LDS; owners in earlier tiles are
-// read from their face meta.  Vertex indices are relative to the tile's grid.
-__global__ __launch_bounds__(64) void k_mc_emit(const GridView* views, const int4* tiles, const int* ntiles,
-                                                const int* tleaf, const int* ttile, const int* tile_base, float iso,
-                                                const uint32_t* meta, const int* vbase, const int* tbase,
-                                                const int* totals, double* verts, uint32_t* tris) {
+// The tile's cell meta is rebuilt in LDS together with a list of its vertices
+// (cell, edge) and of its triangles (cell, row entry) in output order; the
+// lists are then spread over the threads, one vertex or triangle each, so
+// the writes are contiguous.  A tile with more vertices or triangles than
+// the lists hold is emitted cell by cell instead.  Owners in earlier tiles
+// are read from their face meta.  Vertex indices are relative to the tile's
+// grid.
+#define MC_VLIST 1536
+#define MC_TLIST 2048
+__global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, const int4* tiles, const int* ntiles,
+                                                        const int* tleaf, const int* ttile, const int* tile_base,
+                                                        float iso, const uint32_t* meta, const int* tv, const int* tt,
+                                                        const int* vbase, const int* tbase, const int* totals,
+                                                        double* verts, uint32_t* tris) {
   __shared__ float brick[729];
-  __shared__ uint32_t s_meta[512];
+  __shared__ uint32_t s_meta[512];  // (vertex prefix << 12) | owned edges
+  __shared__ uint16_t s_tpre[512];  // triangle prefix inside the tile
+  __shared__ uint8_t s_cube[512];
+  __shared__ uint16_t s_vlist[MC_VLIST];  // cell << 4 | edge
+  __shared__ uint16_t s_tlist[MC_TLIST];  // cell << 3 | triangle of the cell
+  __shared__ int8_t s_own[27 * 12];
+  __shared__ uint16_t s_self[27];
+  __shared__ uint16_t s_emn[256];
+  __shared__ uint32_t s_cand[48];
+  __shared__ uint32_t s_einfo[12];
   __shared__ int s_tleaf[VR_NBR];
   __shared__ int s_ttile[VR_NBR];
-  const int lane = threadIdx.x;
+  __shared__ uint32_t s_proc;
+  __shared__ int s_side[2];
+  __shared__ int s_w[4];
+  __shared__ uint32_t s_ownsort[27 * 12];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
+  for (int i = tid; i < 27 * 12; i += MC_THREADS) s_ownsort[i] = c_mc_ownsort[i];
+  if (tid < 48) s_cand[tid] = c_mc_cand4[tid];
+  if (tid < 12) s_einfo[tid] = c_mc_edgeinfo[tid];
   const int nt = *ntiles;
-  const int y = lane >> 3, z = lane & 7;
+  const int vmax = totals[1], tmax = totals[2];
+  const int wave = tid >> 6, y = (tid >> 3) & 7, z = tid & 7;
   for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-    if (lane < VR_NBR) {
-      s_tleaf[lane] = tleaf[(size_t)t * VR_NBR + lane];
-      s_ttile[lane] = ttile[(size_t)t * VR_NBR + lane];
-    }
+    if (tv[t] == 0 && tt[t] == 0) continue;  // nothing to write (uniform over the block)
+    mc_publish(mc_fetch(tleaf, ttile, t, nt), s_tleaf, s_ttile, &s_proc);
+    if (tid < 2) s_side[tid] = 0;
     __syncthreads();
+    const uint32_t processed = s_proc;
+    for (int i = tid; i < 27 * 12; i += MC_THREADS) s_own[i] = (int8_t)mc_first_processed(s_ownsort[i], processed);
+    mc_build_self(s_self, s_ownsort, processed, tid);
     const int4 to = tiles[t];
     const GridView& g = views[to.w];
     const double vs = g.vs;
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
-    uint32_t processed = 0;
-    for (int d = 0; d < VR_NBR; ++d) processed |= (s_ttile[d] >= 0 ? 1u : 0u) << d;
-    mc_stage(g, s_tleaf, brick);
+    const int vb = vbase[t], tb = tbase[t];
+    mc_stage(g, s_tleaf, brick, iso, s_side);
     __syncthreads();
-    {
-      int vcarry = 0;
-      for (int x = 0; x < 8; ++x) {
-        float vals[8];
-        const uint32_t em = c_mc_edge_mask[mc_case(brick, x, y, z, iso, vals)];
-        const uint32_t owned = em ? mc_owned_mask(em, x, y, z, processed) : 0u;
-        const int nv = __popc(owned);
-        const int vin = wave_incl_scan(nv);
-        s_meta[x * 64 + lane] = ((uint32_t)(vcarry + vin - nv) << 12) | owned;
-        vcarry += __shfl(vin, 63, 64);
-      }
+    // Pass A: per-cell meta, triangle prefix, case; vertex and triangle lists.
+    int vcarry = 0, tcarry = 0;
+    for (int h = 0; h < 2; ++h) {
+      const int x = wave + 4 * h;
+      const int c = h * 256 + tid;
+      const uint32_t cube = mc_case(brick, x, y, z, iso);
+      const uint32_t emn = s_emn[cube];
+      const uint32_t owned = (emn & 0xFFFu) & s_self[mc_cell_class(x, y, z)];
+      const int ntr = (int)(emn >> 12);
+      const int packed = __popc(owned) | (ntr << 16);
+      int tot;
+      const int excl = mc_block_scan(packed, s_w, &tot);
+      const int vp = vcarry + (excl & 0xFFFF), tp = tcarry + (excl >> 16);
+      s_meta[c] = ((uint32_t)vp << 12) | owned;
+      s_tpre[c] = (uint16_t)tp;
+      s_cube[c] = (uint8_t)cube;
+      int k = vp;
+      for (uint32_t m = owned; m; m &= m - 1u, ++k)
+        if (k < MC_VLIST) s_vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
+      for (int q = 0; q < ntr; ++q)
+        if (tp + q < MC_TLIST) s_tlist[tp + q] = (uint16_t)((c << 3) | q);
+      vcarry += tot & 0xFFFF;
+      tcarry += tot >> 16;
     }
     __syncthreads();
-    const int vb = vbase[t];
-    const int vmax = totals[1], tmax = totals[2];
-    int tcarry = tbase[t];
-    for (int x = 0; x < 8; ++x) {
-      float vals[8];
-      const uint32_t cube = mc_case(brick, x, y, z, iso, vals);
-      const uint32_t em = c_mc_edge_mask[cube];
-      const int ntr = c_mc_ntri[cube];
-      const int tin = wave_incl_scan(ntr);
-      if (em) {
-        const uint32_t m = s_meta[x * 64 + lane];
-        const uint32_t owned = m & 0xFFFu;
-        int vi = vb + (int)(m >> 12);
-        for (int e = 0; e < 12; ++e) {
-          if (!((owned >> e) & 1u)) continue;
-          if (vi >= vmax) break;
-          const int a = c_mc_edge[e][0], b = c_mc_edge[e][1];
-          const float va = vals[a], vbv = vals[b];
-          double tpar;
-          if (fabsf(vbv - va) > 1e-10f)
-            tpar = (double)((iso - va) / (vbv - va));
-          else
-            tpar = 0.5;
-          const int ax = to.x + x + c_mc_corner[a][0], ay = to.y + y + c_mc_corner[a][1], az = to.z + z + c_mc_corner[a][2];
-          const int bx = to.x + x + c_mc_corner[b][0], by = to.y + y + c_mc_corner[b][1], bz = to.z + z + c_mc_corner[b][2];
-          const double pax = (double)ax * vs, pay = (double)ay * vs, paz = (double)az * vs;
-          const double pbx = (double)bx * vs, pby = (double)by * vs, pbz = (double)bz * vs;
-          verts[3 * (size_t)vi + 0] = pax + tpar * (pbx - pax);
-          verts[3 * (size_t)vi + 1] = pay + tpar * (pby - pay);
-          verts[3 * (size_t)vi + 2] = paz + tpar * (pbz - paz);
-          ++vi;
-        }
-        int ev[12];
-        for (int e = 0; e < 12; ++e) {
-          ev[e] = 0;
-          if (!((em >> e) & 1u)) continue;
-          const int j = mc_owner(e, x, y, z, processed);
-          if (j < 0) continue;
-          const int cx = x + c_mc_cand[e][j][0], cy = y + c_mc_cand[e][j][1], cz = z + c_mc_cand[e][j][2];
-          const int ep = c_mc_cand[e][j][3];
-          const int d = ((cx >> 3) + 1) * 9 + ((cy >> 3) + 1) * 3 + ((cz >> 3) + 1);
-          uint32_t om;
-          int obase;
-          if (d == 13) {
-            om = s_meta[(cx << 6) + (cy << 3) + cz];
-            obase = vb;
-          } else {
-            const int ot = s_ttile[d];
-            const int fi = mc_face_index(cx & 7, cy & 7, cz & 7);
-            if (ot < 0 || fi < 0) continue;
-            om = meta[(size_t)ot * MC_FACE + fi];
-            obase = vbase[ot];
-          }
-          ev[e] = obase - vrel + (int)(om >> 12) + __popc((om & 0xFFFu) & ((1u << ep) - 1u));
-        }
-        int ti = tcarry + tin - ntr;
-        for (int q = 0; q < ntr && ti < tmax; ++q) {
-          tris[3 * (size_t)ti + 0] = (uint32_t)ev[c_mc_tri[cube][3 * q]];
-          tris[3 * (size_t)ti + 1] = (uint32_t)ev[c_mc_tri[cube][3 * q + 1]];
-          tris[3 * (size_t)ti + 2] = (uint32_t)ev[c_mc_tri[cube][3 * q + 2]];
-          ++ti;
-        }
+    if (vcarry <= MC_VLIST && tcarry <= MC_TLIST) {
+      for (int i = tid; i < vcarry; i += MC_THREADS) {
+        const int vi = vb + i;
+        if (vi >= vmax) break;
+        const int it = s_vlist[i], c = it >> 4;
+        mc_write_vertex(brick, c >> 6, (c >> 3) & 7, c & 7, it & 15, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
       }
-      tcarry += __shfl(tin, 63, 64);
+      for (int i = tid; i < tcarry; i += MC_THREADS) {
+        const int ti = tb + i;
+        if (ti >= tmax) break;
+        const int it = s_tlist[i], c = it >> 3, q = it & 7;
+        const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
+        const uint32_t cube = s_cube[c];
+        const int cls = mc_cell_class(cx, cy, cz);
+        uint32_t* o = tris + 3 * (size_t)ti;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          o[k] = (uint32_t)mc_edge_vertex(cx, cy, cz, mc_tri_edge(cube, 3 * q + k), cls, s_own, s_cand, s_meta, s_ttile,
+                                          meta, vbase, vb, vrel);
+      }
+    } else {
+      // Dense tile: cell by cell.
+      for (int c = tid; c < 512; c += MC_THREADS) {
+        const uint32_t cube = s_cube[c];
+        const int ntr = (int)(s_emn[cube] >> 12);
+        if (!ntr && !(s_meta[c] & 0xFFFu)) continue;
+        const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
+        const uint32_t m = s_meta[c];
+        int vi = vb + (int)(m >> 12);
+        for (uint32_t r = m & 0xFFFu; r && vi < vmax; r &= r - 1u, ++vi)
+          mc_write_vertex(brick, cx, cy, cz, __ffs(r) - 1, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
+        const int cls = mc_cell_class(cx, cy, cz);
+        int ti = tb + (int)s_tpre[c];
+        for (int q = 0; q < ntr && ti < tmax; ++q, ++ti)
+          for (int k = 0; k < 3; ++k)
+            tris[3 * (size_t)ti + k] = (uint32_t)mc_edge_vertex(cx, cy, cz, mc_tri_edge(cube, 3 * q + k), cls, s_own,
+                                                                s_cand, s_meta, s_ttile, meta, vbase, vb, vrel);
+      }
     }
     __syncthreads();
   }
@@ -390,6 +582,13 @@ __global__ void k_mc_ranges(const int* tile_base, int G, const int* ntiles, cons
 static int blocks_for(size_t n, int b) {
   size_t g = (n + b - 1) / b;
   return (int)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
+
+// Blocks that loop over the tiles (tables are loaded into LDS once per
+// block): up to 8 per compute unit.
+static int mc_blocks(const vr_ctx* ctx, int tcap) {
+  const int cap = 8 * (ctx->cus > 0 ? ctx->cus : 256);
+  return tcap < 1 ? 1 : (tcap < cap ? tcap : cap);
 }
 
 // Phase A: tile list for G grids (leaf counts known on the host: ltot).
@@ -435,7 +634,7 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
   if ((e = hipGetLastError()) != hipSuccess) return vr_hip_fail(ctx, e, "mc tiles");
   {
     TimedLaunch tl(ctx, TC_MC, algorithmic_bytes);
-    hipLaunchKernelGGL(k_mc_count, dim3(blocks_for(tcap, 1)), dim3(64), 0, ctx->stream, d_views, s.tiles.as<int4>(),
+    hipLaunchKernelGGL(k_mc_count, dim3(mc_blocks(ctx, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        totals, s.tleaf.as<int>(), s.ttile.as<int>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(),
                        s.tt.as<int>());
   }
@@ -456,9 +655,10 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, fl
   hipError_t e;
   {
     TimedLaunch tl(ctx, TC_MC, algorithmic_bytes);
-    hipLaunchKernelGGL(k_mc_emit, dim3(blocks_for(tcap, 1)), dim3(64), 0, ctx->stream, d_views, s.tiles.as<int4>(),
+    hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(), iso,
-                       s.meta.as<uint32_t>(), s.vb.as<int>(), s.tb.as<int>(), s.totals.as<int>(), verts, tris);
+                       s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
+                       s.totals.as<int>(), verts, tris);
   }
   return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc emit");
 }

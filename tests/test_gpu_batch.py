@@ -71,3 +71,40 @@ def test_batch_graph_replays_are_stable(orc, ctx, vr):
             assert b.output_nano(i) == ref[i][0], f"replay {rep} grid {i}"
             v, t = b.mesh(i, vr.BATCH_GRAPH)
             assert np.array_equal(t, ref[i][1][1]) and np.array_equal(v, ref[i][1][0]), f"replay {rep} mesh {i}"
+
+
+@pytest.mark.parametrize("kinds,tuning", [
+    ("mixed", (1, 0)),      # per-leaf velocity switch, one block per chunk
+    ("mixed", (3, 8)),      # persistent, chunk size not dividing the grids
+    ("vortex", (16, 4)),    # compacted 4-wave leaves, persistent
+    ("rigid", (2, 0)),      # slab leaves, one block per chunk
+])
+def test_batch_velocity_kinds_and_tunings(orc, ctx, vr, kinds, tuning):
+    """Every launch form of the batched advect (specialised or mixed kinds,
+    persistent or one block per chunk) against the oracle, with per-grid dt."""
+    announce(f"batch kinds={kinds} tuning={tuning}")
+    rng = np.random.default_rng(8)
+    vs, hw = 1.0 / 128, 3.0
+    spec = scenes(vr, rng, 4, vs)
+    fields = []
+    for i, (_, vel) in enumerate(spec):
+        k = {"mixed": (vr.VEL_UNIFORM, vr.VEL_RIGID, vr.VEL_VORTEX)[i % 3], "vortex": vr.VEL_VORTEX,
+             "rigid": vr.VEL_RIGID}[kinds]
+        params = {vr.VEL_UNIFORM: vel, vr.VEL_RIGID: (0.0, 0.0, 2.0, 0.5, 0.5, 0.5), vr.VEL_VORTEX: ()}[k]
+        fields.append((k, params))
+    grids = [ctx.from_primitives(p, vs, hw) for p, _ in spec]
+    vels = [ctx.velocity_analytic(k, p) for k, p in fields]
+    dts = [(0.3 + 0.1 * i) * vs for i in range(len(spec))]
+    b = vr.Batch(ctx, grids, vels)
+    b.set_dt(dts)
+    b.set_tuning(*tuning)
+    for mode in (vr.BATCH_EAGER, vr.BATCH_GRAPH):
+        b.step(0.0, 0.0, vr.RK3, 0.0, mode)
+        for i, (k, p) in enumerate(fields):
+            og = orc.from_nano(grids[i].download())
+            want = orc.advect_analytic(og, k, p, dts[i], 3)
+            assert_nano_equal(b.output_nano(i), want.to_nano(), f"mode {mode} grid {i}")
+            vw, tw, _ = orc.mesh(want, 0.0)
+            vg, tg = b.mesh(i, mode)
+            assert tg.shape == tw.shape and np.array_equal(tg, tw), f"mode {mode} grid {i} triangles"
+            assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"mode {mode} grid {i} vertices"

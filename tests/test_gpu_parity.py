@@ -9,7 +9,7 @@ import struct
 import numpy as np
 import pytest
 
-from util import (NANO_HEADER, active_voxels, assert_nano_equal, config1_dt, config1_sphere, header, leaf_array,
+from util import (NANO_HEADER, active_voxels, assert_nano_equal, sphere128_rigid_dt, sphere128, header, leaf_array,
                   mesh_equal, rigid_velocity_grids, slotted_sphere)
 
 pytestmark = pytest.mark.gpu
@@ -22,7 +22,7 @@ def announce(msg):
 # ---------------------------------------------------------------- NanoGrid boundary
 def test_upload_download_round_trip(orc, ctx):
     announce("nano round trip")
-    cases = [config1_sphere(orc), orc.new_grid(5.0, 1.0)]
+    cases = [sphere128(orc), orc.new_grid(5.0, 1.0)]
     g = orc.new_grid(-1.0, 1.0)
     g.set(-5, -10, -3, 77.0)
     g.set(-100, -200, -300, 88.0)
@@ -41,7 +41,7 @@ def test_upload_download_round_trip(orc, ctx):
 
 def test_upload_rejects_invalid_nano(orc, ctx, vr):
     announce("nano validation")
-    b = bytearray(config1_sphere(orc).to_nano())
+    b = bytearray(sphere128(orc).to_nano())
     bad = [bytes(b[:10]), b"XYZW" + bytes(b[4:]), bytes(b[:4]) + struct.pack("<I", 2) + bytes(b[8:]),
            bytes(b[:-1])]
     sw = bytearray(b)  # swap two leaves: not sorted
@@ -55,17 +55,17 @@ def test_upload_rejects_invalid_nano(orc, ctx, vr):
 
 
 # ---------------------------------------------------------------- advection
-def test_advect_rk1_grids3_config1(orc, ctx):
-    """Config 1: reference VelocityField path (vx/vy/vz grids), one RK1 step."""
-    announce("advect rk1 grids3 config1")
-    sdf = config1_sphere(orc)
+def test_advect_rk1_grids3_sphere128(orc, ctx):
+    """Sphere r=0.3 at 1/128, rigid rotation through the reference VelocityField path (vx/vy/vz grids), one RK1 step at CFL 0.5."""
+    announce("advect rk1 grids3 sphere128")
+    sdf = sphere128(orc)
     b = sdf.to_nano()
     vxyz = rigid_velocity_grids(orc, b, 1.0 / 128)
-    want = orc.advect_grids3(sdf, *vxyz, config1_dt()).to_nano()
+    want = orc.advect_grids3(sdf, *vxyz, sphere128_rigid_dt()).to_nano()
     g = ctx.upload(b)
     vg = [ctx.upload(v.to_nano()) for v in vxyz]
     vel = ctx.velocity_grids3(*vg)
-    got = g.advect(vel, config1_dt(), 0.0, 1).download()
+    got = g.advect(vel, sphere128_rigid_dt(), 0.0, 1).download()
     assert_nano_equal(got, want, "advect grids3")
 
 
@@ -97,8 +97,8 @@ def test_uniform_velocity_moves_sphere_sdf(orc, ctx):
 ])
 def test_advect_analytic(orc, ctx, kind, params, scheme, period):
     announce(f"advect analytic kind={kind} scheme={scheme}")
-    sdf = config1_sphere(orc)
-    dt, t = config1_dt(), 0.37
+    sdf = sphere128(orc)
+    dt, t = sphere128_rigid_dt(), 0.37
     tf = orc.stage_factors(period, t, dt, scheme)
     want = orc.advect_analytic(sdf, kind, params, dt, scheme, tf).to_nano()
     vel = ctx.velocity_analytic(kind, params, period)
@@ -137,7 +137,7 @@ def crossing_set(a):
 
 def test_zero_crossings_match(orc, ctx):
     announce("zero crossings")
-    for og in [orc.sphere(5.0, (0, 0, 0), 1.0, 3.0), config1_sphere(orc), slotted_sphere(orc, 1.0 / 128)]:
+    for og in [orc.sphere(5.0, (0, 0, 0), 1.0, 3.0), sphere128(orc), slotted_sphere(orc, 1.0 / 128)]:
         want = crossing_set(orc.zero_crossings(og))
         got = ctx.upload(og.to_nano()).zero_crossings()
         assert len(got) == len(want) and crossing_set(got) == want
@@ -166,17 +166,17 @@ def test_rebuild_matches_brute_force(orc, ctx):
             assert_nano_equal(g.download(), ref.to_nano(), f"rebuild hw={hw}")
 
 
-def test_rebuild_after_advect_config1(orc, ctx):
-    announce("rebuild config1")
-    sdf = config1_sphere(orc)
-    dt = config1_dt()
+def test_rebuild_after_advect_sphere128(orc, ctx):
+    announce("rebuild sphere128")
+    sdf = sphere128(orc)
+    dt = sphere128_rigid_dt()
     params = (0, 0, 2 * math.pi, 0.5, 0.5, 0.5)
     adv = orc.advect_analytic(sdf, 3, params, dt, 1)
     ref = orc.from_nano(adv.to_nano())
     orc.rebuild_bounded(ref, 3.0)
     g = ctx.upload(sdf.to_nano()).advect(ctx.velocity_analytic(3, params), dt)
     g.rebuild(3.0)
-    assert_nano_equal(g.download(), ref.to_nano(), "rebuild config1")
+    assert_nano_equal(g.download(), ref.to_nano(), "rebuild sphere128")
 
 
 def test_rebuild_noop_cases(orc, ctx):
@@ -205,16 +205,16 @@ def test_rebuild_rejects_wide_band(ctx, orc, vr):
 
 
 # ---------------------------------------------------------------- marching cubes
-@pytest.mark.parametrize("case", ["r5", "config1", "config1_advected", "slotted", "offset_negative"])
+@pytest.mark.parametrize("case", ["r5", "sphere128", "sphere128_advected", "slotted", "offset_negative"])
 def test_marching_cubes_matches(orc, ctx, case):
     announce(f"marching cubes {case}")
     if case == "r5":
         og, vs = orc.sphere(5.0, (0, 0, 0), 1.0, 3.0), 1.0
-    elif case == "config1":
-        og, vs = config1_sphere(orc), 1.0 / 128
-    elif case == "config1_advected":
+    elif case == "sphere128":
+        og, vs = sphere128(orc), 1.0 / 128
+    elif case == "sphere128_advected":
         vs = 1.0 / 128
-        og = orc.advect_analytic(config1_sphere(orc), 3, (0, 0, 2 * math.pi, 0.5, 0.5, 0.5), config1_dt())
+        og = orc.advect_analytic(sphere128(orc), 3, (0, 0, 2 * math.pi, 0.5, 0.5, 0.5), sphere128_rigid_dt())
     elif case == "slotted":
         og, vs = slotted_sphere(orc, 1.0 / 128), 1.0 / 128
     else:
@@ -231,7 +231,7 @@ def test_marching_cubes_empty(orc, ctx):
 
 
 def test_counts(orc, ctx):
-    og = config1_sphere(orc)
+    og = sphere128(orc)
     assert ctx.upload(og.to_nano()).counts() == (og.leaf_count(), og.active_count())
 
 

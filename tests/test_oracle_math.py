@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT
-from util import active_voxels, config1_dt, config1_sphere, leaf_array, slotted_sphere
+from util import active_voxels, sphere128_rigid_dt, sphere128, leaf_array, slotted_sphere
 
 REF_MC = "/root/reference/src/tools/volume_to_mesh.rs"
 
@@ -122,7 +122,7 @@ def test_mc_vertices_on_lattice_edges_at_zero(orc):
     """Each vertex lies on a lattice edge (two coordinates on the lattice) and
     the trilinear interpolant of the grid vanishes there (iso = 0)."""
     vs = 1.0 / 128
-    g = config1_sphere(orc)
+    g = sphere128(orc)
     v, t, _ = orc.mesh(g, 0.0)
     idx = v / vs
     on = np.abs(idx - np.round(idx)) < 1e-9
@@ -234,9 +234,9 @@ def test_rk3_is_third_order_rk1_first_order(orc):
 
 
 def test_rk3_uniform_velocity_reduces_to_rk1(orc):
-    g = config1_sphere(orc)
+    g = sphere128(orc)
     u = (0.3, -0.7, 0.2)
-    dt = config1_dt()
+    dt = sphere128_rigid_dt()
     a = active_voxels(orc.advect_analytic(g, 2, u, dt, 1).to_nano())[1]
     b = active_voxels(orc.advect_analytic(g, 2, u, dt, 3).to_nano())[1]
     assert np.max(np.abs(a - b)) < 2e-7

@@ -82,11 +82,11 @@ def rigid_velocity_grids(orc, sdf_nano, vs, omega=(0.0, 0.0, 2 * math.pi), cente
     return grids
 
 
-def config1_sphere(orc):
+def sphere128(orc):
     return orc.sphere(0.3, (0.5, 0.5, 0.5), 1.0 / 128, 3.0)
 
 
-def config1_dt():
+def sphere128_rigid_dt():
     return 0.5 * (1.0 / 128) / math.pi
 
 
