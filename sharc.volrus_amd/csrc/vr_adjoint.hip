@@ -7,15 +7,15 @@
 // through the dense field U.  Only active voxels are variables; inactive and
 // background values are constants.  Per backward step and per leaf (one
 // block of 4 waves):
-//   - the adjoint lambda_{n+1} of each active voxel is scattered with the
-//     trilinear weights into per-wave LDS bricks (leaf + one-voxel halo),
-//     summed in wave order into an outbox; a merge kernel sums, for each
-//     voxel, the entries of the (up to 8) leaves whose brick covers it in a
-//     fixed order, so lambda_n is deterministic;
+//   - lambda_{n+1} of each active voxel is the sum of the brick entries
+//     (leaf + one-voxel halo) that the previous backward step left for it
+//     in the (up to 8) leaves whose brick covers it, in a fixed order;
+//   - it is scattered with the trilinear weights into this leaf's brick as
+//     exact 64-bit fixed-point sums, so lambda_n is deterministic;
 //   - lambda * dS/dp is pulled back through the departure point (and, for
 //     RK3, through the stage positions with the field's Jacobian) onto the
-//     velocity nodes, pre-reduced in an LDS window and added to dL/dU with
-//     global atomics.
+//     velocity nodes, reduced in an LDS window (separably for the voxel's
+//     own position) and added to dL/dU with global atomics.
 // Forward states are kept every `checkpoint_every` steps; the others are
 // recomputed segment by segment during the reverse pass.
 #include <cstring>
@@ -30,11 +30,13 @@
 struct AdjStep {
   GridView phi;  // phi_n
   VelView vel;   // dense, static
-  const float* lam_next;
-  float* outbox;   // [cap][1000]
-  float* lam_far;  // [cap][512], corners beyond the brick (large steps)
-  float* grad;     // 3 n^3
+  const float* ob_in;  // [cap][1000] bricks of step n+1 (lambda_{n+1} = their merge)
+  float* far_in;       // [cap][512] far-corner sums of step n+1 (zeroed once read)
+  float* ob_out;       // [cap][1000] bricks of this step
+  float* far_out;      // [cap][512], corners beyond the brick (large steps)
+  float* grad;         // 3 n^3
   double dt, tf0, tf1, tf2;
+  int separable;   // grid transform diagonal with positive scales (host check)
 };
 
 // Trilinear node weights of the dense field at p, in double, and the base node.
@@ -55,34 +57,43 @@ __device__ __forceinline__ void lds_add(float* p, float v) {
 }
 
 // Adds coef[c] * w_q(p) to node q of component c: LDS window when inside it,
-// else global.  Lanes visit the 8 nodes in a lane-rotated order: voxels of
-// one wave mostly share a velocity cell, and the rotation makes the lanes of
-// one atomic instruction hit different nodes instead of serialising on one.
+// else global.  The branch is taken once per point (all eight nodes inside
+// the window, the usual case, or not), so the LDS path carries no global
+// accesses.  Lanes visit the 8 nodes in a lane-rotated order: voxels of one
+// wave mostly share a velocity cell, and the rotation makes the lanes of one
+// atomic instruction hit different nodes instead of serialising on one.
 __device__ __forceinline__ void vel_scatter(const AdjStep& P, float* vwin, const int wb[3], const double p[3],
                                             const double coef[3]) {
   int b[3];
   double f[3];
   dense_weights(P.vel, p, b, f);
   const int n = P.vel.n;
-  const size_t n3 = (size_t)n * n * n;
   const int lane = threadIdx.x & 63;
   const int rot = lane + (lane >> 3);
-  for (int s = 0; s < 8; ++s) {
-    const int q = (s + rot) & 7;
+  const int wi = b[0] - wb[0], wj = b[1] - wb[1], wk = b[2] - wb[2];
+  // Fast path: all eight nodes inside both the window and the field (nodes
+  // outside the field carry no variable and get nothing).
+  if ((unsigned)wi < VW - 1 && (unsigned)wj < VW - 1 && (unsigned)wk < VW - 1 && (unsigned)b[0] < (unsigned)(n - 1) &&
+      (unsigned)b[1] < (unsigned)(n - 1) && (unsigned)b[2] < (unsigned)(n - 1)) {
+    const int w0 = (wi * VW + wj) * VW + wk;
+    for (int s = 0; s < 8; ++s) {
+      const int q = (s + rot) & 7;
+      const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+      const double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
+      const int o = w0 + (dx * VW + dy) * VW + dz;
+      lds_add(&vwin[o], (float)(coef[0] * w));
+      lds_add(&vwin[VW3 + o], (float)(coef[1] * w));
+      lds_add(&vwin[2 * VW3 + o], (float)(coef[2] * w));
+    }
+    return;
+  }
+  const size_t n3 = (size_t)n * n * n;
+  for (int q = 0; q < 8; ++q) {
     const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
     const int i = b[0] + dx, j = b[1] + dy, k = b[2] + dz;
     if ((unsigned)i >= (unsigned)n || (unsigned)j >= (unsigned)n || (unsigned)k >= (unsigned)n) continue;
     const double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
-    const int wi = i - wb[0], wj = j - wb[1], wk = k - wb[2];
-    const bool in = (unsigned)wi < VW && (unsigned)wj < VW && (unsigned)wk < VW;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float val = (float)(coef[c] * w);
-      if (in)
-        lds_add(&vwin[c * VW3 + (wi * VW + wj) * VW + wk], val);
-      else
-        atomicAdd(&P.grad[c * n3 + ((size_t)i * n + j) * n + k], val);
-    }
+    for (int c = 0; c < 3; ++c) atomicAdd(&P.grad[c * n3 + ((size_t)i * n + j) * n + k], (float)(coef[c] * w));
   }
 }
 
@@ -110,17 +121,70 @@ __device__ __forceinline__ void dense_jt(const VelView& v, const double p[3], co
 #define ADJ_WAVES 4
 #define ADJ_THREADS (64 * ADJ_WAVES)
 
-// One backward step, one leaf per block of ADJ_WAVES waves.  Each wave
-// scatters its voxels' lambda into its own LDS brick (a wave's LDS atomics
-// apply in a fixed order), and the outbox entry is the sum of the wave
-// bricks in wave order, so lambda_n is the same on every run.
+// LDS 64-bit integer add (DS instruction).
+__device__ __forceinline__ void lds_add_i64(long long* p, long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Trilinear weight of node `node` along one axis for a point with base node b
+// and fraction f: 1 - f at b, f at b + 1, 0 elsewhere.
+__device__ __forceinline__ double axis_weight(int node, int b, double f) {
+  return node == b ? 1.0 - f : (node == b + 1 ? f : 0.0);
+}
+
+// lambda(y) for voxel (x, y, z) of leaf li: the brick entries of the leaves
+// whose brick covers it, summed in neighbour order, plus the far-corner sum.
+// A neighbour at offset (dx, dy, dz) covers the voxel only where its local
+// coordinate is 0 (for d = -1) or 7 (for d = +1) on every axis with d != 0,
+// so at most 8 bricks are visited, in ascending neighbour order.
+__device__ __forceinline__ float merged_lambda(const float* ob, const float* far, const int* nb, int li, int x, int y,
+                                               int z) {
+  float s = 0.0f;
+  for (int dx = (x == 0 ? -1 : 0); dx <= (x == 7 ? 1 : 0); ++dx)
+    for (int dy = (y == 0 ? -1 : 0); dy <= (y == 7 ? 1 : 0); ++dy)
+      for (int dz = (z == 0 ? -1 : 0); dz <= (z == 7 ? 1 : 0); ++dz) {
+        const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+        if (src < 0) continue;
+        const int lx = x - 8 * dx, ly = y - 8 * dy, lz = z - 8 * dz;
+        s += ob[(size_t)src * 1000 + (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1)];
+      }
+  return s + far[(size_t)li * 512 + ((x << 6) | (y << 3) | z)];
+}
+
+// One backward step, one leaf per block of ADJ_WAVES waves.
+//
+// lambda_{n+1} of the leaf's voxels is first merged from the previous step's
+// bricks (so no separate merge launch), then:
+//
+// lambda scatter: each contribution (float)(lambda w) is added as a 64-bit
+// fixed-point integer with the leaf's scale 2^(53 - E), max |lambda_{n+1}|
+// over the leaf < 2^E, so a brick cell (at most 512 contributions) stays
+// below 2^62.  Integer sums are exact, so lambda_n does not depend on the
+// order of the adds, and integer LDS atomics run far faster than float ones.
+//
+// Velocity scatter at the voxel's own position (the RK1 term and the last
+// RK3 term): for a diagonal transform the node weights factor per axis, so
+// the per-voxel coefficients are staged in LDS and contracted axis by axis
+// into the leaf's node window (no atomics).  Other scatter points (RK3
+// stages), non-diagonal transforms and windows too small for the leaf use
+// float atomics into the window.
 template <int SCHEME>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(AdjStep P) {
+__device__ __forceinline__ void adj_step(const AdjStep& P) {
   __shared__ float brick[1000];
-  __shared__ float acc[ADJ_WAVES][1000];
+  __shared__ long long acc[1000];
+  __shared__ float coefx[3][512];
   __shared__ float vwin[3 * VW3];
+  __shared__ float t1[3 * 64 * VW];
+  __shared__ float t2[3 * 8 * VW * VW];
   __shared__ uint16_t list[512];
   __shared__ int nb[VR_NBR];
+  __shared__ float s_max[ADJ_WAVES];
+  __shared__ float s_lam[512];
+  // per axis and local coordinate: base node relative to the window, and the
+  // weight of each window node
+  __shared__ int s_bn[3][8];
+  __shared__ float s_wt[3][8][VW];
+  __shared__ int s_sep;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const GridView& g = P.phi;
   const int n = *g.count;
@@ -136,7 +200,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(AdjStep P) {
       if ((x % ADJ_WAVES) == wave && ((m >> lane) & 1ull)) list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
       total += __popcll(m);
     }
-    for (int i = tid; i < ADJ_WAVES * 1000; i += ADJ_THREADS) (&acc[0][0])[i] = 0.0f;
+    for (int i = tid; i < 1000; i += ADJ_THREADS) acc[i] = 0;
+    for (int i = tid; i < 3 * 512; i += ADJ_THREADS) (&coefx[0][0])[i] = 0.0f;
     for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin[i] = 0.0f;
     const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
     int wb[3];
@@ -145,14 +210,50 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(AdjStep P) {
       index_to_world(g.mat, ox, oy, oz, ow);
       for (int a = 0; a < 3; ++a) wb[a] = f64_to_i32(floor(ow[a] * P.vel.inv_h)) - 2;
     }
+    if (tid < 24) {
+      // Node base and fraction per axis of the voxel positions (diagonal
+      // transform: the axis coordinate alone fixes them).
+      const int a = tid >> 3, l = tid & 7;
+      const int o = a == 0 ? ox : (a == 1 ? oy : oz);
+      const double w = g.mat[5 * a] * (double)(o + l) + g.mat[4 * a + 3];
+      const double idx = w * P.vel.inv_h;
+      const double fl = floor(idx);
+      const int bn = f64_to_i32(fl) - wb[a];
+      s_bn[a][l] = bn;
+      for (int k = 0; k < VW; ++k) s_wt[a][l][k] = (float)axis_weight(k, bn, idx - fl);
+    }
     stage_interior<ADJ_THREADS>(g, li, brick, tid);
     __syncthreads();
+    // lambda_{n+1} of the active voxels (merged), its max |.|; the leaf's
+    // far-corner entries are consumed and cleared for the next step.
+    float lm = 0.0f;
+    for (int off = tid; off < 512; off += ADJ_THREADS) {
+      float lam = 0.0f;
+      if ((L.mask[off >> 6] >> (off & 63)) & 1ull)
+        lam = merged_lambda(P.ob_in, P.far_in, nb, li, off >> 6, (off >> 3) & 7, off & 7);
+      s_lam[off] = lam;
+      P.far_in[(size_t)li * 512 + off] = 0.0f;
+      lm = fmaxf(lm, fabsf(lam));
+    }
+    for (int o = 32; o > 0; o >>= 1) lm = fmaxf(lm, __shfl_xor(lm, o));
+    if (lane == 0) s_max[wave] = lm;
     stage_halo<ADJ_THREADS>(g, nb, brick, tid);
+    if (tid == 0) {
+      bool ok = P.separable != 0;
+      for (int a = 0; a < 3; ++a) ok = ok && s_bn[a][0] >= 0 && s_bn[a][7] + 1 < VW;
+      s_sep = ok ? 1 : 0;
+    }
     __syncthreads();
-    float* const wacc = acc[wave];
-    for (int ii = tid; ii < total; ii += ADJ_THREADS) {
+    float mx = s_max[0];
+#pragma unroll
+    for (int w = 1; w < ADJ_WAVES; ++w) mx = fmaxf(mx, s_max[w]);
+    int ex = 0;
+    (void)frexpf(mx, &ex);  // mx < 2^ex
+    const double scale = ldexp(1.0, 53 - ex), inv_scale = ldexp(1.0, ex - 53);
+    const bool sep = s_sep != 0;
+    for (int ii = tid; ii < total && mx > 0.0f; ii += ADJ_THREADS) {
       const int off = list[ii];
-      const float lam = P.lam_next[(size_t)li * 512 + off];
+      const float lam = s_lam[off];
       if (lam == 0.0f) continue;
       const int x = off >> 6, y = (off >> 3) & 7, z = off & 7;
       double xw[3], y1[3], y2[3], p[3], u[3];
@@ -174,19 +275,26 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(AdjStep P) {
       const int bi = i0 - ox, bj = j0 - oy, bk = k0 - oz;
       const bool inb = bi >= -1 && bi <= 7 && bj >= -1 && bj <= 7 && bk >= -1 && bk <= 7;
       double c[8];
+      if (inb) {
+        // Departure cell inside the LDS brick (the usual case).
+        const int b0 = (bi + 1) * 100 + (bj + 1) * 10 + (bk + 1);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
-        const double w = (dx ? fx : 1.0 - fx) * (dy ? fy : 1.0 - fy) * (dz ? fz : 1.0 - fz);
-        if (inb) {
-          const int bidx = (bi + 1 + dx) * 100 + (bj + 1 + dy) * 10 + (bk + 1 + dz);
-          c[q] = (double)brick[bidx];
-          lds_add(&wacc[bidx], (float)((double)lam * w));
-        } else {
+        for (int q = 0; q < 8; ++q) c[q] = (double)brick[b0 + (q & 1) * 100 + ((q >> 1) & 1) * 10 + ((q >> 2) & 1)];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+          const double w = (dx ? fx : 1.0 - fx) * (dy ? fy : 1.0 - fy) * (dz ? fz : 1.0 - fz);
+          const float v = (float)((double)lam * w);
+          lds_add_i64(&acc[b0 + dx * 100 + dy * 10 + dz], (long long)rint((double)v * scale));
+        }
+      } else {
+        for (int q = 0; q < 8; ++q) {
+          const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+          const double w = (dx ? fx : 1.0 - fx) * (dy ? fy : 1.0 - fy) * (dz ? fz : 1.0 - fz);
           const int cx = i0 + dx, cy = j0 + dy, cz = k0 + dz;
           const int tl = find_leaf(g, cx, cy, cz);
           c[q] = tl < 0 ? (double)g.background : (double)g.leaves[tl].values[voxel_offset(cx, cy, cz)];
-          if (tl >= 0) atomicAdd(&P.lam_far[(size_t)tl * 512 + voxel_offset(cx, cy, cz)], (float)((double)lam * w));
+          if (tl >= 0) atomicAdd(&P.far_out[(size_t)tl * 512 + voxel_offset(cx, cy, cz)], (float)((double)lam * w));
         }
       }
       const double dfx = (c[1] - c[0]) * (1 - fy) * (1 - fz) + (c[3] - c[2]) * fy * (1 - fz) +
@@ -196,9 +304,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(AdjStep P) {
       const double dfz = (c[4] - c[0]) * (1 - fx) * (1 - fy) + (c[5] - c[1]) * fx * (1 - fy) +
                          (c[6] - c[2]) * (1 - fx) * fy + (c[7] - c[3]) * fx * fy;
       const double gp[3] = {(double)lam * dfx * g.inv_vs, (double)lam * dfy * g.inv_vs, (double)lam * dfz * g.inv_vs};
+      double cself[3];  // coefficient of the scatter at the voxel's own position
       if (SCHEME == 1) {
-        const double coef[3] = {-P.dt * tf[0] * gp[0], -P.dt * tf[0] * gp[1], -P.dt * tf[0] * gp[2]};
-        vel_scatter(P, vwin, wb, xw, coef);
+        for (int a = 0; a < 3; ++a) cself[a] = -P.dt * tf[0] * gp[a];
       } else {
         double jt[3], gy2[3], gy1[3];
         dense_jt(P.vel, y2, gp, jt);
@@ -211,18 +319,44 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(AdjStep P) {
         const double c2[3] = {-0.25 * P.dt * tf[1] * gy2[0], -0.25 * P.dt * tf[1] * gy2[1],
                               -0.25 * P.dt * tf[1] * gy2[2]};
         vel_scatter(P, vwin, wb, y1, c2);
-        const double c1[3] = {-P.dt * tf[0] * gy1[0], -P.dt * tf[0] * gy1[1], -P.dt * tf[0] * gy1[2]};
-        vel_scatter(P, vwin, wb, xw, c1);
+        for (int a = 0; a < 3; ++a) cself[a] = -P.dt * tf[0] * gy1[a];
+      }
+      if (sep) {
+        for (int a = 0; a < 3; ++a) coefx[a][off] = (float)cself[a];
+      } else {
+        vel_scatter(P, vwin, wb, xw, cself);
       }
     }
     __syncthreads();
-    float* ob = P.outbox + (size_t)li * 1000;
-    for (int i = tid; i < 1000; i += ADJ_THREADS) {
-      float v = acc[0][i];
-#pragma unroll
-      for (int w = 1; w < ADJ_WAVES; ++w) v += acc[w][i];
-      ob[i] = v;
+    if (sep) {
+      // Separable contraction of coefx into the window: z, then y, then x.
+      for (int i = tid; i < 3 * 64 * VW; i += ADJ_THREADS) {
+        const int c = i / (64 * VW), r = i % (64 * VW), xy = r / VW, k = r % VW;
+        float sum = 0.0f;
+        for (int zz = 0; zz < 8; ++zz)
+          sum += coefx[c][xy * 8 + zz] * s_wt[2][zz][k];
+        t1[i] = sum;
+      }
+      __syncthreads();
+      for (int i = tid; i < 3 * 8 * VW * VW; i += ADJ_THREADS) {
+        const int c = i / (8 * VW * VW), r = i % (8 * VW * VW), xx = r / (VW * VW), j = (r / VW) % VW, k = r % VW;
+        float sum = 0.0f;
+        for (int yy = 0; yy < 8; ++yy)
+          sum += t1[(c * 64 + xx * 8 + yy) * VW + k] * s_wt[1][yy][j];
+        t2[i] = sum;
+      }
+      __syncthreads();
+      for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) {
+        const int c = i / VW3, r = i % VW3, ii = r / (VW * VW), jk = r % (VW * VW);
+        float sum = 0.0f;
+        for (int xx = 0; xx < 8; ++xx)
+          sum += t2[(c * 8 + xx) * VW * VW + jk] * s_wt[0][xx][ii];
+        vwin[i] += sum;
+      }
+      __syncthreads();
     }
+    float* ob = P.ob_out + (size_t)li * 1000;
+    for (int i = tid; i < 1000; i += ADJ_THREADS) ob[i] = (float)((double)acc[i] * inv_scale);
     const int nv = P.vel.n;
     const size_t n3 = (size_t)nv * nv * nv;
     for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) {
@@ -230,66 +364,49 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(AdjStep P) {
       if (v == 0.0f) continue;
       const int c = i / VW3, r = i % VW3;
       const int gi = wb[0] + r / (VW * VW), gj = wb[1] + (r / VW) % VW, gk = wb[2] + r % VW;
+      if ((unsigned)gi >= (unsigned)nv || (unsigned)gj >= (unsigned)nv || (unsigned)gk >= (unsigned)nv) continue;
       atomicAdd(&P.grad[c * n3 + ((size_t)gi * nv + gj) * nv + gk], v);
     }
     __syncthreads();
   }
 }
 
-// lambda_n(y) for each active voxel y: the outbox entries of the leaves whose
-// brick covers y, summed in neighbour order, plus the far-corner buffer.  A
-// neighbour at offset (dx, dy, dz) covers y only where y's local coordinate
-// is 0 (for d = -1) or 7 (for d = +1) on every axis with d != 0, so at most 8
-// bricks are visited, in ascending neighbour order.
-__global__ __launch_bounds__(256) void k_adj_merge(GridView g, const float* outbox, float* lam_far, float* lam) {
-  __shared__ int nb[VR_NBR];
-  const int tid = threadIdx.x;
-  const int n = *g.count;
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
-    if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
-    __syncthreads();
-    const NanoLeaf& L = g.leaves[li];
-    for (int off = tid; off < 512; off += 256) {
-      const int x = off >> 6, y = (off >> 3) & 7, z = off & 7;
-      float s = 0.0f;
-      if ((L.mask[x] >> (off & 63)) & 1ull) {
-        for (int dx = (x == 0 ? -1 : 0); dx <= (x == 7 ? 1 : 0); ++dx)
-          for (int dy = (y == 0 ? -1 : 0); dy <= (y == 7 ? 1 : 0); ++dy)
-            for (int dz = (z == 0 ? -1 : 0); dz <= (z == 7 ? 1 : 0); ++dz) {
-              const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
-              if (src < 0) continue;
-              const int lx = x - 8 * dx, ly = y - 8 * dy, lz = z - 8 * dz;
-              s += outbox[(size_t)src * 1000 + (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1)];
-            }
-        s += lam_far[(size_t)li * 512 + off];
-      }
-      lam[(size_t)li * 512 + off] = s;
-      lam_far[(size_t)li * 512 + off] = 0.0f;
-    }
-    __syncthreads();
-  }
+// RK1: held to 128 VGPRs (4 waves per SIMD; a few bytes of spill) -- faster
+// than the unconstrained allocation at 3 waves.  RK3 keeps its registers.
+__global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adj_step1(AdjStep P) {
+  adj_step<1>(P);
 }
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_step3(AdjStep P) { adj_step<3>(P); }
 
-// lambda_K = phi_K - target at active voxels (0 elsewhere); per-leaf loss in f64.
-__global__ __launch_bounds__(64) void k_adj_terminal(GridView g, GridView target, float* lam, double* leaf_loss) {
-  const int lane = threadIdx.x;
+// lambda_K = phi_K - target at active voxels (0 elsewhere), written as each
+// leaf's brick (interior entries; the halo is 0) so the first backward step
+// merges it like any other step's bricks; per-leaf loss in f64.  The target
+// value of a voxel is Grid::get: the target leaf with the same origin holds
+// all of the leaf's voxels, so it is looked up once per leaf.
+__global__ __launch_bounds__(256) void k_adj_terminal(GridView g, GridView target, float* ob, double* leaf_loss) {
+  __shared__ double s_part[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = *g.count;
-  const int y = lane >> 3, z = lane & 7;
   for (int li = blockIdx.x; li < n; li += gridDim.x) {
     const NanoLeaf& L = g.leaves[li];
+    const int tl = find_leaf(target, L.origin[0], L.origin[1], L.origin[2]);
     double part = 0.0;
-    for (int x = 0; x < 8; ++x) {
-      const int off = x * 64 + lane;
+    for (int i = tid; i < 1000; i += 256) {
+      const int bx = i / 100 - 1, by = (i / 10) % 10 - 1, bz = i % 10 - 1;
       double d = 0.0;
-      if ((L.mask[x] >> lane) & 1ull) {
-        const float t = grid_get(target, L.origin[0] + x, L.origin[1] + y, L.origin[2] + z);
+      if (bx >= 0 && bx < 8 && by >= 0 && by < 8 && bz >= 0 && bz < 8 && ((L.mask[bx] >> (by * 8 + bz)) & 1ull)) {
+        const int off = bx * 64 + by * 8 + bz;
+        const float t = tl < 0 ? target.background : target.leaves[tl].values[off];
         d = (double)L.values[off] - (double)t;
         part += 0.5 * d * d;
       }
-      lam[(size_t)li * 512 + off] = (float)d;
+      ob[(size_t)li * 1000 + i] = (float)d;
     }
     for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
-    if (lane == 0) leaf_loss[li] = part;
+    if (lane == 0) s_part[wave] = part;
+    __syncthreads();
+    if (tid == 0) leaf_loss[li] = ((s_part[0] + s_part[1]) + s_part[2]) + s_part[3];
+    __syncthreads();
   }
 }
 
@@ -338,7 +455,7 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   // Buffers: checkpoints (segment starts 1..nseg-1), segment states, final state.
   std::vector<NanoLeaf*> ck((size_t)nseg, nullptr), seg((size_t)c, nullptr);
   NanoLeaf* fin = nullptr;
-  float *lam_a = nullptr, *lam_b = nullptr, *outbox = nullptr, *far = nullptr, *grad = nullptr;
+  float *ob[2] = {nullptr, nullptr}, *far[2] = {nullptr, nullptr}, *grad = nullptr;
   double *leaf_loss = nullptr, *loss_d = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   int rc = VR_OK;
@@ -350,10 +467,10 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   for (int k = 1; k < nseg; ++k) alloc((void**)&ck[k], sizeof(NanoLeaf) * cap);
   for (int j = 1; j < c; ++j) alloc((void**)&seg[j], sizeof(NanoLeaf) * cap);
   alloc((void**)&fin, sizeof(NanoLeaf) * cap);
-  alloc((void**)&lam_a, sizeof(float) * 512 * cap);
-  alloc((void**)&lam_b, sizeof(float) * 512 * cap);
-  alloc((void**)&outbox, sizeof(float) * 1000 * cap);
-  alloc((void**)&far, sizeof(float) * 512 * cap);
+  for (int i = 0; i < 2; ++i) {
+    alloc((void**)&ob[i], sizeof(float) * 1000 * cap);
+    alloc((void**)&far[i], sizeof(float) * 512 * cap);
+  }
   alloc((void**)&grad, sizeof(float) * 3 * nv3);
   alloc((void**)&leaf_loss, sizeof(double) * cap);
   alloc((void**)&loss_d, sizeof(double));
@@ -364,6 +481,12 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   copy_header_into(&view, phi0);
   view.topo = phi0->topo;
   const VelView vv = make_vel_view(vel);
+  // Diagonal transform with positive scales: velocity-node weights of voxel
+  // positions factor per axis.
+  int separable = 1;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      if (r == c ? !(phi0->mat[r][c] > 0.0) : phi0->mat[r][c] != 0.0) separable = 0;
   const double tf[3] = {1.0, 1.0, 1.0};
   auto step_forward = [&](NanoLeaf* src, NanoLeaf* dst) {
     if (rc != VR_OK) return;
@@ -374,8 +497,9 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   };
   if (rc == VR_OK) {
     hipLaunchKernelGGL(k_zero_f32, dim3(grid_blocks(3 * nv3, 256)), dim3(256), 0, ctx->stream, grad, 3 * nv3);
-    hipLaunchKernelGGL(k_zero_f32, dim3(grid_blocks(512 * (size_t)cap, 256)), dim3(256), 0, ctx->stream, far,
-                       512 * (size_t)cap);
+    for (int i = 0; i < 2; ++i)
+      hipLaunchKernelGGL(k_zero_f32, dim3(grid_blocks(512 * (size_t)cap, 256)), dim3(256), 0, ctx->stream, far[i],
+                         512 * (size_t)cap);
     (void)hipEventRecord(ev[0], ctx->stream);
   }
   // Forward: states at segment starts are kept; the last state goes to `fin`.
@@ -395,14 +519,13 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   // Terminal adjoint and loss.
   if (rc == VR_OK) {
     view.d_leaves = fin;
-    hipLaunchKernelGGL(k_adj_terminal, dim3(grid_blocks(L, 1)), dim3(64), 0, ctx->stream, make_view(&view),
-                       make_view(target), lam_a, leaf_loss);
+    hipLaunchKernelGGL(k_adj_terminal, dim3(grid_blocks(L, 1)), dim3(256), 0, ctx->stream, make_view(&view),
+                       make_view(target), ob[0], leaf_loss);
     hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, ctx->stream, leaf_loss, phi0->topo->d_count, loss_d);
     if ((e = hipGetLastError()) != hipSuccess) rc = vr_hip_fail(ctx, e, "terminal");
   }
-  // Reverse, segment by segment.
-  float* lam_next = lam_a;
-  float* lam_cur = lam_b;
+  // Reverse, segment by segment; ob[bi] / far[bi] hold the last step's bricks.
+  int bi = 0;
   for (int k = nseg - 1; k >= 0 && rc == VR_OK; --k) {
     const int s0 = k * c, s1 = (k + 1) * c < steps ? (k + 1) * c : steps;
     // Recompute states s0+1 .. s1-1 into seg[1..].
@@ -416,19 +539,20 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
       AdjStep P;
       P.phi = make_view(&view);
       P.vel = vv;
-      P.lam_next = lam_next;
-      P.outbox = outbox;
-      P.lam_far = far;
+      P.ob_in = ob[bi];
+      P.far_in = far[bi];
+      P.ob_out = ob[bi ^ 1];
+      P.far_out = far[bi ^ 1];
       P.grad = grad;
       P.dt = dt;
       P.tf0 = P.tf1 = P.tf2 = 1.0;
+      P.separable = separable;
       if (scheme == VR_SCHEME_RK3)
-        hipLaunchKernelGGL(k_adj_step<3>, dim3(grid_blocks(L, 1)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+        hipLaunchKernelGGL(k_adj_step3, dim3(grid_blocks(L, 1)), dim3(ADJ_THREADS), 0, ctx->stream, P);
       else
-        hipLaunchKernelGGL(k_adj_step<1>, dim3(grid_blocks(L, 1)), dim3(ADJ_THREADS), 0, ctx->stream, P);
-      hipLaunchKernelGGL(k_adj_merge, dim3(grid_blocks(L, 1)), dim3(256), 0, ctx->stream, P.phi, outbox, far, lam_cur);
+        hipLaunchKernelGGL(k_adj_step1, dim3(grid_blocks(L, 1)), dim3(ADJ_THREADS), 0, ctx->stream, P);
       if ((e = hipGetLastError()) != hipSuccess) rc = vr_hip_fail(ctx, e, "adjoint step");
-      std::swap(lam_next, lam_cur);
+      bi ^= 1;
     }
   }
   view.d_leaves = nullptr;
@@ -459,7 +583,7 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
     if (ck[k]) (void)hipFree(ck[k]);
   for (int j = 1; j < c; ++j)
     if (seg[j]) (void)hipFree(seg[j]);
-  for (void* p : {(void*)fin, (void*)lam_a, (void*)lam_b, (void*)outbox, (void*)far, (void*)grad, (void*)leaf_loss,
+  for (void* p : {(void*)fin, (void*)ob[0], (void*)ob[1], (void*)far[0], (void*)far[1], (void*)grad, (void*)leaf_loss,
                   (void*)loss_d})
     if (p) (void)hipFree(p);
   for (int i = 0; i < 3; ++i)
