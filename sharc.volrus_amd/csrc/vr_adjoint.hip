@@ -190,7 +190,8 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
   const int n = *g.count;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double tf[3] = {P.tf0, P.tf1, P.tf2};
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
+  int li_end, li_step;
+  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
     if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
     int total = 0;
@@ -387,7 +388,8 @@ __global__ __launch_bounds__(256) void k_adj_terminal(GridView g, GridView targe
   __shared__ double s_part[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = *g.count;
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
+  int li_end, li_step;
+  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
     const int tl = find_leaf(target, L.origin[0], L.origin[1], L.origin[2]);
     double part = 0.0;

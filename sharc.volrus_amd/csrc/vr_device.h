@@ -259,6 +259,26 @@ __device__ __forceinline__ void departure(const VelView& v, const double x[3], d
   }
 }
 
+// ------------------------------------------------------------------ XCD-aware item order
+// Items [0, n) are cut into 8 contiguous ranges, one per XCD: workgroups are
+// dispatched to the 8 XCDs round-robin (workgroup b on XCD b % 8), so items
+// next to each other (leaves whose halos overlap, tiles sharing leaves) are
+// processed under one XCD's L2 instead of being fetched by several.  Block b
+// starts at the returned item and steps by *stride up to *end.  Grids of
+// fewer than 8 blocks use the plain order.
+__device__ __forceinline__ int xcd_first(int n, int* end, int* stride) {
+  const int B = (int)gridDim.x, b = (int)blockIdx.x;
+  if (B < 8) {
+    *end = n;
+    *stride = B;
+    return b;
+  }
+  const int x = b & 7;
+  *stride = (B - x + 7) >> 3;  // blocks on this XCD
+  *end = (int)((long long)n * (x + 1) / 8);
+  return (int)((long long)n * x / 8) + (b >> 3);
+}
+
 // ------------------------------------------------------------------ wave helpers
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 

@@ -95,7 +95,8 @@ __global__ __launch_bounds__(64) void k_gen_leaves(const int4* cand, int n, cons
                                                    double vs, NanoLeaf* out, int* keep) {
   const int lane = threadIdx.x;
   const int y = lane >> 3, z = lane & 7;
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
+  int li_end, li_step;
+  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const int4 o = cand[li];
     NanoLeaf& O = out[li];
     if (lane < 3) O.origin[lane] = lane == 0 ? o.x : (lane == 1 ? o.y : o.z);

@@ -147,7 +147,8 @@ __global__ __launch_bounds__(64 * NW) void k_advect(GridView in, VelView vel, Na
   __shared__ AdvectLDS s;
   const int n = *in.count;
   const double tf[3] = {tf0, tf1, tf2};
-  for (int li = blockIdx.x; li < n; li += gridDim.x)
+  int li_end, li_step;
+  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step)
     advect_leaf<VK, SCHEME, NW>(in, vel, out, li, s, threadIdx.x, dt, tf);
 }
 
@@ -187,7 +188,8 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
   const int lane = threadIdx.x;
   const int n = *g.count;
   const int y = lane >> 3, z = lane & 7;
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
+  int li_end, li_step;
+  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
     if (lane < VR_NBR) nb[lane] = g.nbr[(size_t)li * VR_NBR + lane];
     __syncthreads();
@@ -241,7 +243,8 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
   const int n = *g.count;
   const bool have = *any != 0;
   const int y = lane >> 3, z = lane & 7;
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
+  int li_end, li_step;
+  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
     NanoLeaf& O = out[li];
     if (lane < 3) O.origin[lane] = L.origin[lane];
@@ -321,7 +324,8 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
 __global__ __launch_bounds__(64) void k_trim(GridView g, NanoLeaf* out, int* keep, float threshold) {
   const int lane = threadIdx.x;
   const int n = *g.count;
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
+  int li_end, li_step;
+  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
     NanoLeaf& O = out[li];
     if (lane < 3) O.origin[lane] = L.origin[lane];

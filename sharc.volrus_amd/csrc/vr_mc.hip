@@ -402,10 +402,12 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   for (int i = tid; i < 27 * 12; i += MC_THREADS) s_ownsort[i] = c_mc_ownsort[i];
   const int nt = *ntiles;
   const int wave = tid >> 6, y = (tid >> 3) & 7, z = tid & 7;
-  MCNext nx = mc_fetch(tleaf, ttile, blockIdx.x, nt);
-  for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+  int t_end, t_step;
+  const int t0 = xcd_first(nt, &t_end, &t_step);
+  MCNext nx = mc_fetch(tleaf, ttile, t0, t_end);
+  for (int t = t0; t < t_end; t += t_step) {
     mc_publish(nx, s_tleaf, s_ttile, &s_proc);
-    nx = mc_fetch(tleaf, ttile, t + gridDim.x, nt);
+    nx = mc_fetch(tleaf, ttile, t + t_step, t_end);
     if (tid < 2) s_side[tid] = 0;
     __syncthreads();
     mc_build_self(s_self, s_ownsort, s_proc, tid);
@@ -478,7 +480,8 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
   const int nt = *ntiles;
   const int vmax = totals[1], tmax = totals[2];
   const int wave = tid >> 6, y = (tid >> 3) & 7, z = tid & 7;
-  for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+  int t_end, t_step;
+  for (int t = xcd_first(nt, &t_end, &t_step); t < t_end; t += t_step) {
     if (tv[t] == 0 && tt[t] == 0) continue;  // nothing to write (uniform over the block)
     mc_publish(mc_fetch(tleaf, ttile, t, nt), s_tleaf, s_ttile, &s_proc);
     if (tid < 2) s_side[tid] = 0;
