@@ -150,6 +150,37 @@ def adjoint_extra(vr, ctx):
             "loss": st.loss}
 
 
+def adjoint_noisy_extra(vr, ctx, count=8):
+    """8 level sets at 1/2048, each a sphere r = 0.35 displaced by seeded 4-octave value noise (amplitude
+    0.02, base frequency 8), band 3; per grid a dense 128^3 field of seeded Gaussian noise (sigma 0.5);
+    10 RK3 steps forward + adjoint with a checkpoint every 2 steps; loss against the undisplaced sphere.
+    Grids run one after another through vr_adjoint_rollout."""
+    import numpy as np
+
+    vs, n, steps, ckpt = 1.0 / 2048, 128, 10, 2
+    target = ctx.from_primitives([(vr.PRIM_SPHERE, 0, (0.5, 0.5, 0.5, 0.35))], vs, HW)
+    fwd = adj = 0.0
+    active = leaves = 0
+    for i in range(count):
+        phi = ctx.from_primitives([(vr.PRIM_NOISY_SPHERE, 0, (0.5, 0.5, 0.5, 0.35, 0.02, 8.0, 1000 + i))], vs, HW)
+        lc, ac = phi.counts()
+        u = np.random.default_rng(100 + i).normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32)
+        vel = ctx.velocity_dense(u, n, 1.0 / n)
+        dt = 0.5 * vs / float(np.abs(u).max())
+        if i == 0:
+            phi.adjoint_rollout(vel, n, dt, vr.RK3, steps, ckpt, target, False)  # warm
+        st, _ = phi.adjoint_rollout(vel, n, dt, vr.RK3, steps, ckpt, target)
+        fwd += st.forward_ms
+        adj += st.adjoint_ms
+        active += ac
+        leaves += lc
+        log(f"noisy sphere {i}: {lc} leaves, {ac} active voxels, forward {st.forward_ms:.2f} ms, "
+            f"adjoint {st.adjoint_ms:.2f} ms")
+        del phi, vel
+    return {"grids": count, "leaves": leaves, "active_voxels": active, "steps": steps, "checkpoint_every": ckpt,
+            "forward_ms": round(fwd, 3), "adjoint_ms": round(adj, 3), "adjoint_over_forward": round(adj / fwd, 3)}
+
+
 def cpu_baseline(seconds_budget):
     """Oracle (literal C++ restatement of the reference CPU path) on a bounded
     sample of the same workload: the first steps of the rollout, single thread."""
@@ -186,7 +217,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint lines")
-    ap.add_argument("--only", choices=["batched", "adjoint"], help="run one side workload alone (for profiling)")
+    ap.add_argument("--only", choices=["batched", "adjoint", "adjoint_noisy"],
+                    help="run one side workload alone (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -195,7 +227,7 @@ def main():
     vr = load_package()
     ctx = vr.Context(0)
     if args.only:
-        res = batched_extra(vr, ctx) if args.only == "batched" else adjoint_extra(vr, ctx)
+        res = {"batched": batched_extra, "adjoint": adjoint_extra, "adjoint_noisy": adjoint_noisy_extra}[args.only](vr, ctx)
         print(json.dumps({args.only: res}), flush=True)
         ctx.close()
         return
@@ -251,6 +283,11 @@ def main():
             log(f"adjoint: {extras['adjoint_sphere256_dense64']}")
         except Exception as e:  # reported, not fatal for the headline line
             extras["adjoint_sphere256_dense64"] = {"error": str(e)}
+        try:
+            extras["adjoint_8_noisy_spheres_2048"] = adjoint_noisy_extra(vr, ctx)
+            log(f"adjoint noisy: {extras['adjoint_8_noisy_spheres_2048']}")
+        except Exception as e:
+            extras["adjoint_8_noisy_spheres_2048"] = {"error": str(e)}
         try:
             extras["batched_64_sphere_unions_1024"] = batched_extra(vr, ctx)
             log(f"batched: {extras['batched_64_sphere_unions_1024']}")

@@ -312,3 +312,30 @@ def test_rollout_matches_oracle_composition(orc, ctx, vr, shared):
         vg, tg = mesh.arrays()
         mesh_equal(vg, tg, wmesh[0], wmesh[1], vs, f"rollout mesh rep {rep}")
         assert (st.last_vertex_count, st.last_triangle_count) == (len(wmesh[0]), len(wmesh[1]))
+
+
+# ---------------------------------------------------------------- FP32 error against the FP64 oracle
+@pytest.mark.parametrize("kind,params,scheme,steps", [
+    (3, (0, 0, 2 * math.pi, 0.5, 0.5, 0.5), 1, 1),   # rigid rotation, one RK1 step
+    (4, (), 3, 4),                                    # single vortex, four RK3 steps
+])
+def test_fp32_error_against_fp64_oracle(orc, ctx, vr, kind, params, scheme, steps):
+    """The GPU's f32 values after `steps` advection steps against the same
+    steps carried out in f64 by the oracle: the error, in voxels, stays below
+    1e-5 (the stated tolerance) and is printed."""
+    announce(f"fp32 vs fp64 kind={kind} scheme={scheme} steps={steps}")
+    vs = 1.0 / 128
+    og = sphere128(orc)
+    dt, period = sphere128_rigid_dt(), (8.0 if kind == 4 else 0.0)
+    vel = ctx.velocity_analytic(kind, params, period)
+    g = ctx.upload(og.to_nano())
+    g64 = orc.to_f64(og)
+    for s in range(steps):
+        tf = orc.stage_factors(period, s * dt, dt, scheme)
+        g = g.advect(vel, dt, s * dt, scheme)
+        g64 = orc.advect_analytic_f64(g64, kind, params, dt, scheme, tf)
+    c, v = active_voxels(g.download())
+    ref = np.array([g64.get(*xyz) for xyz in c.tolist()])
+    err = float(np.abs(v.astype(np.float64) - ref).max()) / vs
+    print(f"[gpu] max |f32 - f64| = {err:.3e} voxel over {len(v)} active voxels", flush=True)
+    assert err < 1e-5
