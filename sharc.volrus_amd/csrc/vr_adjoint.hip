@@ -156,26 +156,50 @@ __device__ __forceinline__ float merged_lambda(const float* ob, const float* far
 // lambda_{n+1} of the leaf's voxels is first merged from the previous step's
 // bricks (so no separate merge launch), then:
 //
-// lambda scatter: each contribution (float)(lambda w) is added as a 64-bit
-// fixed-point integer with the leaf's scale 2^(53 - E), max |lambda_{n+1}|
-// over the leaf < 2^E, so a brick cell (at most 512 contributions) stays
-// below 2^62.  Integer sums are exact, so lambda_n does not depend on the
-// order of the adds, and integer LDS atomics run far faster than float ones.
+// lambda scatter: each contribution (float)(lambda w) is quantised to an
+// int32 with the leaf's scale 2^(30 - E), max |lambda_{n+1}| over the leaf
+// < 2^E, and summed in 64 bits (a brick cell gets at most 512 of them).
+// Integer sums are exact, so lambda_n does not depend on the order of the
+// adds, and integer LDS atomics run far faster than float ones.
 //
 // Velocity scatter at the voxel's own position (the RK1 term and the last
 // RK3 term): for a diagonal transform the node weights factor per axis, so
 // the per-voxel coefficients are staged in LDS and contracted axis by axis
-// into the leaf's node window (no atomics).  Other scatter points (RK3
-// stages), non-diagonal transforms and windows too small for the leaf use
-// float atomics into the window.
+// into the leaf's node window (no atomics).  RK3 stage positions (and the
+// voxel's own position when that form does not apply) are kept as records
+// and added in a second pass as int32-quantised 64-bit sums scaled by the
+// leaf's max |coefficient|; RK1 without the separable form uses float
+// atomics into the window.
+
+// Record of one velocity scatter point of a voxel (RK3 stage positions, and
+// the voxel's own position when the separable form does not apply): base
+// node relative to the leaf's window packed as 3 x 10-bit signed, the three
+// fractions and the three coefficients.  REC_NONE marks no record.
+#define REC_NONE (-2147483647 - 1)
+__device__ __forceinline__ int rec_unpack(int v, int a) {
+  const int s = (v >> (20 - 10 * a)) & 1023;
+  return s >= 512 ? s - 1024 : s;
+}
+
 template <int SCHEME>
 __device__ __forceinline__ void adj_step(const AdjStep& P) {
+  // RK3 keeps 3 scatter records per voxel (y2, y1, and x when not separable).
+  constexpr int NREC = SCHEME == 3 ? 3 : 0;
+  constexpr int REC_WORDS = NREC * 7 * 512;
+  constexpr int T_WORDS = 3 * 64 * VW + 3 * 8 * VW * VW;
   __shared__ float brick[1000];
   __shared__ long long acc[1000];
   __shared__ float coefx[3][512];
   __shared__ float vwin[3 * VW3];
-  __shared__ float t1[3 * 64 * VW];
-  __shared__ float t2[3 * 8 * VW * VW];
+  __shared__ long long vwin_i[SCHEME == 3 ? 3 * VW3 : 1];
+  // records (pass 1 -> pass 2), then the separable contraction stages
+  __shared__ float s_raw[REC_WORDS > T_WORDS ? REC_WORDS : T_WORDS];
+  float* const t1 = s_raw;
+  float* const t2 = s_raw + 3 * 64 * VW;
+  float* const rec_c = s_raw;                              // [NREC][3][512]
+  float* const rec_f = s_raw + NREC * 3 * 512;             // [NREC][3][512]
+  int* const rec_b = (int*)(s_raw + NREC * 6 * 512);       // [NREC][512]
+  __shared__ float s_cmax[ADJ_WAVES];
   __shared__ uint16_t list[512];
   __shared__ int nb[VR_NBR];
   __shared__ float s_max[ADJ_WAVES];
@@ -204,6 +228,10 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
     for (int i = tid; i < 1000; i += ADJ_THREADS) acc[i] = 0;
     for (int i = tid; i < 3 * 512; i += ADJ_THREADS) (&coefx[0][0])[i] = 0.0f;
     for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin[i] = 0.0f;
+    if constexpr (SCHEME == 3) {
+      for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin_i[i] = 0;
+      for (int i = tid; i < NREC * 512; i += ADJ_THREADS) rec_b[i] = REC_NONE;
+    }
     const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
     int wb[3];
     {
@@ -250,8 +278,38 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
     for (int w = 1; w < ADJ_WAVES; ++w) mx = fmaxf(mx, s_max[w]);
     int ex = 0;
     (void)frexpf(mx, &ex);  // mx < 2^ex
-    const double scale = ldexp(1.0, 53 - ex), inv_scale = ldexp(1.0, ex - 53);
+    // contributions |v| < 2^ex quantise to int32 (|q| < 2^30, f32 -> i32 is
+    // one instruction); at most 512 per cell sum below 2^39 in int64
+    const float scale = ldexpf(1.0f, 30 - ex);
+    const double inv_scale = ldexp(1.0, ex - 30);
     const bool sep = s_sep != 0;
+    float cmax = 0.0f;  // max |coefficient| over this thread's records
+    // Stores a scatter record (RK3), or adds far points straight to dL/dU.
+    auto rec_put = [&](int r, int off, const double pt[3], const double c[3]) {
+      int b[3];
+      double f[3];
+      dense_weights(P.vel, pt, b, f);
+      const int r0 = b[0] - wb[0], r1 = b[1] - wb[1], r2 = b[2] - wb[2];
+      if (r0 < -512 || r0 > 511 || r1 < -512 || r1 > 511 || r2 < -512 || r2 > 511) {
+        const int nn = P.vel.n;
+        const size_t n3 = (size_t)nn * nn * nn;
+        for (int q = 0; q < 8; ++q) {
+          const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+          const int i = b[0] + dx, j = b[1] + dy, k = b[2] + dz;
+          if ((unsigned)i >= (unsigned)nn || (unsigned)j >= (unsigned)nn || (unsigned)k >= (unsigned)nn) continue;
+          const double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
+          for (int a = 0; a < 3; ++a) atomicAdd(&P.grad[a * n3 + ((size_t)i * nn + j) * nn + k], (float)(c[a] * w));
+        }
+        return;
+      }
+      rec_b[r * 512 + off] = ((r0 & 1023) << 20) | ((r1 & 1023) << 10) | (r2 & 1023);
+      for (int a = 0; a < 3; ++a) {
+        rec_f[(r * 3 + a) * 512 + off] = (float)f[a];
+        const float cf = (float)c[a];
+        rec_c[(r * 3 + a) * 512 + off] = cf;
+        cmax = fmaxf(cmax, fabsf(cf));
+      }
+    };
     for (int ii = tid; ii < total && mx > 0.0f; ii += ADJ_THREADS) {
       const int off = list[ii];
       const float lam = s_lam[off];
@@ -286,7 +344,7 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
           const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
           const double w = (dx ? fx : 1.0 - fx) * (dy ? fy : 1.0 - fy) * (dz ? fz : 1.0 - fz);
           const float v = (float)((double)lam * w);
-          lds_add_i64(&acc[b0 + dx * 100 + dy * 10 + dz], (long long)rint((double)v * scale));
+          lds_add_i64(&acc[b0 + dx * 100 + dy * 10 + dz], (long long)(int)rintf(v * scale));
         }
       } else {
         for (int q = 0; q < 8; ++q) {
@@ -314,45 +372,125 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
         for (int a = 0; a < 3; ++a) gy2[a] = (2.0 / 3.0) * gp[a] - (2.0 / 3.0) * P.dt * tf[2] * jt[a];
         const double c3[3] = {-(2.0 / 3.0) * P.dt * tf[2] * gp[0], -(2.0 / 3.0) * P.dt * tf[2] * gp[1],
                               -(2.0 / 3.0) * P.dt * tf[2] * gp[2]};
-        vel_scatter(P, vwin, wb, y2, c3);
+        rec_put(0, off, y2, c3);
         dense_jt(P.vel, y1, gy2, jt);
         for (int a = 0; a < 3; ++a) gy1[a] = 0.25 * gy2[a] - 0.25 * P.dt * tf[1] * jt[a];
         const double c2[3] = {-0.25 * P.dt * tf[1] * gy2[0], -0.25 * P.dt * tf[1] * gy2[1],
                               -0.25 * P.dt * tf[1] * gy2[2]};
-        vel_scatter(P, vwin, wb, y1, c2);
+        rec_put(1, off, y1, c2);
         for (int a = 0; a < 3; ++a) cself[a] = -P.dt * tf[0] * gy1[a];
       }
       if (sep) {
         for (int a = 0; a < 3; ++a) coefx[a][off] = (float)cself[a];
+      } else if constexpr (SCHEME == 3) {
+        rec_put(2, off, xw, cself);
       } else {
         vel_scatter(P, vwin, wb, xw, cself);
       }
     }
+    double inv_scale_v = 0.0;
+    if constexpr (SCHEME == 3) {
+      // Pass 2: records into the node window as exact 64-bit fixed-point
+      // sums of int32-quantised contributions.
+      for (int o = 32; o > 0; o >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, o));
+      if (lane == 0) s_cmax[wave] = cmax;
+      __syncthreads();
+      float cm = s_cmax[0];
+#pragma unroll
+      for (int w = 1; w < ADJ_WAVES; ++w) cm = fmaxf(cm, s_cmax[w]);
+      int ev = 0;
+      (void)frexpf(cm, &ev);
+      // |c w| < 2^ev quantises to int32 (|q| < 2^30); at most 3 x 512
+      // contributions per node sum below 2^41 in int64
+      const float scale_v = ldexpf(1.0f, 30 - ev);
+      inv_scale_v = ldexp(1.0, ev - 30);
+      const int nn = P.vel.n;
+      const size_t n3 = (size_t)nn * nn * nn;
+      const int rot = lane + (lane >> 3);
+      for (int ii = tid; ii < total; ii += ADJ_THREADS) {
+        const int off = list[ii];
+        for (int r = 0; r < NREC; ++r) {
+          const int rb = rec_b[r * 512 + off];
+          if (rb == REC_NONE) continue;
+          const int rel[3] = {rec_unpack(rb, 0), rec_unpack(rb, 1), rec_unpack(rb, 2)};
+          float f[3], cs[3];
+          for (int a = 0; a < 3; ++a) {
+            f[a] = rec_f[(r * 3 + a) * 512 + off];
+            cs[a] = rec_c[(r * 3 + a) * 512 + off] * scale_v;
+          }
+          const bool inwin = (unsigned)rel[0] < VW - 1 && (unsigned)rel[1] < VW - 1 && (unsigned)rel[2] < VW - 1 &&
+                             (unsigned)(wb[0] + rel[0]) < (unsigned)(nn - 1) &&
+                             (unsigned)(wb[1] + rel[1]) < (unsigned)(nn - 1) &&
+                             (unsigned)(wb[2] + rel[2]) < (unsigned)(nn - 1);
+          if (inwin) {
+            // all 8 nodes in the window and the field: nodes and components
+            // in a lane-rotated order, so lanes sharing a cell spread over
+            // 24 addresses
+            const int o0 = (rel[0] * VW + rel[1]) * VW + rel[2];
+            const int crot = (lane >> 3) % 3;
+#pragma unroll 1
+            for (int st = 0; st < 8; ++st) {
+              const int q = (st + rot) & 7;
+              const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+              const float w = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
+              const int o = o0 + (dx * VW + dy) * VW + dz;
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                const int cc = a + crot >= 3 ? a + crot - 3 : a + crot;
+                const float c = cc == 0 ? cs[0] : (cc == 1 ? cs[1] : cs[2]);
+                lds_add_i64(&vwin_i[cc * VW3 + o], (long long)(int)rintf(c * w));
+              }
+            }
+          } else {
+            for (int q = 0; q < 8; ++q) {
+              const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+              const int wi = rel[0] + dx, wj = rel[1] + dy, wk = rel[2] + dz;
+              const int gi = wb[0] + wi, gj = wb[1] + wj, gk = wb[2] + wk;
+              if ((unsigned)gi >= (unsigned)nn || (unsigned)gj >= (unsigned)nn || (unsigned)gk >= (unsigned)nn) continue;
+              const float w = (dx ? f[0] : 1.0f - f[0]) * (dy ? f[1] : 1.0f - f[1]) * (dz ? f[2] : 1.0f - f[2]);
+              if ((unsigned)wi < VW && (unsigned)wj < VW && (unsigned)wk < VW) {
+                const int o = (wi * VW + wj) * VW + wk;
+                for (int a = 0; a < 3; ++a) lds_add_i64(&vwin_i[a * VW3 + o], (long long)(int)rintf(cs[a] * w));
+              } else {
+                for (int a = 0; a < 3; ++a)
+                  atomicAdd(&P.grad[a * n3 + ((size_t)gi * nn + gj) * nn + gk], (float)((double)cs[a] * w * inv_scale_v));
+              }
+            }
+          }
+        }
+      }
+    }
     __syncthreads();
     if (sep) {
-      // Separable contraction of coefx into the window: z, then y, then x.
-      for (int i = tid; i < 3 * 64 * VW; i += ADJ_THREADS) {
-        const int c = i / (64 * VW), r = i % (64 * VW), xy = r / VW, k = r % VW;
+      // Separable contraction of coefx into the window: z, then y, then x,
+      // over the nodes the leaf's voxels touch (per axis s_bn[a][0] ..
+      // s_bn[a][7] + 1; two or three for fields coarser than the voxels).
+      const int lx = s_bn[0][0], ly = s_bn[1][0], lz = s_bn[2][0];
+      const int nx = s_bn[0][7] + 2 - lx, ny = s_bn[1][7] + 2 - ly, nz = s_bn[2][7] + 2 - lz;
+      for (int i = tid; i < 3 * 64 * nz; i += ADJ_THREADS) {
+        const int c = i / (64 * nz), r = i - c * 64 * nz, xy = r / nz, k = r - xy * nz;
         float sum = 0.0f;
-        for (int zz = 0; zz < 8; ++zz)
-          sum += coefx[c][xy * 8 + zz] * s_wt[2][zz][k];
+#pragma unroll
+        for (int zz = 0; zz < 8; ++zz) sum += coefx[c][xy * 8 + zz] * s_wt[2][zz][lz + k];
         t1[i] = sum;
       }
       __syncthreads();
-      for (int i = tid; i < 3 * 8 * VW * VW; i += ADJ_THREADS) {
-        const int c = i / (8 * VW * VW), r = i % (8 * VW * VW), xx = r / (VW * VW), j = (r / VW) % VW, k = r % VW;
+      for (int i = tid; i < 3 * 8 * ny * nz; i += ADJ_THREADS) {
+        const int c = i / (8 * ny * nz), r = i - c * 8 * ny * nz, xx = r / (ny * nz), jk = r - xx * ny * nz;
+        const int j = jk / nz, k = jk - j * nz;
         float sum = 0.0f;
-        for (int yy = 0; yy < 8; ++yy)
-          sum += t1[(c * 64 + xx * 8 + yy) * VW + k] * s_wt[1][yy][j];
+#pragma unroll
+        for (int yy = 0; yy < 8; ++yy) sum += t1[(c * 64 + xx * 8 + yy) * nz + k] * s_wt[1][yy][ly + j];
         t2[i] = sum;
       }
       __syncthreads();
-      for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) {
-        const int c = i / VW3, r = i % VW3, ii = r / (VW * VW), jk = r % (VW * VW);
+      for (int i = tid; i < 3 * nx * ny * nz; i += ADJ_THREADS) {
+        const int c = i / (nx * ny * nz), r = i - c * nx * ny * nz, ii = r / (ny * nz), jk = r - ii * ny * nz;
+        const int j = jk / nz, k = jk - j * nz;
         float sum = 0.0f;
-        for (int xx = 0; xx < 8; ++xx)
-          sum += t2[(c * 8 + xx) * VW * VW + jk] * s_wt[0][xx][ii];
-        vwin[i] += sum;
+#pragma unroll
+        for (int xx = 0; xx < 8; ++xx) sum += t2[(c * 8 + xx) * ny * nz + jk] * s_wt[0][xx][lx + ii];
+        vwin[c * VW3 + ((lx + ii) * VW + ly + j) * VW + lz + k] += sum;
       }
       __syncthreads();
     }
@@ -361,7 +499,8 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
     const int nv = P.vel.n;
     const size_t n3 = (size_t)nv * nv * nv;
     for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) {
-      const float v = vwin[i];
+      float v = vwin[i];
+      if constexpr (SCHEME == 3) v += (float)((double)vwin_i[i] * inv_scale_v);
       if (v == 0.0f) continue;
       const int c = i / VW3, r = i % VW3;
       const int gi = wb[0] + r / (VW * VW), gj = wb[1] + (r / VW) % VW, gk = wb[2] + r % VW;
@@ -377,7 +516,9 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
 __global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adj_step1(AdjStep P) {
   adj_step<1>(P);
 }
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_step3(AdjStep P) { adj_step<3>(P); }
+__global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adj_step3(AdjStep P) {
+  adj_step<3>(P);
+}
 
 // lambda_K = phi_K - target at active voxels (0 elsewhere), written as each
 // leaf's brick (interior entries; the halo is 0) so the first backward step

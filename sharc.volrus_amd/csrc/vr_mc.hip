@@ -242,17 +242,6 @@ __device__ __forceinline__ int mc_face_index(int lx, int ly, int lz) {
 }
 #define MC_FACE 169
 
-// Cube index of cell (x, y, z): bit c set when corner c is below iso
-// (volume_to_mesh.rs:115-123).
-__device__ __forceinline__ uint32_t mc_case(const float* brick, int x, int y, int z, float iso) {
-  const int b = x * 81 + y * 9 + z;
-  // corners in the reference's order: (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1)
-  const float v0 = brick[b], v1 = brick[b + 81], v2 = brick[b + 90], v3 = brick[b + 9];
-  const float v4 = brick[b + 1], v5 = brick[b + 82], v6 = brick[b + 91], v7 = brick[b + 10];
-  return (uint32_t)(v0 < iso) | ((uint32_t)(v1 < iso) << 1) | ((uint32_t)(v2 < iso) << 2) |
-         ((uint32_t)(v3 < iso) << 3) | ((uint32_t)(v4 < iso) << 4) | ((uint32_t)(v5 < iso) << 5) |
-         ((uint32_t)(v6 < iso) << 6) | ((uint32_t)(v7 < iso) << 7);
-}
 
 // ------------------------------------------------------------------ per-tile block helpers
 // Both passes run one tile per block of MC_THREADS threads (4 waves); the
@@ -303,23 +292,51 @@ __device__ __forceinline__ void mc_publish(const MCNext& n, int* s_tleaf, int* s
   }
 }
 
-// Stages the 9^3 corner values of a tile (local 0..8) into LDS and records
-// whether they lie on both sides of iso (some value < iso and some not): only
-// then can a cell of the tile have a case other than 0 or 255.  s_side[0]
-// and s_side[1] must be 0 on entry; read them after a synchronisation.
+// Stages the 9^3 corner values of a tile (local 0..8) into LDS and the sign
+// rows: bit bz of s_rows[bx * 9 + by] is set when the value at (bx, by, bz)
+// is below iso (the reference's corner test, volume_to_mesh.rs:115-123).
+// s_rows must be 0 on entry; read it after a synchronisation.
 __device__ __forceinline__ void mc_stage(const GridView& g, const int* s_tleaf, float* brick, float iso,
-                                         int* s_side) {
-  bool lt = false, ge = false;
+                                         uint32_t* s_rows) {
   for (int i = threadIdx.x; i < 729; i += MC_THREADS) {
     const int bx = i / 81, by = (i / 9) % 9, bz = i % 9;
     const int d = ((bx >> 3) + 1) * 9 + ((by >> 3) + 1) * 3 + ((bz >> 3) + 1);
     const int src = s_tleaf[d];
     const float v = src < 0 ? g.background : g.leaves[src].values[voxel_offset(bx, by, bz)];
     brick[i] = v;
-    if (v < iso) lt = true; else ge = true;
+    if (v < iso) atomicOr(&s_rows[i / 9], 1u << bz);
   }
-  if (__any(lt) && (threadIdx.x & 63) == 0) s_side[0] = 1;
-  if (__any(ge) && (threadIdx.x & 63) == 0) s_side[1] = 1;
+}
+
+// The tile's mixed cells (corners on both sides of iso: a case other than 0
+// or 255, so crossing edges), listed in visiting order (x, y, z) by wave 0:
+// lane x * 8 + y tests its 8 cells along z with bit operations on the four
+// sign rows, and a prefix of the counts places them.  Returns the count on
+// every lane of wave 0 and stores it in *s_n.  Caller synchronises.
+__device__ __forceinline__ void mc_mixed_list(const uint32_t* s_rows, uint16_t* s_list, int* s_n) {
+  const int lane = threadIdx.x;
+  if (lane >= 64) return;
+  const int x = lane >> 3, y = lane & 7;
+  const uint32_t r00 = s_rows[x * 9 + y], r10 = s_rows[(x + 1) * 9 + y];
+  const uint32_t r01 = s_rows[x * 9 + y + 1], r11 = s_rows[(x + 1) * 9 + y + 1];
+  const uint32_t all = r00 & r10 & r01 & r11, any = r00 | r10 | r01 | r11;
+  const uint32_t one = all & (all >> 1), zero = ~any & ~(any >> 1);
+  uint32_t mixed = ~(one | zero) & 0xFFu;  // bit z: cell (x, y, z)
+  const int cnt = __popc(mixed);
+  const int incl = wave_incl_scan(cnt);
+  int k = incl - cnt;
+  for (; mixed; mixed &= mixed - 1u, ++k) s_list[k] = (uint16_t)(lane * 8 + (__ffs(mixed) - 1));
+  if (lane == 63) *s_n = incl;
+}
+
+// Cube index of cell c = x * 64 + y * 8 + z from the sign rows, corners in
+// the reference's order (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1)
+// (1,1,1) (0,1,1).
+__device__ __forceinline__ uint32_t mc_case_rows(const uint32_t* s_rows, int x, int y, int z) {
+  const uint32_t r00 = s_rows[x * 9 + y] >> z, r10 = s_rows[(x + 1) * 9 + y] >> z;
+  const uint32_t r11 = s_rows[(x + 1) * 9 + y + 1] >> z, r01 = s_rows[x * 9 + y + 1] >> z;
+  return (r00 & 1u) | ((r10 & 1u) << 1) | ((r11 & 1u) << 2) | ((r01 & 1u) << 3) | (((r00 >> 1) & 1u) << 4) |
+         (((r10 >> 1) & 1u) << 5) | (((r11 >> 1) & 1u) << 6) | (((r01 >> 1) & 1u) << 7);
 }
 
 // Global index of the vertex on local edge e of cell (x, y, z) of tile t,
@@ -381,54 +398,64 @@ __device__ __forceinline__ int mc_tri_edge(uint32_t cube, int k) {
 }
 
 // ------------------------------------------------------------------ pass 1
-// Per cell: case, owned crossing edges, triangle count.  The face cells
-// (mc_face_index >= 0) with crossing edges store meta = (vertex prefix inside
-// the tile << 12) | owned-edge mask for the emit pass of later tiles; per-tile
-// vertex and triangle totals.  Blocks loop over tiles.
+// Per mixed cell (in visiting order): case, owned crossing edges, triangle
+// count.  The face cells (mc_face_index >= 0) store meta = (vertex prefix
+// inside the tile << 12) | owned-edge mask for the emit pass of later tiles;
+// per-tile vertex and triangle totals.  Cells with case 0 or 255 add nothing
+// and are never visited.  Blocks loop over tiles.
 __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
                                                          const int* tleaf, const int* ttile, float iso,
                                                          uint32_t* meta, int* tv, int* tt) {
   __shared__ float brick[729];
+  __shared__ uint32_t s_rows[81];
+  __shared__ uint16_t s_mlist[512];
+  __shared__ int s_n;
   __shared__ uint16_t s_emn[256];
   __shared__ uint16_t s_self[27];
   __shared__ int s_tleaf[VR_NBR];
   __shared__ int s_ttile[VR_NBR];
   __shared__ uint32_t s_proc;
-  __shared__ int s_side[2];
   __shared__ int s_w[4];
   __shared__ uint32_t s_ownsort[27 * 12];
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
   for (int i = tid; i < 27 * 12; i += MC_THREADS) s_ownsort[i] = c_mc_ownsort[i];
   const int nt = *ntiles;
-  const int wave = tid >> 6, y = (tid >> 3) & 7, z = tid & 7;
   int t_end, t_step;
   const int t0 = xcd_first(nt, &t_end, &t_step);
   MCNext nx = mc_fetch(tleaf, ttile, t0, t_end);
   for (int t = t0; t < t_end; t += t_step) {
     mc_publish(nx, s_tleaf, s_ttile, &s_proc);
     nx = mc_fetch(tleaf, ttile, t + t_step, t_end);
-    if (tid < 2) s_side[tid] = 0;
+    if (tid < 81) s_rows[tid] = 0u;
     __syncthreads();
     mc_build_self(s_self, s_ownsort, s_proc, tid);
     const GridView& g = views[tiles[t].w];
-    mc_stage(g, s_tleaf, brick, iso, s_side);
+    mc_stage(g, s_tleaf, brick, iso, s_rows);
     __syncthreads();
+    mc_mixed_list(s_rows, s_mlist, &s_n);
+    __syncthreads();
+    const int n = s_n;
     int vcarry = 0, tcarry = 0;
-    if (s_side[0] && s_side[1]) {
-      for (int h = 0; h < 2; ++h) {
-        const int x = wave + 4 * h;
-        const uint32_t emn = s_emn[mc_case(brick, x, y, z, iso)];
-        const uint32_t em = emn & 0xFFFu;
-        const uint32_t owned = em & s_self[mc_cell_class(x, y, z)];
-        const int packed = __popc(owned) | (int)((emn >> 12) << 16);
-        int tot;
-        const int excl = mc_block_scan(packed, s_w, &tot);
-        const int fi = mc_face_index(x, y, z);
-        if (em && fi >= 0) meta[(size_t)t * MC_FACE + fi] = ((uint32_t)(vcarry + (excl & 0xFFFF)) << 12) | owned;
-        vcarry += tot & 0xFFFF;
-        tcarry += tot >> 16;
+    for (int base = 0; base < n; base += MC_THREADS) {
+      const int i = base + tid;
+      int c = 0, packed = 0;
+      uint32_t owned = 0u;
+      if (i < n) {
+        c = s_mlist[i];
+        const int x = c >> 6, y = (c >> 3) & 7, z = c & 7;
+        const uint32_t emn = s_emn[mc_case_rows(s_rows, x, y, z)];
+        owned = (emn & 0xFFFu) & s_self[mc_cell_class(x, y, z)];
+        packed = __popc(owned) | (int)((emn >> 12) << 16);
       }
+      int tot;
+      const int excl = mc_block_scan(packed, s_w, &tot);
+      if (i < n) {
+        const int fi = mc_face_index(c >> 6, (c >> 3) & 7, c & 7);
+        if (fi >= 0) meta[(size_t)t * MC_FACE + fi] = ((uint32_t)(vcarry + (excl & 0xFFFF)) << 12) | owned;
+      }
+      vcarry += tot & 0xFFFF;
+      tcarry += tot >> 16;
     }
     if (tid == 0) {
       tv[t] = vcarry;
@@ -441,8 +468,8 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
 // ------------------------------------------------------------------ pass 2
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
-// The tile's cell meta is rebuilt in LDS together with a list of its vertices
-// (cell, edge) and of its triangles (cell, row entry) in output order; the
+// The mixed cells' meta is rebuilt in LDS together with a list of the tile's
+// vertices (cell, edge) and triangles (cell, row entry) in output order; the
 // lists are then spread over the threads, one vertex or triangle each, so
 // the writes are contiguous.  A tile with more vertices or triangles than
 // the lists hold is emitted cell by cell instead.  Owners in earlier tiles
@@ -456,8 +483,11 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
                                                         const int* vbase, const int* tbase, const int* totals,
                                                         double* verts, uint32_t* tris) {
   __shared__ float brick[729];
-  __shared__ uint32_t s_meta[512];  // (vertex prefix << 12) | owned edges
-  __shared__ uint16_t s_tpre[512];  // triangle prefix inside the tile
+  __shared__ uint32_t s_rows[81];
+  __shared__ uint16_t s_mlist[512];
+  __shared__ int s_n;
+  __shared__ uint32_t s_meta[512];  // (vertex prefix << 12) | owned edges, mixed cells
+  __shared__ uint16_t s_tpre[512];  // triangle prefix inside the tile, mixed cells
   __shared__ uint8_t s_cube[512];
   __shared__ uint16_t s_vlist[MC_VLIST];  // cell << 4 | edge
   __shared__ uint16_t s_tlist[MC_TLIST];  // cell << 3 | triangle of the cell
@@ -469,7 +499,6 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
   __shared__ int s_tleaf[VR_NBR];
   __shared__ int s_ttile[VR_NBR];
   __shared__ uint32_t s_proc;
-  __shared__ int s_side[2];
   __shared__ int s_w[4];
   __shared__ uint32_t s_ownsort[27 * 12];
   const int tid = threadIdx.x;
@@ -479,12 +508,11 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
   if (tid < 12) s_einfo[tid] = c_mc_edgeinfo[tid];
   const int nt = *ntiles;
   const int vmax = totals[1], tmax = totals[2];
-  const int wave = tid >> 6, y = (tid >> 3) & 7, z = tid & 7;
   int t_end, t_step;
   for (int t = xcd_first(nt, &t_end, &t_step); t < t_end; t += t_step) {
     if (tv[t] == 0 && tt[t] == 0) continue;  // nothing to write (uniform over the block)
     mc_publish(mc_fetch(tleaf, ttile, t, nt), s_tleaf, s_ttile, &s_proc);
-    if (tid < 2) s_side[tid] = 0;
+    if (tid < 81) s_rows[tid] = 0u;
     __syncthreads();
     const uint32_t processed = s_proc;
     for (int i = tid; i < 27 * 12; i += MC_THREADS) s_own[i] = (int8_t)mc_first_processed(s_ownsort[i], processed);
@@ -495,29 +523,38 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
     const int vb = vbase[t], tb = tbase[t];
-    mc_stage(g, s_tleaf, brick, iso, s_side);
+    mc_stage(g, s_tleaf, brick, iso, s_rows);
     __syncthreads();
-    // Pass A: per-cell meta, triangle prefix, case; vertex and triangle lists.
+    mc_mixed_list(s_rows, s_mlist, &s_n);
+    __syncthreads();
+    const int n = s_n;
+    // Pass A over the mixed cells: meta, triangle prefix, case; item lists.
     int vcarry = 0, tcarry = 0;
-    for (int h = 0; h < 2; ++h) {
-      const int x = wave + 4 * h;
-      const int c = h * 256 + tid;
-      const uint32_t cube = mc_case(brick, x, y, z, iso);
-      const uint32_t emn = s_emn[cube];
-      const uint32_t owned = (emn & 0xFFFu) & s_self[mc_cell_class(x, y, z)];
-      const int ntr = (int)(emn >> 12);
-      const int packed = __popc(owned) | (ntr << 16);
+    for (int base = 0; base < n; base += MC_THREADS) {
+      const int i = base + tid;
+      int c = 0, packed = 0, ntr = 0;
+      uint32_t owned = 0u, cube = 0u;
+      if (i < n) {
+        c = s_mlist[i];
+        cube = mc_case_rows(s_rows, c >> 6, (c >> 3) & 7, c & 7);
+        const uint32_t emn = s_emn[cube];
+        owned = (emn & 0xFFFu) & s_self[mc_cell_class(c >> 6, (c >> 3) & 7, c & 7)];
+        ntr = (int)(emn >> 12);
+        packed = __popc(owned) | (ntr << 16);
+      }
       int tot;
       const int excl = mc_block_scan(packed, s_w, &tot);
-      const int vp = vcarry + (excl & 0xFFFF), tp = tcarry + (excl >> 16);
-      s_meta[c] = ((uint32_t)vp << 12) | owned;
-      s_tpre[c] = (uint16_t)tp;
-      s_cube[c] = (uint8_t)cube;
-      int k = vp;
-      for (uint32_t m = owned; m; m &= m - 1u, ++k)
-        if (k < MC_VLIST) s_vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
-      for (int q = 0; q < ntr; ++q)
-        if (tp + q < MC_TLIST) s_tlist[tp + q] = (uint16_t)((c << 3) | q);
+      if (i < n) {
+        const int vp = vcarry + (excl & 0xFFFF), tp = tcarry + (excl >> 16);
+        s_meta[c] = ((uint32_t)vp << 12) | owned;
+        s_tpre[c] = (uint16_t)tp;
+        s_cube[c] = (uint8_t)cube;
+        int k = vp;
+        for (uint32_t m = owned; m; m &= m - 1u, ++k)
+          if (k < MC_VLIST) s_vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
+        for (int q = 0; q < ntr; ++q)
+          if (tp + q < MC_TLIST) s_tlist[tp + q] = (uint16_t)((c << 3) | q);
+      }
       vcarry += tot & 0xFFFF;
       tcarry += tot >> 16;
     }
@@ -543,11 +580,11 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
                                           meta, vbase, vb, vrel);
       }
     } else {
-      // Dense tile: cell by cell.
-      for (int c = tid; c < 512; c += MC_THREADS) {
+      // Dense tile: cell by cell over the mixed cells.
+      for (int i = tid; i < n; i += MC_THREADS) {
+        const int c = s_mlist[i];
         const uint32_t cube = s_cube[c];
         const int ntr = (int)(s_emn[cube] >> 12);
-        if (!ntr && !(s_meta[c] & 0xFFFu)) continue;
         const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
         const uint32_t m = s_meta[c];
         int vi = vb + (int)(m >> 12);
