@@ -126,6 +126,9 @@ def batched_extra(vr, ctx, reps=3):
         res[f"{name}_ms"] = round(1e3 * dtw, 3)
         res[f"{name}_updates_per_s"] = round(active / dtw, 1)
     res["speedup_graph_vs_loop"] = round(res["loop_ms"] / res["graph_ms"], 3)
+    counts = [b.mesh_counts(i) for i in range(len(grids))]
+    res["vertices"] = sum(v for v, _ in counts)
+    res["triangles"] = sum(t for _, t in counts)
     b.close()
     return res
 

@@ -352,6 +352,13 @@ class Batch:
     def step(self, dt, t, scheme, iso=0.0, mode=BATCH_GRAPH):
         self.ctx.check(self.ctx.lib.vr_batch_step(self.ctx.h, self.h, dt, t, scheme, iso, mode))
 
+    def mesh_counts(self, i, mode=BATCH_GRAPH):
+        """(vertex count, triangle count) of grid i's mesh from the last step."""
+        nv, nt = ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.vr_batch_mesh_counts(self.ctx.h, self.h, i, mode, ctypes.byref(nv),
+                                                         ctypes.byref(nt)))
+        return nv.value, nt.value
+
     def mesh(self, i, mode=BATCH_GRAPH):
         nv, nt = ctypes.c_uint64(), ctypes.c_uint64()
         self.ctx.check(self.ctx.lib.vr_batch_mesh_counts(self.ctx.h, self.h, i, mode, ctypes.byref(nv),

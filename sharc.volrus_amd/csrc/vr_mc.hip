@@ -41,6 +41,10 @@ __constant__ int c_mc_self[12];
 // candidates in ascending visiting-key order, one byte each: j << 5 | the
 // neighbour-tile index (0..26) that candidate lies in.
 __constant__ uint32_t c_mc_ownsort[27 * 12];
+// Per cell class and local edge: bit set of the neighbour tiles (0..26) that
+// hold a candidate ordered before the cell itself.  The cell owns the edge
+// when none of those tiles is processed.
+__constant__ uint32_t c_mc_before[27 * 12];
 __constant__ int c_dir7[7][3] = {{0, 0, 0}, {-8, 0, 0}, {8, 0, 0}, {0, -8, 0}, {0, 8, 0}, {0, 0, -8}, {0, 0, 8}};
 
 // Uploads the tables derived from the reference's EDGE_TABLE / TRI_TABLE
@@ -57,7 +61,7 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
       tri4[c * 4 + w] = v;
     }
   }
-  uint32_t cand4[48], einfo[12], ownsort[27 * 12];
+  uint32_t cand4[48], einfo[12], ownsort[27 * 12], before[27 * 12];
   int self[12];
   for (int e = 0; e < 12; ++e) {
     self[e] = 0;
@@ -90,9 +94,15 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
           ord[k] = ord[k - 1];
           ord[k - 1] = tmp;
         }
-      uint32_t v = 0;
-      for (int k = 0; k < 4; ++k) v |= (uint32_t)((ord[k] << 5) | tile[ord[k]]) << (8 * k);
+      uint32_t v = 0, b = 0;
+      bool seen_self = false;
+      for (int k = 0; k < 4; ++k) {
+        v |= (uint32_t)((ord[k] << 5) | tile[ord[k]]) << (8 * k);
+        if (ord[k] == self[e]) seen_self = true;
+        if (!seen_self) b |= 1u << tile[ord[k]];
+      }
       ownsort[c * 12 + e] = v;
+      before[c * 12 + e] = b;
     }
   hipError_t e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_emn), emn, sizeof(emn))) != hipSuccess) return e;
@@ -100,6 +110,7 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_cand4), cand4, sizeof(cand4))) != hipSuccess) return e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_edgeinfo), einfo, sizeof(einfo))) != hipSuccess) return e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_self), self, sizeof(self))) != hipSuccess) return e;
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_before), before, sizeof(before))) != hipSuccess) return e;
   return hipMemcpyToSymbol(HIP_SYMBOL(c_mc_ownsort), ownsort, sizeof(ownsort));
 }
 
@@ -217,15 +228,15 @@ __device__ __forceinline__ int mc_first_processed(uint32_t w, uint32_t processed
   return j;
 }
 
-// s_self[c] = edges that a cell of class c owns itself (tile-specific);
-// s_ownsort is c_mc_ownsort staged in LDS.
-__device__ __forceinline__ void mc_build_self(uint16_t* s_self, const uint32_t* s_ownsort, uint32_t processed,
+// s_self[c] = edges that a cell of class c owns itself (tile-specific): no
+// candidate ordered before the cell lies in a processed tile.  s_before is
+// c_mc_before staged in LDS; lane < 27 builds class lane.
+__device__ __forceinline__ void mc_build_self(uint16_t* s_self, const uint32_t* s_before, uint32_t processed,
                                               int lane) {
   if (lane < 27) {
     uint32_t m = 0;
 #pragma unroll
-    for (int e = 0; e < 12; ++e)
-      if (mc_first_processed(s_ownsort[lane * 12 + e], processed) == c_mc_self[e]) m |= 1u << e;
+    for (int e = 0; e < 12; ++e) m |= (uint32_t)((s_before[lane * 12 + e] & processed) == 0u) << e;
     s_self[lane] = (uint16_t)m;
   }
 }
@@ -292,19 +303,41 @@ __device__ __forceinline__ void mc_publish(const MCNext& n, int* s_tleaf, int* s
   }
 }
 
-// Stages the 9^3 corner values of a tile (local 0..8) into LDS and the sign
-// rows: bit bz of s_rows[bx * 9 + by] is set when the value at (bx, by, bz)
-// is below iso (the reference's corner test, volume_to_mesh.rs:115-123).
-// s_rows must be 0 on entry; read it after a synchronisation.
-__device__ __forceinline__ void mc_stage(const GridView& g, const int* s_tleaf, float* brick, float iso,
-                                         uint32_t* s_rows) {
-  for (int i = threadIdx.x; i < 729; i += MC_THREADS) {
-    const int bx = i / 81, by = (i / 9) % 9, bz = i % 9;
-    const int d = ((bx >> 3) + 1) * 9 + ((by >> 3) + 1) * 3 + ((bz >> 3) + 1);
-    const int src = s_tleaf[d];
-    const float v = src < 0 ? g.background : g.leaves[src].values[voxel_offset(bx, by, bz)];
-    brick[i] = v;
-    if (v < iso) atomicOr(&s_rows[i / 9], 1u << bz);
+// Stages a tile's 9^3 corner values (local 0..8) as 81 sign rows: bit z of
+// s_rows[x * 9 + y] is set when the value at (x, y, z) is below iso (the
+// reference's corner test, volume_to_mesh.rs:115-123), and, when brick is
+// given, the values themselves at brick[x * 81 + y * 9 + z].  Row (x, y)
+// holds z = 0..7 from one leaf (voxels (x & 7, y & 7, 0..7), 32 contiguous
+// bytes) and z = 8 from that leaf's +z neighbour.  Threads 0..161 each load
+// half a row: 2r reads z = 0..3, 2r + 1 reads z = 4..7 and z = 8; the two
+// halves combine in registers.  Absent leaves read the background.  Caller
+// synchronises before reading s_rows or brick.
+template <bool BRICK>
+__device__ __forceinline__ void mc_stage(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* s_tleaf, float iso,
+                                         uint32_t* s_rows, float* brick) {
+  const int tid = threadIdx.x;
+  if (tid >= 162) return;
+  const int r = tid >> 1, h = tid & 1;
+  const int x = (r * 57) >> 9, y = r - 9 * x;  // r / 9 for r < 81
+  const int d = ((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + 1;
+  const int row = (x & 7) * 64 + (y & 7) * 8;
+  const int la = s_tleaf[d], lb = h ? s_tleaf[d + 1] : -1;
+  float4 a = make_float4(bg, bg, bg, bg);
+  float c = bg;
+  if (la >= 0) a = *reinterpret_cast<const float4*>(&leaves[la].values[row + 4 * h]);
+  if (lb >= 0) c = leaves[lb].values[row];
+  uint32_t bits = (uint32_t)(a.x < iso) | ((uint32_t)(a.y < iso) << 1) | ((uint32_t)(a.z < iso) << 2) |
+                  ((uint32_t)(a.w < iso) << 3);
+  if (h) bits = (bits << 4) | ((uint32_t)(c < iso) << 8);
+  bits |= (uint32_t)__builtin_amdgcn_mov_dpp((int)bits, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]: lane ^ 1
+  if (!h) s_rows[r] = bits;
+  if (BRICK) {
+    float* o = brick + x * 81 + y * 9 + 4 * h;
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+    o[3] = a.w;
+    if (h) o[4] = c;
   }
 }
 
@@ -406,7 +439,6 @@ __device__ __forceinline__ int mc_tri_edge(uint32_t cube, int k) {
 __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
                                                          const int* tleaf, const int* ttile, float iso,
                                                          uint32_t* meta, int* tv, int* tt) {
-  __shared__ float brick[729];
   __shared__ uint32_t s_rows[81];
   __shared__ uint16_t s_mlist[512];
   __shared__ int s_n;
@@ -416,10 +448,10 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   __shared__ int s_ttile[VR_NBR];
   __shared__ uint32_t s_proc;
   __shared__ int s_w[4];
-  __shared__ uint32_t s_ownsort[27 * 12];
+  __shared__ uint32_t s_before[27 * 12];
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
-  for (int i = tid; i < 27 * 12; i += MC_THREADS) s_ownsort[i] = c_mc_ownsort[i];
+  for (int i = tid; i < 27 * 12; i += MC_THREADS) s_before[i] = c_mc_before[i];
   const int nt = *ntiles;
   int t_end, t_step;
   const int t0 = xcd_first(nt, &t_end, &t_step);
@@ -427,13 +459,12 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   for (int t = t0; t < t_end; t += t_step) {
     mc_publish(nx, s_tleaf, s_ttile, &s_proc);
     nx = mc_fetch(tleaf, ttile, t + t_step, t_end);
-    if (tid < 81) s_rows[tid] = 0u;
     __syncthreads();
-    mc_build_self(s_self, s_ownsort, s_proc, tid);
     const GridView& g = views[tiles[t].w];
-    mc_stage(g, s_tleaf, brick, iso, s_rows);
+    mc_stage<false>(g.leaves, g.background, s_tleaf, iso, s_rows, nullptr);
     __syncthreads();
     mc_mixed_list(s_rows, s_mlist, &s_n);
+    if (tid >= 64) mc_build_self(s_self, s_before, s_proc, tid - 64);
     __syncthreads();
     const int n = s_n;
     int vcarry = 0, tcarry = 0;
@@ -501,9 +532,13 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
   __shared__ uint32_t s_proc;
   __shared__ int s_w[4];
   __shared__ uint32_t s_ownsort[27 * 12];
+  __shared__ uint32_t s_before[27 * 12];
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
-  for (int i = tid; i < 27 * 12; i += MC_THREADS) s_ownsort[i] = c_mc_ownsort[i];
+  for (int i = tid; i < 27 * 12; i += MC_THREADS) {
+    s_ownsort[i] = c_mc_ownsort[i];
+    s_before[i] = c_mc_before[i];
+  }
   if (tid < 48) s_cand[tid] = c_mc_cand4[tid];
   if (tid < 12) s_einfo[tid] = c_mc_edgeinfo[tid];
   const int nt = *ntiles;
@@ -512,20 +547,19 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
   for (int t = xcd_first(nt, &t_end, &t_step); t < t_end; t += t_step) {
     if (tv[t] == 0 && tt[t] == 0) continue;  // nothing to write (uniform over the block)
     mc_publish(mc_fetch(tleaf, ttile, t, nt), s_tleaf, s_ttile, &s_proc);
-    if (tid < 81) s_rows[tid] = 0u;
     __syncthreads();
     const uint32_t processed = s_proc;
-    for (int i = tid; i < 27 * 12; i += MC_THREADS) s_own[i] = (int8_t)mc_first_processed(s_ownsort[i], processed);
-    mc_build_self(s_self, s_ownsort, processed, tid);
     const int4 to = tiles[t];
     const GridView& g = views[to.w];
     const double vs = g.vs;
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
     const int vb = vbase[t], tb = tbase[t];
-    mc_stage(g, s_tleaf, brick, iso, s_rows);
+    mc_stage<true>(g.leaves, g.background, s_tleaf, iso, s_rows, brick);
     __syncthreads();
     mc_mixed_list(s_rows, s_mlist, &s_n);
+    if (tid >= 64) mc_build_self(s_self, s_before, processed, tid - 64);
+    for (int i = tid; i < 27 * 12; i += MC_THREADS) s_own[i] = (int8_t)mc_first_processed(s_ownsort[i], processed);
     __syncthreads();
     const int n = s_n;
     // Pass A over the mixed cells: meta, triangle prefix, case; item lists.
