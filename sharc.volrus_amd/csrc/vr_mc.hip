@@ -228,19 +228,6 @@ __device__ __forceinline__ int mc_first_processed(uint32_t w, uint32_t processed
   return j;
 }
 
-// s_self[c] = edges that a cell of class c owns itself (tile-specific): no
-// candidate ordered before the cell lies in a processed tile.  s_before is
-// c_mc_before staged in LDS; lane < 27 builds class lane.
-__device__ __forceinline__ void mc_build_self(uint16_t* s_self, const uint32_t* s_before, uint32_t processed,
-                                              int lane) {
-  if (lane < 27) {
-    uint32_t m = 0;
-#pragma unroll
-    for (int e = 0; e < 12; ++e) m |= (uint32_t)((s_before[lane * 12 + e] & processed) == 0u) << e;
-    s_self[lane] = (uint16_t)m;
-  }
-}
-
 // Index of a cell among the 169 cells of a tile with a local coordinate 7
 // (-1 for the other 343).  An owner in another tile always has one: its tile
 // precedes the cell's, so its first differing tile offset is -1 and its local
@@ -254,132 +241,131 @@ __device__ __forceinline__ int mc_face_index(int lx, int ly, int lz) {
 #define MC_FACE 169
 
 
-// ------------------------------------------------------------------ per-tile block helpers
-// Both passes run one tile per block of MC_THREADS threads (4 waves); the
-// 512 cells of a tile are visited in two halves of 256 (cell = half * 256 +
-// tid, i.e. x = wave + 4 half, y = lane / 8, z = lane % 8), so prefix sums in
-// visiting order are block scans.
-#define MC_THREADS 256
+// ------------------------------------------------------------------ per-tile wave helpers
+// Both passes run one tile per wave: a block of MC_WAVES waves works on
+// MC_WAVES tiles at once, each wave on its own with its own LDS slots and no
+// block barriers (a tile holds ~50 mixed cells, so a wave covers it).  Prefix
+// sums in visiting order are wave scans.
+#define MC_WAVES 4
+#define MC_THREADS (64 * MC_WAVES)
 
-// Exclusive block scan of v over the 256 threads (visiting order = tid).
-__device__ __forceinline__ int mc_block_scan(int v, int* s_w, int* total) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int incl = wave_incl_scan(v);
-  if (lane == 63) s_w[wave] = incl;
-  __syncthreads();
-  const int w0 = s_w[0], w1 = s_w[1], w2 = s_w[2], w3 = s_w[3];
-  __syncthreads();
-  *total = w0 + w1 + w2 + w3;
-  const int before = (wave > 0 ? w0 : 0) + (wave > 1 ? w1 : 0) + (wave > 2 ? w2 : 0);
-  return before + incl - v;
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
-// Neighbour lists of a tile, held in registers of wave 0 (lanes < 27) so the
-// next tile's lists load while the current tile is processed.
-struct MCNext {
-  int tl = -1, tt = -1;
+// First tile of worker `sub` (of nsub per block) and the stride, with the
+// tiles cut into 8 contiguous ranges, one per XCD (block b runs on XCD
+// b % 8); plain order when the grid has fewer than 8 blocks.
+__device__ __forceinline__ int xcd_first_sub(int n, int sub, int nsub, int* end, int* stride) {
+  const int B = (int)gridDim.x, b = (int)blockIdx.x;
+  if (B < 8) {
+    *end = n;
+    *stride = B * nsub;
+    return b * nsub + sub;
+  }
+  const int x = b & 7;
+  *stride = ((B - x + 7) >> 3) * nsub;
+  *end = (int)((long long)n * (x + 1) / 8);
+  return (int)((long long)n * x / 8) + (b >> 3) * nsub + sub;
+}
+
+// Per-wave LDS of a tile.
+struct MCTileLDS {
+  uint32_t rows[81];      // sign rows: bit z of rows[x * 9 + y] = value at (x, y, z) < iso
+  int tleaf[VR_NBR];      // neighbouring leaves (-1: none)
+  int ttile[VR_NBR];      // neighbouring tiles (-1: not processed)
+  uint16_t self[28];      // per cell class: edges the cell owns itself
+  uint16_t mlist[512];    // mixed cells in visiting order
 };
 
-__device__ __forceinline__ MCNext mc_fetch(const int* tleaf, const int* ttile, int t, int nt) {
-  MCNext n;
-  if (threadIdx.x < VR_NBR && t < nt) {
-    n.tl = tleaf[(size_t)t * VR_NBR + threadIdx.x];
-    n.tt = ttile[(size_t)t * VR_NBR + threadIdx.x];
-  }
-  return n;
-}
-
-// Publishes fetched lists into LDS and the processed-neighbour mask into
-// *s_proc (wave 0).  Caller synchronises.
-__device__ __forceinline__ void mc_publish(const MCNext& n, int* s_tleaf, int* s_ttile, uint32_t* s_proc) {
-  const int tid = threadIdx.x;
-  if (tid < 64) {
-    if (tid < VR_NBR) {
-      s_tleaf[tid] = n.tl;
-      s_ttile[tid] = n.tt;
-    }
-    const uint32_t pm = (uint32_t)__ballot(tid < VR_NBR && n.tt >= 0);
-    if (tid == 0) *s_proc = pm;
-  }
-}
-
-// Stages a tile's 9^3 corner values (local 0..8) as 81 sign rows: bit z of
-// s_rows[x * 9 + y] is set when the value at (x, y, z) is below iso (the
-// reference's corner test, volume_to_mesh.rs:115-123), and, when brick is
-// given, the values themselves at brick[x * 81 + y * 9 + z].  Row (x, y)
-// holds z = 0..7 from one leaf (voxels (x & 7, y & 7, 0..7), 32 contiguous
-// bytes) and z = 8 from that leaf's +z neighbour.  Threads 0..161 each load
-// half a row: 2r reads z = 0..3, 2r + 1 reads z = 4..7 and z = 8; the two
-// halves combine in registers.  Absent leaves read the background.  Caller
-// synchronises before reading s_rows or brick.
-template <bool BRICK>
-__device__ __forceinline__ void mc_stage(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* s_tleaf, float iso,
-                                         uint32_t* s_rows, float* brick) {
-  const int tid = threadIdx.x;
-  if (tid >= 162) return;
-  const int r = tid >> 1, h = tid & 1;
+// Half row h (0..161) of a tile's 9^3 corner values (local 0..8): row
+// r = h / 2 = x * 9 + y holds z = 0..7 from one leaf (voxels (x & 7, y & 7,
+// 0..7), 32 contiguous bytes) and z = 8 from that leaf's +z neighbour; even
+// h reads z = 0..3, odd h reads z = 4..7 and z = 8.  Sets rows[r] to the
+// corner signs (the reference's corner test, volume_to_mesh.rs:115-123);
+// the two halves sit in adjacent lanes and combine in registers, so lanes
+// h and h ^ 1 must both be active.  Absent leaves read the background.
+__device__ __forceinline__ void mc_stage_half(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, float iso,
+                                              int h, uint32_t* rows) {
+  const int r = h >> 1, odd = h & 1;
   const int x = (r * 57) >> 9, y = r - 9 * x;  // r / 9 for r < 81
   const int d = ((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + 1;
   const int row = (x & 7) * 64 + (y & 7) * 8;
-  const int la = s_tleaf[d], lb = h ? s_tleaf[d + 1] : -1;
+  const int la = tl[d], lb = odd ? tl[d + 1] : -1;
   float4 a = make_float4(bg, bg, bg, bg);
   float c = bg;
-  if (la >= 0) a = *reinterpret_cast<const float4*>(&leaves[la].values[row + 4 * h]);
+  if (la >= 0) a = *reinterpret_cast<const VR_GLOBAL float4*>(&leaves[la].values[row + 4 * odd]);
   if (lb >= 0) c = leaves[lb].values[row];
   uint32_t bits = (uint32_t)(a.x < iso) | ((uint32_t)(a.y < iso) << 1) | ((uint32_t)(a.z < iso) << 2) |
                   ((uint32_t)(a.w < iso) << 3);
-  if (h) bits = (bits << 4) | ((uint32_t)(c < iso) << 8);
+  if (odd) bits = (bits << 4) | ((uint32_t)(c < iso) << 8);
   bits |= (uint32_t)__builtin_amdgcn_mov_dpp((int)bits, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]: lane ^ 1
-  if (!h) s_rows[r] = bits;
-  if (BRICK) {
-    float* o = brick + x * 81 + y * 9 + 4 * h;
-    o[0] = a.x;
-    o[1] = a.y;
-    o[2] = a.z;
-    o[3] = a.w;
-    if (h) o[4] = c;
-  }
+  if (!odd) rows[r] = bits;
 }
 
-// The tile's mixed cells (corners on both sides of iso: a case other than 0
-// or 255, so crossing edges), listed in visiting order (x, y, z) by wave 0:
-// lane x * 8 + y tests its 8 cells along z with bit operations on the four
-// sign rows, and a prefix of the counts places them.  Returns the count on
-// every lane of wave 0 and stores it in *s_n.  Caller synchronises.
-__device__ __forceinline__ void mc_mixed_list(const uint32_t* s_rows, uint16_t* s_list, int* s_n) {
-  const int lane = threadIdx.x;
-  if (lane >= 64) return;
+// Loads the tile's neighbour lists (fetched into lanes < 27), stages its
+// sign rows, lists its mixed cells (corners on both sides of iso: a case
+// other than 0 or 255) in visiting order and builds the per-class self
+// masks (no candidate ordered before the cell lies in a processed tile;
+// s_before is c_mc_before in LDS).  Returns the mixed-cell count.  Whole
+// wave, uniform control flow.
+__device__ __forceinline__ int mc_tile_setup(const VR_GLOBAL NanoLeaf* leaves, float bg, float iso, int tl, int tt,
+                                             uint32_t processed, const uint32_t* s_before, MCTileLDS& w) {
+  const int lane = lane_id();
+  if (lane < VR_NBR) {
+    w.tleaf[lane] = tl;
+    w.ttile[lane] = tt;
+  }
+  wave_lds_sync();
+  mc_stage_half(leaves, bg, w.tleaf, iso, lane, w.rows);
+  mc_stage_half(leaves, bg, w.tleaf, iso, lane + 64, w.rows);
+  if (lane < 162 - 128) mc_stage_half(leaves, bg, w.tleaf, iso, lane + 128, w.rows);
+  if (lane < 27) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int e = 0; e < 12; ++e) m |= (uint32_t)((s_before[lane * 12 + e] & processed) == 0u) << e;
+    w.self[lane] = (uint16_t)m;
+  }
+  wave_lds_sync();
+  // Lane x * 8 + y tests its 8 cells along z with bit operations on the
+  // four sign rows around it.
   const int x = lane >> 3, y = lane & 7;
-  const uint32_t r00 = s_rows[x * 9 + y], r10 = s_rows[(x + 1) * 9 + y];
-  const uint32_t r01 = s_rows[x * 9 + y + 1], r11 = s_rows[(x + 1) * 9 + y + 1];
+  const uint32_t r00 = w.rows[x * 9 + y], r10 = w.rows[(x + 1) * 9 + y];
+  const uint32_t r01 = w.rows[x * 9 + y + 1], r11 = w.rows[(x + 1) * 9 + y + 1];
   const uint32_t all = r00 & r10 & r01 & r11, any = r00 | r10 | r01 | r11;
   const uint32_t one = all & (all >> 1), zero = ~any & ~(any >> 1);
   uint32_t mixed = ~(one | zero) & 0xFFu;  // bit z: cell (x, y, z)
   const int cnt = __popc(mixed);
   const int incl = wave_incl_scan(cnt);
   int k = incl - cnt;
-  for (; mixed; mixed &= mixed - 1u, ++k) s_list[k] = (uint16_t)(lane * 8 + (__ffs(mixed) - 1));
-  if (lane == 63) *s_n = incl;
+  for (; mixed; mixed &= mixed - 1u, ++k) w.mlist[k] = (uint16_t)(lane * 8 + (__ffs(mixed) - 1));
+  wave_lds_sync();
+  return __builtin_amdgcn_readlane(incl, 63);
 }
 
 // Cube index of cell c = x * 64 + y * 8 + z from the sign rows, corners in
 // the reference's order (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1)
 // (1,1,1) (0,1,1).
-__device__ __forceinline__ uint32_t mc_case_rows(const uint32_t* s_rows, int x, int y, int z) {
-  const uint32_t r00 = s_rows[x * 9 + y] >> z, r10 = s_rows[(x + 1) * 9 + y] >> z;
-  const uint32_t r11 = s_rows[(x + 1) * 9 + y + 1] >> z, r01 = s_rows[x * 9 + y + 1] >> z;
+__device__ __forceinline__ uint32_t mc_case_rows(const uint32_t* rows, int x, int y, int z) {
+  const uint32_t r00 = rows[x * 9 + y] >> z, r10 = rows[(x + 1) * 9 + y] >> z;
+  const uint32_t r11 = rows[(x + 1) * 9 + y + 1] >> z, r01 = rows[x * 9 + y + 1] >> z;
   return (r00 & 1u) | ((r10 & 1u) << 1) | ((r11 & 1u) << 2) | ((r01 & 1u) << 3) | (((r00 >> 1) & 1u) << 4) |
          (((r10 >> 1) & 1u) << 5) | (((r11 >> 1) & 1u) << 6) | (((r01 >> 1) & 1u) << 7);
 }
 
 // Global index of the vertex on local edge e of cell (x, y, z) of tile t,
-// relative to the tile's grid: its owner's meta gives the owner cell's first
-// vertex and owned-edge mask; -1 if no owner (not reached for edges of a
-// crossing cell of a processed tile).
-__device__ __forceinline__ int mc_edge_vertex(int x, int y, int z, int e, int cls, const int8_t* s_own,
-                                              const uint32_t* s_cand, const uint32_t* s_meta, const int* s_ttile,
-                                              const uint32_t* meta, const int* vbase, int vb, int vrel) {
-  const int j = s_own[cls * 12 + e];
+// relative to the tile's grid.  The owner is the first candidate of
+// (class, edge) in visiting-key order whose tile is processed; its meta
+// gives its first vertex and owned-edge mask (LDS for this tile, the face
+// meta of pass 1 for an earlier tile).  0 if no owner (not reached for
+// edges of a crossing cell of a processed tile).
+__device__ __forceinline__ int mc_edge_vertex(int x, int y, int z, int e, int cls, uint32_t processed,
+                                              const uint32_t* s_ownsort, const uint32_t* s_cand,
+                                              const uint32_t* s_meta, const int* ttile, const uint32_t* meta,
+                                              const int* vbase, int vb, int vrel) {
+  const int j = mc_first_processed(s_ownsort[cls * 12 + e], processed);
   if (j < 0) return 0;
   const uint32_t cw = s_cand[e * 4 + j];
   const int nx = x + (int)(int8_t)(cw & 0xFFu), ny = y + (int)(int8_t)((cw >> 8) & 0xFFu),
@@ -392,7 +378,7 @@ __device__ __forceinline__ int mc_edge_vertex(int x, int y, int z, int e, int cl
     om = s_meta[(nx << 6) + (ny << 3) + nz];
     obase = vb;
   } else {
-    const int ot = s_ttile[d];
+    const int ot = ttile[d];
     const int fi = mc_face_index(nx & 7, ny & 7, nz & 7);
     if (ot < 0 || fi < 0) return 0;
     om = meta[(size_t)ot * MC_FACE + fi];
@@ -401,33 +387,35 @@ __device__ __forceinline__ int mc_edge_vertex(int x, int y, int z, int e, int cl
   return obase - vrel + (int)(om >> 12) + __popc((om & 0xFFFu) & ((1u << ep) - 1u));
 }
 
+// Corner value at tile-local (x, y, z), 0..8 each.
+__device__ __forceinline__ float mc_corner(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, int x, int y,
+                                           int z) {
+  const int l = tl[((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + ((z >> 3) + 1)];
+  return l < 0 ? bg : leaves[l].values[voxel_offset(x, y, z)];
+}
+
 // Vertex on owned local edge e of cell (x, y, z) (volume_to_mesh.rs:162-186):
 // t = (iso - va) / (vb - va) in f32 (0.5 when |vb - va| <= 1e-10), position
 // pa + t (pb - pa) in f64 from the corners' world positions (index * vs).
-__device__ __forceinline__ void mc_write_vertex(const float* brick, int x, int y, int z, int e, const int4& to,
-                                                double vs, float iso, const uint32_t* s_einfo, double* out) {
+__device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, int x, int y,
+                                                int z, int e, const int4& to, double vs, float iso,
+                                                const uint32_t* s_einfo, double* out) {
   const uint32_t ei = s_einfo[e];
   const int ao = (int)((ei >> 6) & 7u), bo = (int)((ei >> 9) & 7u);
-  const int bb = x * 81 + y * 9 + z;
-  const float va = brick[bb + (ao & 1) * 81 + ((ao >> 1) & 1) * 9 + (ao >> 2)];
-  const float vbv = brick[bb + (bo & 1) * 81 + ((bo >> 1) & 1) * 9 + (bo >> 2)];
+  const int ax = x + (ao & 1), ay = y + ((ao >> 1) & 1), az = z + (ao >> 2);
+  const int bx = x + (bo & 1), by = y + ((bo >> 1) & 1), bz = z + (bo >> 2);
+  const float va = mc_corner(leaves, bg, tl, ax, ay, az);
+  const float vbv = mc_corner(leaves, bg, tl, bx, by, bz);
   double tpar;
   if (fabsf(vbv - va) > 1e-10f)
     tpar = (double)((iso - va) / (vbv - va));
   else
     tpar = 0.5;
-  const double pax = (double)(to.x + x + (ao & 1)) * vs, pay = (double)(to.y + y + ((ao >> 1) & 1)) * vs,
-               paz = (double)(to.z + z + (ao >> 2)) * vs;
-  const double pbx = (double)(to.x + x + (bo & 1)) * vs, pby = (double)(to.y + y + ((bo >> 1) & 1)) * vs,
-               pbz = (double)(to.z + z + (bo >> 2)) * vs;
+  const double pax = (double)(to.x + ax) * vs, pay = (double)(to.y + ay) * vs, paz = (double)(to.z + az) * vs;
+  const double pbx = (double)(to.x + bx) * vs, pby = (double)(to.y + by) * vs, pbz = (double)(to.z + bz) * vs;
   out[0] = pax + tpar * (pbx - pax);
   out[1] = pay + tpar * (pby - pay);
   out[2] = paz + tpar * (pbz - paz);
-}
-
-// Edge k (0..14) of a case's TRI_TABLE row.
-__device__ __forceinline__ int mc_tri_edge(uint32_t cube, int k) {
-  return (int)(int8_t)((c_mc_tri4[cube * 4 + (k >> 2)] >> (8 * (k & 3))) & 0xFFu);
 }
 
 // ------------------------------------------------------------------ pass 1
@@ -435,204 +423,166 @@ __device__ __forceinline__ int mc_tri_edge(uint32_t cube, int k) {
 // count.  The face cells (mc_face_index >= 0) store meta = (vertex prefix
 // inside the tile << 12) | owned-edge mask for the emit pass of later tiles;
 // per-tile vertex and triangle totals.  Cells with case 0 or 255 add nothing
-// and are never visited.  Blocks loop over tiles.
+// and are never visited.  Each wave loops over tiles; the next tile's
+// neighbour lists load while the current one is processed.
 __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
                                                          const int* tleaf, const int* ttile, float iso,
                                                          uint32_t* meta, int* tv, int* tt) {
-  __shared__ uint32_t s_rows[81];
-  __shared__ uint16_t s_mlist[512];
-  __shared__ int s_n;
   __shared__ uint16_t s_emn[256];
-  __shared__ uint16_t s_self[27];
-  __shared__ int s_tleaf[VR_NBR];
-  __shared__ int s_ttile[VR_NBR];
-  __shared__ uint32_t s_proc;
-  __shared__ int s_w[4];
   __shared__ uint32_t s_before[27 * 12];
-  const int tid = threadIdx.x;
+  __shared__ MCTileLDS s_w[MC_WAVES];
+  const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
   for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
   for (int i = tid; i < 27 * 12; i += MC_THREADS) s_before[i] = c_mc_before[i];
+  __syncthreads();
+  MCTileLDS& w = s_w[wv];
   const int nt = *ntiles;
   int t_end, t_step;
-  const int t0 = xcd_first(nt, &t_end, &t_step);
-  MCNext nx = mc_fetch(tleaf, ttile, t0, t_end);
-  for (int t = t0; t < t_end; t += t_step) {
-    mc_publish(nx, s_tleaf, s_ttile, &s_proc);
-    nx = mc_fetch(tleaf, ttile, t + t_step, t_end);
-    __syncthreads();
+  int t = xcd_first_sub(nt, wv, MC_WAVES, &t_end, &t_step);
+  int ntl = -1, ntt = -1;
+  if (t < t_end && lane < VR_NBR) {
+    ntl = tleaf[(size_t)t * VR_NBR + lane];
+    ntt = ttile[(size_t)t * VR_NBR + lane];
+  }
+  for (; t < t_end; t += t_step) {
+    const int tl = ntl, ttv = ntt;
+    const uint32_t processed = (uint32_t)__ballot(lane < VR_NBR && ttv >= 0);
+    const int tn = t + t_step;
+    if (tn < t_end && lane < VR_NBR) {
+      ntl = tleaf[(size_t)tn * VR_NBR + lane];
+      ntt = ttile[(size_t)tn * VR_NBR + lane];
+    }
     const GridView& g = views[tiles[t].w];
-    mc_stage<false>(g.leaves, g.background, s_tleaf, iso, s_rows, nullptr);
-    __syncthreads();
-    mc_mixed_list(s_rows, s_mlist, &s_n);
-    if (tid >= 64) mc_build_self(s_self, s_before, s_proc, tid - 64);
-    __syncthreads();
-    const int n = s_n;
+    const int n = mc_tile_setup(g.leaves, g.background, iso, tl, ttv, processed, s_before, w);
     int vcarry = 0, tcarry = 0;
-    for (int base = 0; base < n; base += MC_THREADS) {
-      const int i = base + tid;
+    for (int base = 0; base < n; base += 64) {
+      const int i = base + lane;
       int c = 0, packed = 0;
       uint32_t owned = 0u;
       if (i < n) {
-        c = s_mlist[i];
+        c = w.mlist[i];
         const int x = c >> 6, y = (c >> 3) & 7, z = c & 7;
-        const uint32_t emn = s_emn[mc_case_rows(s_rows, x, y, z)];
-        owned = (emn & 0xFFFu) & s_self[mc_cell_class(x, y, z)];
+        const uint32_t emn = s_emn[mc_case_rows(w.rows, x, y, z)];
+        owned = (emn & 0xFFFu) & w.self[mc_cell_class(x, y, z)];
         packed = __popc(owned) | (int)((emn >> 12) << 16);
       }
-      int tot;
-      const int excl = mc_block_scan(packed, s_w, &tot);
+      const int incl = wave_incl_scan(packed);
       if (i < n) {
         const int fi = mc_face_index(c >> 6, (c >> 3) & 7, c & 7);
-        if (fi >= 0) meta[(size_t)t * MC_FACE + fi] = ((uint32_t)(vcarry + (excl & 0xFFFF)) << 12) | owned;
+        if (fi >= 0) meta[(size_t)t * MC_FACE + fi] = ((uint32_t)(vcarry + ((incl - packed) & 0xFFFF)) << 12) | owned;
       }
+      const int tot = __builtin_amdgcn_readlane(incl, 63);
       vcarry += tot & 0xFFFF;
       tcarry += tot >> 16;
     }
-    if (tid == 0) {
+    if (lane == 0) {
       tv[t] = vcarry;
       tt[t] = tcarry;
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
 // ------------------------------------------------------------------ pass 2
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
-// The mixed cells' meta is rebuilt in LDS together with a list of the tile's
-// vertices (cell, edge) and triangles (cell, row entry) in output order; the
-// lists are then spread over the threads, one vertex or triangle each, so
-// the writes are contiguous.  A tile with more vertices or triangles than
-// the lists hold is emitted cell by cell instead.  Owners in earlier tiles
-// are read from their face meta.  Vertex indices are relative to the tile's
-// grid.
-#define MC_VLIST 1536
-#define MC_TLIST 2048
+// One lane per mixed cell: a first sweep writes the cell's meta and its
+// owned vertices at the scanned prefix, a second its triangles, whose
+// vertices come from the owners' meta (this tile's in LDS, earlier tiles'
+// face meta).  Vertex indices are relative to the tile's grid.
 __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, const int4* tiles, const int* ntiles,
                                                         const int* tleaf, const int* ttile, const int* tile_base,
                                                         float iso, const uint32_t* meta, const int* tv, const int* tt,
                                                         const int* vbase, const int* tbase, const int* totals,
                                                         double* verts, uint32_t* tris) {
-  __shared__ float brick[729];
-  __shared__ uint32_t s_rows[81];
-  __shared__ uint16_t s_mlist[512];
-  __shared__ int s_n;
-  __shared__ uint32_t s_meta[512];  // (vertex prefix << 12) | owned edges, mixed cells
-  __shared__ uint16_t s_tpre[512];  // triangle prefix inside the tile, mixed cells
-  __shared__ uint8_t s_cube[512];
-  __shared__ uint16_t s_vlist[MC_VLIST];  // cell << 4 | edge
-  __shared__ uint16_t s_tlist[MC_TLIST];  // cell << 3 | triangle of the cell
-  __shared__ int8_t s_own[27 * 12];
-  __shared__ uint16_t s_self[27];
   __shared__ uint16_t s_emn[256];
-  __shared__ uint32_t s_cand[48];
-  __shared__ uint32_t s_einfo[12];
-  __shared__ int s_tleaf[VR_NBR];
-  __shared__ int s_ttile[VR_NBR];
-  __shared__ uint32_t s_proc;
-  __shared__ int s_w[4];
+  __shared__ __align__(16) int8_t s_tri[256 * 16];  // TRI_TABLE rows
   __shared__ uint32_t s_ownsort[27 * 12];
   __shared__ uint32_t s_before[27 * 12];
-  const int tid = threadIdx.x;
+  __shared__ uint32_t s_cand[48];
+  __shared__ uint32_t s_einfo[12];
+  __shared__ MCTileLDS s_w[MC_WAVES];
+  __shared__ uint32_t s_meta[MC_WAVES][512];  // (vertex prefix << 12) | owned edges, mixed cells
+  __shared__ uint16_t s_tpre[MC_WAVES][512];  // triangle prefix, by mixed-list index
+  const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
   for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
+  for (int i = tid; i < 256 * 4; i += MC_THREADS) reinterpret_cast<uint32_t*>(s_tri)[i] = c_mc_tri4[i];
   for (int i = tid; i < 27 * 12; i += MC_THREADS) {
     s_ownsort[i] = c_mc_ownsort[i];
     s_before[i] = c_mc_before[i];
   }
   if (tid < 48) s_cand[tid] = c_mc_cand4[tid];
   if (tid < 12) s_einfo[tid] = c_mc_edgeinfo[tid];
+  __syncthreads();
+  MCTileLDS& w = s_w[wv];
+  uint32_t* sm = s_meta[wv];
+  uint16_t* stp = s_tpre[wv];
   const int nt = *ntiles;
   const int vmax = totals[1], tmax = totals[2];
   int t_end, t_step;
-  for (int t = xcd_first(nt, &t_end, &t_step); t < t_end; t += t_step) {
-    if (tv[t] == 0 && tt[t] == 0) continue;  // nothing to write (uniform over the block)
-    mc_publish(mc_fetch(tleaf, ttile, t, nt), s_tleaf, s_ttile, &s_proc);
-    __syncthreads();
-    const uint32_t processed = s_proc;
+  for (int t = xcd_first_sub(nt, wv, MC_WAVES, &t_end, &t_step); t < t_end; t += t_step) {
+    if (tv[t] == 0 && tt[t] == 0) continue;  // nothing to write (uniform over the wave)
+    int tl = -1, ttv = -1;
+    if (lane < VR_NBR) {
+      tl = tleaf[(size_t)t * VR_NBR + lane];
+      ttv = ttile[(size_t)t * VR_NBR + lane];
+    }
+    const uint32_t processed = (uint32_t)__ballot(lane < VR_NBR && ttv >= 0);
     const int4 to = tiles[t];
     const GridView& g = views[to.w];
+    const VR_GLOBAL NanoLeaf* leaves = g.leaves;
+    const float bg = g.background;
     const double vs = g.vs;
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
     const int vb = vbase[t], tb = tbase[t];
-    mc_stage<true>(g.leaves, g.background, s_tleaf, iso, s_rows, brick);
-    __syncthreads();
-    mc_mixed_list(s_rows, s_mlist, &s_n);
-    if (tid >= 64) mc_build_self(s_self, s_before, processed, tid - 64);
-    for (int i = tid; i < 27 * 12; i += MC_THREADS) s_own[i] = (int8_t)mc_first_processed(s_ownsort[i], processed);
-    __syncthreads();
-    const int n = s_n;
-    // Pass A over the mixed cells: meta, triangle prefix, case; item lists.
+    const int n = mc_tile_setup(leaves, bg, iso, tl, ttv, processed, s_before, w);
+    // Sweep 1: meta, triangle prefix, owned vertices.
     int vcarry = 0, tcarry = 0;
-    for (int base = 0; base < n; base += MC_THREADS) {
-      const int i = base + tid;
-      int c = 0, packed = 0, ntr = 0;
-      uint32_t owned = 0u, cube = 0u;
+    for (int base = 0; base < n; base += 64) {
+      const int i = base + lane;
+      int c = 0, packed = 0;
+      uint32_t owned = 0u;
       if (i < n) {
-        c = s_mlist[i];
-        cube = mc_case_rows(s_rows, c >> 6, (c >> 3) & 7, c & 7);
-        const uint32_t emn = s_emn[cube];
-        owned = (emn & 0xFFFu) & s_self[mc_cell_class(c >> 6, (c >> 3) & 7, c & 7)];
-        ntr = (int)(emn >> 12);
-        packed = __popc(owned) | (ntr << 16);
+        c = w.mlist[i];
+        const uint32_t emn = s_emn[mc_case_rows(w.rows, c >> 6, (c >> 3) & 7, c & 7)];
+        owned = (emn & 0xFFFu) & w.self[mc_cell_class(c >> 6, (c >> 3) & 7, c & 7)];
+        packed = __popc(owned) | (int)((emn >> 12) << 16);
       }
-      int tot;
-      const int excl = mc_block_scan(packed, s_w, &tot);
+      const int incl = wave_incl_scan(packed);
       if (i < n) {
-        const int vp = vcarry + (excl & 0xFFFF), tp = tcarry + (excl >> 16);
-        s_meta[c] = ((uint32_t)vp << 12) | owned;
-        s_tpre[c] = (uint16_t)tp;
-        s_cube[c] = (uint8_t)cube;
-        int k = vp;
-        for (uint32_t m = owned; m; m &= m - 1u, ++k)
-          if (k < MC_VLIST) s_vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
-        for (int q = 0; q < ntr; ++q)
-          if (tp + q < MC_TLIST) s_tlist[tp + q] = (uint16_t)((c << 3) | q);
+        const int excl = incl - packed;
+        const int vp = vcarry + (excl & 0xFFFF);
+        sm[c] = ((uint32_t)vp << 12) | owned;
+        stp[i] = (uint16_t)(tcarry + (excl >> 16));
+        int vi = vb + vp;
+        for (uint32_t m = owned; m && vi < vmax; m &= m - 1u, ++vi)
+          mc_write_vertex(leaves, bg, w.tleaf, c >> 6, (c >> 3) & 7, c & 7, __ffs(m) - 1, to, vs, iso, s_einfo,
+                          verts + 3 * (size_t)vi);
       }
+      const int tot = __builtin_amdgcn_readlane(incl, 63);
       vcarry += tot & 0xFFFF;
       tcarry += tot >> 16;
     }
-    __syncthreads();
-    if (vcarry <= MC_VLIST && tcarry <= MC_TLIST) {
-      for (int i = tid; i < vcarry; i += MC_THREADS) {
-        const int vi = vb + i;
-        if (vi >= vmax) break;
-        const int it = s_vlist[i], c = it >> 4;
-        mc_write_vertex(brick, c >> 6, (c >> 3) & 7, c & 7, it & 15, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
-      }
-      for (int i = tid; i < tcarry; i += MC_THREADS) {
-        const int ti = tb + i;
-        if (ti >= tmax) break;
-        const int it = s_tlist[i], c = it >> 3, q = it & 7;
-        const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
-        const uint32_t cube = s_cube[c];
-        const int cls = mc_cell_class(cx, cy, cz);
+    wave_lds_sync();
+    // Sweep 2: triangles.
+    for (int i = lane; i < n; i += 64) {
+      const int c = w.mlist[i];
+      const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
+      const uint32_t cube = mc_case_rows(w.rows, cx, cy, cz);
+      const int ntr = (int)(s_emn[cube] >> 12);
+      const int cls = mc_cell_class(cx, cy, cz);
+      const int8_t* row = s_tri + cube * 16;
+      int ti = tb + (int)stp[i];
+      for (int q = 0; q < ntr && ti < tmax; ++q, ++ti) {
         uint32_t* o = tris + 3 * (size_t)ti;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          o[k] = (uint32_t)mc_edge_vertex(cx, cy, cz, mc_tri_edge(cube, 3 * q + k), cls, s_own, s_cand, s_meta, s_ttile,
+          o[k] = (uint32_t)mc_edge_vertex(cx, cy, cz, row[3 * q + k], cls, processed, s_ownsort, s_cand, sm, w.ttile,
                                           meta, vbase, vb, vrel);
       }
-    } else {
-      // Dense tile: cell by cell over the mixed cells.
-      for (int i = tid; i < n; i += MC_THREADS) {
-        const int c = s_mlist[i];
-        const uint32_t cube = s_cube[c];
-        const int ntr = (int)(s_emn[cube] >> 12);
-        const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
-        const uint32_t m = s_meta[c];
-        int vi = vb + (int)(m >> 12);
-        for (uint32_t r = m & 0xFFFu; r && vi < vmax; r &= r - 1u, ++vi)
-          mc_write_vertex(brick, cx, cy, cz, __ffs(r) - 1, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
-        const int cls = mc_cell_class(cx, cy, cz);
-        int ti = tb + (int)s_tpre[c];
-        for (int q = 0; q < ntr && ti < tmax; ++q, ++ti)
-          for (int k = 0; k < 3; ++k)
-            tris[3 * (size_t)ti + k] = (uint32_t)mc_edge_vertex(cx, cy, cz, mc_tri_edge(cube, 3 * q + k), cls, s_own,
-                                                                s_cand, s_meta, s_ttile, meta, vbase, vb, vrel);
-      }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
@@ -659,10 +609,15 @@ static int blocks_for(size_t n, int b) {
 }
 
 // Blocks that loop over the tiles (tables are loaded into LDS once per
-// block): up to 8 per compute unit.
-static int mc_blocks(const vr_ctx* ctx, int tcap) {
-  const int cap = 8 * (ctx->cus > 0 ? ctx->cus : 256);
-  return tcap < 1 ? 1 : (tcap < cap ? tcap : cap);
+// block, MC_WAVES tiles in flight per block): as many as are resident at
+// once, since the tiles are split statically between them.
+static int mc_blocks(const vr_ctx* ctx, const void* kernel, int tcap) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, MC_THREADS, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const int cap = per_cu * (ctx->cus > 0 ? ctx->cus : 256);
+  const int need = (tcap + MC_WAVES - 1) / MC_WAVES;
+  return need < 1 ? 1 : (need < cap ? need : cap);
 }
 
 // Phase A: tile list for G grids (leaf counts known on the host: ltot).
@@ -708,7 +663,7 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
   if ((e = hipGetLastError()) != hipSuccess) return vr_hip_fail(ctx, e, "mc tiles");
   {
     TimedLaunch tl(ctx, TC_MC, algorithmic_bytes);
-    hipLaunchKernelGGL(k_mc_count, dim3(mc_blocks(ctx, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
+    hipLaunchKernelGGL(k_mc_count, dim3(mc_blocks(ctx, (const void*)k_mc_count, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        totals, s.tleaf.as<int>(), s.ttile.as<int>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(),
                        s.tt.as<int>());
   }
@@ -729,7 +684,7 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, fl
   hipError_t e;
   {
     TimedLaunch tl(ctx, TC_MC, algorithmic_bytes);
-    hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
+    hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, (const void*)k_mc_emit, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(), iso,
                        s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
                        s.totals.as<int>(), verts, tris);
