@@ -366,37 +366,40 @@ __device__ __forceinline__ int mc_edge_vertex(int x, int y, int z, int e, int cl
                                               const uint32_t* s_meta, const int* ttile, const uint32_t* meta,
                                               const int* vbase, int vb, int vrel) {
   const int j = mc_first_processed(s_ownsort[cls * 12 + e], processed);
-  if (j < 0) return 0;
-  const uint32_t cw = s_cand[e * 4 + j];
+  const uint32_t cw = s_cand[e * 4 + (j < 0 ? 0 : j)];
   const int nx = x + (int)(int8_t)(cw & 0xFFu), ny = y + (int)(int8_t)((cw >> 8) & 0xFFu),
             nz = z + (int)(int8_t)((cw >> 16) & 0xFFu);
   const int ep = (int)(int8_t)(cw >> 24);
   const int d = ((nx >> 3) + 1) * 9 + ((ny >> 3) + 1) * 3 + ((nz >> 3) + 1);
-  uint32_t om;
-  int obase;
-  if (d == 13) {
-    om = s_meta[(nx << 6) + (ny << 3) + nz];
-    obase = vb;
-  } else {
-    const int ot = ttile[d];
-    const int fi = mc_face_index(nx & 7, ny & 7, nz & 7);
-    if (ot < 0 || fi < 0) return 0;
-    om = meta[(size_t)ot * MC_FACE + fi];
-    obase = vbase[ot];
-  }
-  return obase - vrel + (int)(om >> 12) + __popc((om & 0xFFFu) & ((1u << ep) - 1u));
+  // Both sources are read unconditionally (the global one at index 0 when
+  // the owner is in this tile) so that the lookups of a triangle overlap.
+  const bool ext = d != 13;
+  const int ot = ttile[d];
+  const int fi = mc_face_index(nx & 7, ny & 7, nz & 7);
+  const bool ok = j >= 0 && (!ext || (ot >= 0 && fi >= 0));
+  const int go = ext && ok ? ot : 0;
+  const uint32_t omg = meta[(size_t)go * MC_FACE + (ext && ok ? fi : 0)];
+  const int obg = vbase[go];
+  const uint32_t oml = s_meta[((nx & 7) << 6) + ((ny & 7) << 3) + (nz & 7)];
+  const uint32_t om = ext ? omg : oml;
+  const int obase = ext ? obg : vb;
+  return ok ? obase - vrel + (int)(om >> 12) + __popc((om & 0xFFFu) & ((1u << ep) - 1u)) : 0;
 }
 
-// Corner value at tile-local (x, y, z), 0..8 each.
+// Corner value at tile-local (x, y, z), 0..8 each (leaf 0 is read and
+// discarded where the leaf is absent, so the two corner loads overlap).
 __device__ __forceinline__ float mc_corner(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, int x, int y,
                                            int z) {
   const int l = tl[((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + ((z >> 3) + 1)];
-  return l < 0 ? bg : leaves[l].values[voxel_offset(x, y, z)];
+  const float v = leaves[l < 0 ? 0 : l].values[voxel_offset(x, y, z)];
+  return l < 0 ? bg : v;
 }
 
 // Vertex on owned local edge e of cell (x, y, z) (volume_to_mesh.rs:162-186):
 // t = (iso - va) / (vb - va) in f32 (0.5 when |vb - va| <= 1e-10), position
 // pa + t (pb - pa) in f64 from the corners' world positions (index * vs).
+// An edge spans one axis; on the other two pb - pa is +0 exactly, so
+// pa + t (pb - pa) is pa there and only the spanned axis is interpolated.
 __device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, int x, int y,
                                                 int z, int e, const int4& to, double vs, float iso,
                                                 const uint32_t* s_einfo, double* out) {
@@ -411,11 +414,15 @@ __device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves
     tpar = (double)((iso - va) / (vbv - va));
   else
     tpar = 0.5;
+  const int span = ao ^ bo;  // 1, 2 or 4: the axis of the edge
   const double pax = (double)(to.x + ax) * vs, pay = (double)(to.y + ay) * vs, paz = (double)(to.z + az) * vs;
-  const double pbx = (double)(to.x + bx) * vs, pby = (double)(to.y + by) * vs, pbz = (double)(to.z + bz) * vs;
-  out[0] = pax + tpar * (pbx - pax);
-  out[1] = pay + tpar * (pby - pay);
-  out[2] = paz + tpar * (pbz - paz);
+  const int ia = span == 1 ? to.x + ax : (span == 2 ? to.y + ay : to.z + az);
+  const int ib = span == 1 ? to.x + bx : (span == 2 ? to.y + by : to.z + bz);
+  const double pa = (double)ia * vs, pb = (double)ib * vs;
+  const double p = pa + tpar * (pb - pa);
+  out[0] = span == 1 ? p : pax;
+  out[1] = span == 2 ? p : pay;
+  out[2] = span == 4 ? p : paz;
 }
 
 // ------------------------------------------------------------------ pass 1
@@ -431,7 +438,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   __shared__ uint16_t s_emn[256];
   __shared__ uint32_t s_before[27 * 12];
   __shared__ MCTileLDS s_w[MC_WAVES];
-  const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+  const int tid = threadIdx.x, lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
   for (int i = tid; i < 27 * 12; i += MC_THREADS) s_before[i] = c_mc_before[i];
   __syncthreads();
@@ -486,28 +493,41 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
 // ------------------------------------------------------------------ pass 2
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
-// One lane per mixed cell: a first sweep writes the cell's meta and its
-// owned vertices at the scanned prefix, a second its triangles, whose
-// vertices come from the owners' meta (this tile's in LDS, earlier tiles'
-// face meta).  Vertex indices are relative to the tile's grid.
-__global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, const int4* tiles, const int* ntiles,
-                                                        const int* tleaf, const int* ttile, const int* tile_base,
-                                                        float iso, const uint32_t* meta, const int* tv, const int* tt,
-                                                        const int* vbase, const int* tbase, const int* totals,
-                                                        double* verts, uint32_t* tris) {
+// Mixed cells go in rounds of 64, one per lane: a wave scan places the
+// round's vertices and triangles, the cells' meta goes to LDS, and lists of
+// the round's vertices (cell, edge) and triangles (cell, row entry, case)
+// spread the writes one item per lane.  A triangle's vertices come from the
+// owners' meta: this tile's in LDS (an owner precedes the cell in visiting
+// order, so its round is done), earlier tiles' face meta from pass 1.  A
+// round with more items than the lists hold is written cell by cell.
+// Vertex indices are relative to the tile's grid.
+#define MC_EWAVES 8
+#define MC_ETHREADS (64 * MC_EWAVES)
+#define MC_RV 192
+#define MC_RT 256
+struct MCEmitLDS {
+  uint32_t meta[512];     // (vertex prefix << 12) | owned edges, mixed cells
+  uint32_t tlist[MC_RT];  // cell | triangle of the cell << 9 | case << 12
+  uint16_t vlist[MC_RV];  // cell << 4 | edge
+};
+
+__global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_mc_emit(const GridView* views, const int4* tiles, const int* ntiles,
+                                                         const int* tleaf, const int* ttile, const int* tile_base,
+                                                         float iso, const uint32_t* meta, const int* tv, const int* tt,
+                                                         const int* vbase, const int* tbase, const int* totals,
+                                                         double* verts, uint32_t* tris) {
   __shared__ uint16_t s_emn[256];
   __shared__ __align__(16) int8_t s_tri[256 * 16];  // TRI_TABLE rows
   __shared__ uint32_t s_ownsort[27 * 12];
   __shared__ uint32_t s_before[27 * 12];
   __shared__ uint32_t s_cand[48];
   __shared__ uint32_t s_einfo[12];
-  __shared__ MCTileLDS s_w[MC_WAVES];
-  __shared__ uint32_t s_meta[MC_WAVES][512];  // (vertex prefix << 12) | owned edges, mixed cells
-  __shared__ uint16_t s_tpre[MC_WAVES][512];  // triangle prefix, by mixed-list index
-  const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-  for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
-  for (int i = tid; i < 256 * 4; i += MC_THREADS) reinterpret_cast<uint32_t*>(s_tri)[i] = c_mc_tri4[i];
-  for (int i = tid; i < 27 * 12; i += MC_THREADS) {
+  __shared__ MCTileLDS s_w[MC_EWAVES];
+  __shared__ MCEmitLDS s_e[MC_EWAVES];
+  const int tid = threadIdx.x, lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 256; i += MC_ETHREADS) s_emn[i] = c_mc_emn[i];
+  for (int i = tid; i < 256 * 4; i += MC_ETHREADS) reinterpret_cast<uint32_t*>(s_tri)[i] = c_mc_tri4[i];
+  for (int i = tid; i < 27 * 12; i += MC_ETHREADS) {
     s_ownsort[i] = c_mc_ownsort[i];
     s_before[i] = c_mc_before[i];
   }
@@ -515,74 +535,118 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_emit(const GridView* views, c
   if (tid < 12) s_einfo[tid] = c_mc_edgeinfo[tid];
   __syncthreads();
   MCTileLDS& w = s_w[wv];
-  uint32_t* sm = s_meta[wv];
-  uint16_t* stp = s_tpre[wv];
+  MCEmitLDS& el = s_e[wv];
   const int nt = *ntiles;
   const int vmax = totals[1], tmax = totals[2];
   int t_end, t_step;
-  for (int t = xcd_first_sub(nt, wv, MC_WAVES, &t_end, &t_step); t < t_end; t += t_step) {
-    if (tv[t] == 0 && tt[t] == 0) continue;  // nothing to write (uniform over the wave)
-    int tl = -1, ttv = -1;
+  int t = xcd_first_sub(nt, wv, MC_EWAVES, &t_end, &t_step);
+  // Header of the next tile, loaded while the current one is processed.
+  int h_nv = 0, h_nt = 0, h_vb = 0, h_tb = 0, h_tl = -1, h_tt = -1;
+  int4 h_to = make_int4(0, 0, 0, 0);
+  if (t < t_end) {
+    h_nv = tv[t];
+    h_nt = tt[t];
+    h_vb = vbase[t];
+    h_tb = tbase[t];
+    h_to = tiles[t];
     if (lane < VR_NBR) {
-      tl = tleaf[(size_t)t * VR_NBR + lane];
-      ttv = ttile[(size_t)t * VR_NBR + lane];
+      h_tl = tleaf[(size_t)t * VR_NBR + lane];
+      h_tt = ttile[(size_t)t * VR_NBR + lane];
     }
+  }
+  for (; t < t_end; t += t_step) {
+    const int c_nv = h_nv, c_nt = h_nt, vb = h_vb, tb = h_tb, tl = h_tl, ttv = h_tt;
+    const int4 to = h_to;
+    const int tn = t + t_step;
+    if (tn < t_end) {
+      h_nv = tv[tn];
+      h_nt = tt[tn];
+      h_vb = vbase[tn];
+      h_tb = tbase[tn];
+      h_to = tiles[tn];
+      if (lane < VR_NBR) {
+        h_tl = tleaf[(size_t)tn * VR_NBR + lane];
+        h_tt = ttile[(size_t)tn * VR_NBR + lane];
+      }
+    }
+    if (c_nv == 0 && c_nt == 0) continue;  // nothing to write (uniform over the wave)
     const uint32_t processed = (uint32_t)__ballot(lane < VR_NBR && ttv >= 0);
-    const int4 to = tiles[t];
     const GridView& g = views[to.w];
     const VR_GLOBAL NanoLeaf* leaves = g.leaves;
     const float bg = g.background;
     const double vs = g.vs;
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
-    const int vb = vbase[t], tb = tbase[t];
     const int n = mc_tile_setup(leaves, bg, iso, tl, ttv, processed, s_before, w);
-    // Sweep 1: meta, triangle prefix, owned vertices.
     int vcarry = 0, tcarry = 0;
     for (int base = 0; base < n; base += 64) {
       const int i = base + lane;
-      int c = 0, packed = 0;
-      uint32_t owned = 0u;
+      int c = 0, packed = 0, ntr = 0;
+      uint32_t owned = 0u, cube = 0u;
       if (i < n) {
         c = w.mlist[i];
-        const uint32_t emn = s_emn[mc_case_rows(w.rows, c >> 6, (c >> 3) & 7, c & 7)];
+        cube = mc_case_rows(w.rows, c >> 6, (c >> 3) & 7, c & 7);
+        const uint32_t emn = s_emn[cube];
         owned = (emn & 0xFFFu) & w.self[mc_cell_class(c >> 6, (c >> 3) & 7, c & 7)];
-        packed = __popc(owned) | (int)((emn >> 12) << 16);
+        ntr = (int)(emn >> 12);
+        packed = __popc(owned) | (ntr << 16);
       }
       const int incl = wave_incl_scan(packed);
-      if (i < n) {
-        const int excl = incl - packed;
-        const int vp = vcarry + (excl & 0xFFFF);
-        sm[c] = ((uint32_t)vp << 12) | owned;
-        stp[i] = (uint16_t)(tcarry + (excl >> 16));
-        int vi = vb + vp;
-        for (uint32_t m = owned; m && vi < vmax; m &= m - 1u, ++vi)
-          mc_write_vertex(leaves, bg, w.tleaf, c >> 6, (c >> 3) & 7, c & 7, __ffs(m) - 1, to, vs, iso, s_einfo,
-                          verts + 3 * (size_t)vi);
-      }
-      const int tot = __builtin_amdgcn_readlane(incl, 63);
-      vcarry += tot & 0xFFFF;
-      tcarry += tot >> 16;
-    }
-    wave_lds_sync();
-    // Sweep 2: triangles.
-    for (int i = lane; i < n; i += 64) {
-      const int c = w.mlist[i];
+      const int excl = incl - packed, tot = __builtin_amdgcn_readlane(incl, 63);
+      const int rv = tot & 0xFFFF, rt = tot >> 16;
+      const int vr = excl & 0xFFFF, tr = excl >> 16;
+      if (i < n) el.meta[c] = ((uint32_t)(vcarry + vr) << 12) | owned;
       const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
-      const uint32_t cube = mc_case_rows(w.rows, cx, cy, cz);
-      const int ntr = (int)(s_emn[cube] >> 12);
-      const int cls = mc_cell_class(cx, cy, cz);
-      const int8_t* row = s_tri + cube * 16;
-      int ti = tb + (int)stp[i];
-      for (int q = 0; q < ntr && ti < tmax; ++q, ++ti) {
-        uint32_t* o = tris + 3 * (size_t)ti;
+      if (rv <= MC_RV && rt <= MC_RT) {
+        if (i < n) {
+          int k = vr;
+          for (uint32_t m = owned; m; m &= m - 1u, ++k) el.vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
+          for (int q = 0; q < ntr; ++q) el.tlist[tr + q] = (uint32_t)c | ((uint32_t)q << 9) | (cube << 12);
+        }
+        wave_lds_sync();
+        for (int k = lane; k < rv; k += 64) {
+          const int vi = vb + vcarry + k;
+          const int it = el.vlist[k], cc = it >> 4;
+          if (vi < vmax)
+            mc_write_vertex(leaves, bg, w.tleaf, cc >> 6, (cc >> 3) & 7, cc & 7, it & 15, to, vs, iso, s_einfo,
+                            verts + 3 * (size_t)vi);
+        }
+        for (int k = lane; k < rt; k += 64) {
+          const int ti = tb + tcarry + k;
+          const uint32_t it = el.tlist[k];
+          const int cc = (int)(it & 511u), q = (int)((it >> 9) & 7u);
+          const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
+          const int8_t* row = s_tri + (it >> 12) * 16 + 3 * q;
+          const int cls = mc_cell_class(ux, uy, uz);
+          if (ti < tmax) {
+            uint32_t* o = tris + 3 * (size_t)ti;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          o[k] = (uint32_t)mc_edge_vertex(cx, cy, cz, row[3 * q + k], cls, processed, s_ownsort, s_cand, sm, w.ttile,
-                                          meta, vbase, vb, vrel);
+            for (int j = 0; j < 3; ++j)
+              o[j] = (uint32_t)mc_edge_vertex(ux, uy, uz, row[j], cls, processed, s_ownsort, s_cand, el.meta, w.ttile,
+                                              meta, vbase, vb, vrel);
+          }
+        }
+      } else {
+        // Dense round: cell by cell.
+        wave_lds_sync();
+        if (i < n) {
+          int vi = vb + vcarry + vr;
+          for (uint32_t m = owned; m && vi < vmax; m &= m - 1u, ++vi)
+            mc_write_vertex(leaves, bg, w.tleaf, cx, cy, cz, __ffs(m) - 1, to, vs, iso, s_einfo,
+                            verts + 3 * (size_t)vi);
+          const int cls = mc_cell_class(cx, cy, cz);
+          int ti = tb + tcarry + tr;
+          for (int q = 0; q < ntr && ti < tmax; ++q, ++ti)
+            for (int j = 0; j < 3; ++j)
+              tris[3 * (size_t)ti + j] = (uint32_t)mc_edge_vertex(cx, cy, cz, s_tri[cube * 16 + 3 * q + j], cls, processed,
+                                                                  s_ownsort, s_cand, el.meta, w.ttile, meta, vbase, vb,
+                                                                  vrel);
+        }
       }
+      wave_lds_sync();
+      vcarry += rv;
+      tcarry += rt;
     }
-    wave_lds_sync();
   }
 }
 
@@ -611,12 +675,12 @@ static int blocks_for(size_t n, int b) {
 // Blocks that loop over the tiles (tables are loaded into LDS once per
 // block, MC_WAVES tiles in flight per block): as many as are resident at
 // once, since the tiles are split statically between them.
-static int mc_blocks(const vr_ctx* ctx, const void* kernel, int tcap) {
+static int mc_blocks(const vr_ctx* ctx, const void* kernel, int waves, int tcap) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, MC_THREADS, 0) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * waves, 0) != hipSuccess || per_cu < 1)
     per_cu = 1;
   const int cap = per_cu * (ctx->cus > 0 ? ctx->cus : 256);
-  const int need = (tcap + MC_WAVES - 1) / MC_WAVES;
+  const int need = (tcap + waves - 1) / waves;
   return need < 1 ? 1 : (need < cap ? need : cap);
 }
 
@@ -663,7 +727,7 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
   if ((e = hipGetLastError()) != hipSuccess) return vr_hip_fail(ctx, e, "mc tiles");
   {
     TimedLaunch tl(ctx, TC_MC, algorithmic_bytes);
-    hipLaunchKernelGGL(k_mc_count, dim3(mc_blocks(ctx, (const void*)k_mc_count, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
+    hipLaunchKernelGGL(k_mc_count, dim3(mc_blocks(ctx, (const void*)k_mc_count, MC_WAVES, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        totals, s.tleaf.as<int>(), s.ttile.as<int>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(),
                        s.tt.as<int>());
   }
@@ -684,7 +748,7 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, fl
   hipError_t e;
   {
     TimedLaunch tl(ctx, TC_MC, algorithmic_bytes);
-    hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, (const void*)k_mc_emit, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
+    hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, (const void*)k_mc_emit, MC_EWAVES, tcap)), dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(), iso,
                        s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
                        s.totals.as<int>(), verts, tris);
