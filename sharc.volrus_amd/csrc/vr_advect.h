@@ -5,10 +5,11 @@
 #include "vr_device.h"
 
 // New value of the active voxel (ox + x, oy + y, oz + z): back-trace, then
-// trilinear sampling of the input (src/tools/advection.rs:39-53).
-// Departure-cell corners inside the LDS brick (leaf plus one-voxel halo,
-// 10^3) come from LDS; any other corner goes through the root/internal
-// lookup, so any dt is handled.
+// trilinear sampling of the input (src/tools/advection.rs:39-53).  The
+// departure cell's corners come from the LDS brick (leaf plus one-voxel halo,
+// 10^3) when they all lie in it; otherwise each goes through the
+// root/internal lookup (any dt), one corner at a time to keep that rare path
+// small in registers.
 template <int VK, int SCHEME>
 __device__ __forceinline__ float advect_voxel(const GridView& in, const VelView& vel, const float* brick, int ox,
                                               int oy, int oz, int x, int y, int z, double dt, const double tf[3]) {
@@ -26,7 +27,7 @@ __device__ __forceinline__ float advect_voxel(const GridView& in, const VelView&
 #pragma unroll
     for (int q = 0; q < 8; ++q) c[q] = brick[b0 + (q & 1) * 100 + ((q >> 1) & 1) * 10 + ((q >> 2) & 1)];
   } else {
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < 8; ++q) c[q] = grid_get(in, i + (q & 1), j + ((q >> 1) & 1), k + ((q >> 2) & 1));
   }
   return trilinear_blend(c, fx, fy, fz);
@@ -51,41 +52,46 @@ __device__ __forceinline__ void stage_interior(const GridView& in, int li, float
   }
 }
 
-// Halo: the 1000 - 512 brick positions with some local coordinate -1 or 8.
+// Halo: the 1000 - 512 brick positions with some local coordinate -1 or 8,
+// as a table built at compile time: brick index | neighbour slot (0..26) <<
+// 10 | voxel offset in that neighbour << 15.
+struct AdvectHalo {
+  uint32_t e[488];
+};
+constexpr AdvectHalo make_advect_halo() {
+  AdvectHalo t{};
+  int h = 0;
+  for (int bx = -1; bx <= 8; ++bx)
+    for (int by = -1; by <= 8; ++by)
+      for (int bz = -1; bz <= 8; ++bz) {
+        if (bx >= 0 && bx <= 7 && by >= 0 && by <= 7 && bz >= 0 && bz <= 7) continue;
+        const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
+                  dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
+        const uint32_t b = (uint32_t)((bx + 1) * 100 + (by + 1) * 10 + (bz + 1));
+        const uint32_t d = (uint32_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
+        const uint32_t off = (uint32_t)(((bx & 7) << 6) | ((by & 7) << 3) | (bz & 7));
+        t.e[h++] = b | (d << 10) | (off << 15);
+      }
+  return t;
+}
+static __constant__ AdvectHalo c_advect_halo = make_advect_halo();
+
+// Absent neighbours read the background (leaf 0 is read and discarded, so
+// the load does not wait on the branch).
 template <int NT>
 __device__ __forceinline__ void stage_halo(const GridView& in, const int* nb, float* brick, int tid) {
+  const float bg = in.background;
   for (int h = tid; h < 488; h += NT) {
-    int bx, by, bz;
-    if (h < 200) {  // x = -1 or x = 8 planes (10 x 10 each)
-      bx = h < 100 ? -1 : 8;
-      const int r = h % 100;
-      by = r / 10 - 1;
-      bz = r % 10 - 1;
-    } else if (h < 360) {  // y = -1 or y = 8 rows, x in 0..7 (8 x 10 each)
-      const int r = h - 200;
-      by = r < 80 ? -1 : 8;
-      const int u = r % 80;
-      bx = u / 10;
-      bz = u % 10 - 1;
-    } else {  // z = -1 or z = 8, x, y in 0..7 (8 x 8 each)
-      const int r = h - 360;
-      bz = r < 64 ? -1 : 8;
-      const int u = r & 63;
-      bx = u >> 3;
-      by = u & 7;
-    }
-    const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
-              dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
-    const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
-    brick[(bx + 1) * 100 + (by + 1) * 10 + (bz + 1)] =
-        src < 0 ? in.background : in.leaves[src].values[voxel_offset(bx, by, bz)];
+    const uint32_t e = c_advect_halo.e[h];
+    const int src = nb[(e >> 10) & 31u];
+    const float v = in.leaves[src < 0 ? 0 : src].values[e >> 15];
+    brick[e & 1023u] = src < 0 ? bg : v;
   }
 }
 
 // LDS of one advecting block.
 struct AdvectLDS {
   float brick[1000];
-  float res[512];
   uint16_t list[512];
   int nb[VR_NBR];
 };
@@ -93,8 +99,8 @@ struct AdvectLDS {
 // Advects leaf li of `in` into out[li] with NW waves (the whole block,
 // tid in [0, 64 NW)): the active voxels are listed by a ballot/popcount
 // prefix over the 8 mask words and spread densely over the block's lanes
-// (one pass for a band leaf with 4 waves), then the leaf is written back
-// slab by slab, coalesced, with the background in the inactive positions.
+// (one pass for a band leaf with 4 waves); each result is stored straight
+// into the output leaf, whose inactive positions get the background.
 // Output origin and mask are the input's (the topology does not change,
 // src/tools/advection.rs:34-53).
 template <int VK, int SCHEME, int NW>
@@ -104,19 +110,24 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   const int lane = tid & 63, wave = tid >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const NanoLeaf& L = in.leaves[li];
+  NanoLeaf& O = out[li];
+  const float bg = in.background;
   if (tid < VR_NBR) s.nb[tid] = in.nbr[(size_t)li * VR_NBR + tid];
-  uint64_t m[8];
-  int pre[8];
+  if (tid < 3) O.origin[tid] = L.origin[tid];
+  if (tid == 3) O.pad = 0;
   int total = 0;
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
-    m[x] = L.mask[x];
-    pre[x] = total;
-    total += __popcll(m[x]);
+    const uint64_t m = L.mask[x];
+    if ((x % NW) == wave) {
+      if (lane == 0) O.mask[x] = m;
+      if ((m >> lane) & 1ull)
+        s.list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
+      else
+        O.values[x * 64 + lane] = bg;
+    }
+    total += __popcll(m);
   }
-#pragma unroll
-  for (int x = 0; x < 8; ++x)
-    if ((x % NW) == wave && ((m[x] >> lane) & 1ull)) s.list[pre[x] + __popcll(m[x] & below)] = (uint16_t)(x * 64 + lane);
   stage_interior<NT>(in, li, s.brick, tid);
   __syncthreads();
   stage_halo<NT>(in, s.nb, s.brick, tid);
@@ -124,21 +135,7 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
   for (int i = tid; i < total; i += NT) {
     const int off = s.list[i];
-    s.res[off] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt, tf);
+    O.values[off] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt, tf);
   }
-  __syncthreads();
-  NanoLeaf& O = out[li];
-  if (tid < 3) O.origin[tid] = L.origin[tid];
-  if (tid == 3) O.pad = 0;
-  if (tid < 8) O.mask[tid] = m[tid];
-  const float bg = in.background;
-#pragma unroll
-  for (int x = 0; x < 8; ++x)
-    if ((x % NW) == wave) {
-      // Both operands are read, so the select is of values, not of an LDS
-      // and a global address (which would need a FLAT load).
-      const float r = s.res[x * 64 + lane];
-      O.values[x * 64 + lane] = ((m[x] >> lane) & 1ull) ? r : bg;
-    }
   __syncthreads();
 }

@@ -64,7 +64,7 @@ struct vr_batch {
 // specialised for it), or VK_MIXED when kinds differ (a per-leaf switch);
 // NW waves per leaf over its compacted active list.
 
-template <int VK, int SCHEME, int NW, class LDS>
+template <int VK, int SCHEME, int NW, bool ONE, class LDS>
 __device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* vels, NanoLeaf* const* outs,
                                             const int4& ck, LDS& s, int tid, const double* dts, double dt,
                                             const double tf[3]) {
@@ -74,7 +74,9 @@ __device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* 
   // host pass has no address spaces and must name the same kernel).
   NanoLeaf* const out = (VR_GLOBAL NanoLeaf*)reinterpret_cast<const uintptr_t*>(outs)[ck.x];
   const double gdt = dts ? dts[ck.x] : dt;
-  for (int li = ck.y; li < ck.y + ck.z; ++li) {
+  // ONE: chunks of one leaf, so no loop (nothing is kept live across leaves).
+  const int li_end = ONE ? ck.y + 1 : ck.y + ck.z;
+  for (int li = ck.y; li < li_end; ++li) {
     if constexpr (VK != VK_MIXED) {
       advect_leaf<VK, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf);
     } else {
@@ -91,8 +93,9 @@ __device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* 
 
 // persistent != 0: `gridDim.x` blocks loop, taking chunks from their XCD's
 // counter.  persistent == 0: one chunk per block, block b taking chunk
-// b / 8 of XCD b % 8's range (the grid covers 8 x the longest range).
-template <int VK, int SCHEME, int NW>
+// b / 8 of XCD b % 8's range (the grid covers 8 x the longest range).  ONE:
+// one chunk per block and chunks of one leaf.
+template <int VK, int SCHEME, int NW, bool ONE>
 __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, const VelView* vels,
                                                           NanoLeaf* const* outs, const int4* chunks, int* counters,
                                                           const int* ranges, int persistent, const double* dts,
@@ -103,9 +106,9 @@ __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, c
   const int xcd = blockIdx.x & 7;
   const int lo = ranges[xcd], hi = ranges[xcd + 1];
   const double tf[3] = {tf0, tf1, tf2};
-  if (!persistent) {
+  if (ONE || !persistent) {
     const int c = lo + (int)(blockIdx.x >> 3);
-    if (c < hi) batch_chunk<VK, SCHEME, NW>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf);
+    if (c < hi) batch_chunk<VK, SCHEME, NW, ONE>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf);
     return;
   }
   // Chunk indices read from LDS are block-uniform; readfirstlane tells the
@@ -118,7 +121,7 @@ __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, c
     // Grab the next chunk now; its latency overlaps this chunk's loads.
     int next = 0;
     if (tid == 0) next = atomicAdd(&counters[xcd], 1);
-    batch_chunk<VK, SCHEME, NW>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf);
+    batch_chunk<VK, SCHEME, NW, false>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf);
     if (tid == 0) s_item = next;
     __syncthreads();
     c = __builtin_amdgcn_readfirstlane(lo + s_item);
@@ -126,24 +129,39 @@ __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, c
   }
 }
 
-template <int VK, int SCHEME>
-static void launch_batch_t(int nw, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
+template <int VK, int SCHEME, int NW, bool ONE>
+static void launch_batch_k(int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
                            NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges, int persistent,
                            const double* dts, double dt, const double tf[3]) {
-  if (nw == 1)
-    hipLaunchKernelGGL((k_batch_advect<VK, SCHEME, 1>), dim3(blocks), dim3(64), 0, st, ins, vels, outs, chunks,
-                       counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2]);
-  else
-    hipLaunchKernelGGL((k_batch_advect<VK, SCHEME, 4>), dim3(blocks), dim3(256), 0, st, ins, vels, outs, chunks,
-                       counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2]);
+  hipLaunchKernelGGL((k_batch_advect<VK, SCHEME, NW, ONE>), dim3(blocks), dim3(64 * NW), 0, st, ins, vels, outs,
+                     chunks, counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2]);
+}
+
+template <int VK, int SCHEME>
+static void launch_batch_t(int nw, bool one, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
+                           NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges, int persistent,
+                           const double* dts, double dt, const double tf[3]) {
+  if (nw == 1) {
+    if (one)
+      launch_batch_k<VK, SCHEME, 1, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf);
+    else
+      launch_batch_k<VK, SCHEME, 1, false>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt,
+                                           tf);
+  } else {
+    if (one)
+      launch_batch_k<VK, SCHEME, 4, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf);
+    else
+      launch_batch_k<VK, SCHEME, 4, false>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt,
+                                           tf);
+  }
 }
 
 template <int SCHEME>
-static void launch_batch_s(int vk, int nw, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
+static void launch_batch_s(int vk, int nw, bool one, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
                            NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges,
                            int persistent, const double* dts, double dt, const double tf[3]) {
 #define VR_BK(K) \
-  launch_batch_t<K, SCHEME>(nw, blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf)
+  launch_batch_t<K, SCHEME>(nw, one, blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf)
   switch (vk) {
     case VK_UNIFORM: VR_BK(VK_UNIFORM); break;
     case VK_RIGID: VR_BK(VK_RIGID); break;
@@ -292,16 +310,17 @@ static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int schem
   int blocks = b->wpc > 0 ? b->cus * b->wpc / nw : 8 * b->max_range;
   if (blocks < 8) blocks = 8;
   const int persistent = b->wpc > 0 ? 1 : 0;
+  const bool one = !persistent && b->lpg == 1;
   double bytes = 2.0 * VR_LEAF_BYTES * (double)b->ltot;
   {
     TimedLaunch tl(ctx, TC_ADVECT, bytes);
     const double* dts = b->has_dts ? b->d_dts.as<double>() : nullptr;
     if (scheme == VR_SCHEME_RK3)
-      launch_batch_s<3>(b->vk, nw, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
+      launch_batch_s<3>(b->vk, nw, one, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
                         b->d_outs.as<NanoLeaf*>(), b->d_chunks.as<int4>(), b->d_counters.as<int>(),
                         b->d_ranges.as<int>(), persistent, dts, dt, tf);
     else
-      launch_batch_s<1>(b->vk, nw, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
+      launch_batch_s<1>(b->vk, nw, one, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
                         b->d_outs.as<NanoLeaf*>(), b->d_chunks.as<int4>(), b->d_counters.as<int>(),
                         b->d_ranges.as<int>(), persistent, dts, dt, tf);
   }

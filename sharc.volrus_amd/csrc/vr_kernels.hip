@@ -135,10 +135,11 @@ __global__ void k_scan_add(int* out, const int* sums, size_t n) {
 }
 
 // ------------------------------------------------------------------ advection
-// One leaf per block of NW waves: the active voxels are listed in LDS by a
+// One leaf per block of NW waves (no loop over leaves, so nothing per-leaf
+// is kept live across leaves): the active voxels are listed in LDS by a
 // ballot/popcount prefix over the 8 mask words and spread over the block's
-// lanes, and the leaf is then written back slab by slab, coalesced, with the
-// background in the inactive positions.  NW = 1 issues the fewest
+// lanes; results go straight to the output leaf, the background to its
+// inactive positions.  NW = 1 issues the fewest
 // instructions per leaf (dense passes of 64 active voxels); NW = 4 finishes
 // a leaf sooner, for grids with too few leaves to fill the device.
 template <int VK, int SCHEME, int NW>
@@ -148,14 +149,15 @@ __global__ __launch_bounds__(64 * NW) void k_advect(GridView in, VelView vel, Na
   const int n = *in.count;
   const double tf[3] = {tf0, tf1, tf2};
   int li_end, li_step;
-  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step)
-    advect_leaf<VK, SCHEME, NW>(in, vel, out, li, s, threadIdx.x, dt, tf);
+  // One leaf per block: the grid has at least ceil(n / 8) blocks per XCD.
+  const int li = xcd_first(n, &li_end, &li_step);
+  if (li < li_end) advect_leaf<VK, SCHEME, NW>(in, vel, out, li, s, threadIdx.x, dt, tf);
 }
 
 template <int VK, int SCHEME>
 static hipError_t launch_advect_t(const GridView& in, const VelView& vel, NanoLeaf* out, int cap, double dt,
                                   const double tf[3], int nw, hipStream_t s) {
-  const int blocks = cap < 1 ? 1 : (cap < 65536 ? cap : 65536);
+  const int blocks = 8 * ((cap + 7) / 8 + 1);  // >= ceil(count / 8) blocks on each XCD
   if (nw == 1)
     hipLaunchKernelGGL((k_advect<VK, SCHEME, 1>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1], tf[2]);
   else
