@@ -14,7 +14,7 @@ template <int VK, int SCHEME>
 __device__ __forceinline__ float advect_voxel(const GridView& in, const VelView& vel, const float* brick, int ox,
                                               int oy, int oz, int x, int y, int z, double dt, const double tf[3]) {
   double w[3], p[3];
-  index_to_world(in.mat, ox + x, oy + y, oz + z, w);
+  view_index_to_world(in, ox + x, oy + y, oz + z, w);
   departure<VK, SCHEME>(vel, w, dt, tf, p);
   const double ix = p[0] * in.inv_vs, iy = p[1] * in.inv_vs, iz = p[2] * in.inv_vs;
   const double flx = floor(ix), fly = floor(iy), flz = floor(iz);

@@ -184,8 +184,12 @@ GridView make_view(const vr_grid* g) {
   v.background = g->background;
   v.vs = g->voxel_size;
   v.inv_vs = 1.0 / g->voxel_size;
+  v.diag = 1;
   for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 4; ++c) v.mat[4 * r + c] = g->mat[r][c];
+    for (int c = 0; c < 4; ++c) {
+      v.mat[4 * r + c] = g->mat[r][c];
+      if (c < 3 && (c == r ? !(g->mat[r][c] > 0.0) : g->mat[r][c] != 0.0)) v.diag = 0;
+    }
   return v;
 }
 

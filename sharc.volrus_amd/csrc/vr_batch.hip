@@ -68,8 +68,8 @@ template <int VK, int SCHEME, int NW, bool ONE, class LDS>
 __device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* vels, NanoLeaf* const* outs,
                                             const int4& ck, LDS& s, int tid, const double* dts, double dt,
                                             const double tf[3]) {
-  const GridView& in = ins[ck.x];
-  const VelView& v = vels[ck.x];
+  const GridView in = ins[ck.x];
+  const VelView v = vels[ck.x];
   // Typed global where loaded (kernel parameter types stay unqualified: the
   // host pass has no address spaces and must name the same kernel).
   NanoLeaf* const out = (VR_GLOBAL NanoLeaf*)reinterpret_cast<const uintptr_t*>(outs)[ck.x];

@@ -51,6 +51,7 @@ struct GridView {
   const VR_GLOBAL int* nbr;
   int root_count;
   float background;
+  int diag;         // mat has zero off-diagonal terms and a positive diagonal
   double vs;        // voxel_size of the uniform transform
   double inv_vs;    // 1.0 / voxel_size (grid.rs:114-121)
   double mat[12];   // forward affine rows 0..2 (grid.rs:248-257)
@@ -133,6 +134,20 @@ __device__ __forceinline__ void index_to_world(const double m[12], int x, int y,
   const double fx = (double)x, fy = (double)y, fz = (double)z;
 #pragma unroll
   for (int r = 0; r < 3; ++r) w[r] = m[4 * r] * fx + m[4 * r + 1] * fy + m[4 * r + 2] * fz + m[4 * r + 3];
+}
+
+// The same for a view; with zero off-diagonal terms each product with them
+// is +-0, and adding +-0 leaves the partial sum m_rr * x_r unchanged (it is
+// never -0: m_rr > 0 and the integer converts to +0, never -0), so the row
+// is m_rr * x_r + m_r3, bit for bit.
+__device__ __forceinline__ void view_index_to_world(const GridView& g, int x, int y, int z, double w[3]) {
+  if (g.diag) {
+    w[0] = g.mat[0] * (double)x + g.mat[3];
+    w[1] = g.mat[5] * (double)y + g.mat[7];
+    w[2] = g.mat[10] * (double)z + g.mat[11];
+  } else {
+    index_to_world(g.mat, x, y, z, w);
+  }
 }
 
 // Stages the 10^3 brick (local coordinates -1..8) around the leaf whose 27
