@@ -118,6 +118,58 @@ __device__ __forceinline__ void dense_jt(const VelView& v, const double p[3], co
   for (int a = 0; a < 3; ++a) out[a] *= v.inv_h;
 }
 
+// The dense field at p (vel_sample<VK_DENSE>) with the cell's nodes read from
+// s_u, the leaf's window of nodes wb + (0..VW-1)^3 (0 outside the field, as
+// dense_at), when the cell lies in it: the same values, so the same bits.
+__device__ __forceinline__ void win_vel(const VelView& v, const float* s_u, const int wb[3], const double p[3],
+                                        double out[3]) {
+  const double ix = p[0] * v.inv_h, iy = p[1] * v.inv_h, iz = p[2] * v.inv_h;
+  const double flx = floor(ix), fly = floor(iy), flz = floor(iz);
+  const float fx = (float)(ix - flx), fy = (float)(iy - fly), fz = (float)(iz - flz);
+  const int ri = f64_to_i32(flx) - wb[0], rj = f64_to_i32(fly) - wb[1], rk = f64_to_i32(flz) - wb[2];
+  if ((unsigned)ri < VW - 1 && (unsigned)rj < VW - 1 && (unsigned)rk < VW - 1) {
+    const int o = (ri * VW + rj) * VW + rk;
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      float c[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) c[q] = s_u[comp * VW3 + o + ((q & 1) * VW + ((q >> 1) & 1)) * VW + ((q >> 2) & 1)];
+      out[comp] = (double)trilinear_blend(c, fx, fy, fz);
+    }
+  } else {
+    vel_sample<VK_DENSE>(v, p, out);
+  }
+}
+
+// dense_jt with the nodes from the window when the cell lies in it.
+__device__ __forceinline__ void win_jt(const VelView& v, const float* s_u, const int wb[3], const double p[3],
+                                       const double g[3], double out[3]) {
+  int b[3];
+  double f[3];
+  dense_weights(v, p, b, f);
+  const int ri = b[0] - wb[0], rj = b[1] - wb[1], rk = b[2] - wb[2];
+  if (!((unsigned)ri < VW - 1 && (unsigned)rj < VW - 1 && (unsigned)rk < VW - 1)) {
+    dense_jt(v, p, g, out);
+    return;
+  }
+  const int o = (ri * VW + rj) * VW + rk;
+  out[0] = out[1] = out[2] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+    const double wx = dx ? f[0] : 1.0 - f[0], wy = dy ? f[1] : 1.0 - f[1], wz = dz ? f[2] : 1.0 - f[2];
+    const double sx = dx ? 1.0 : -1.0, sy = dy ? 1.0 : -1.0, sz = dz ? 1.0 : -1.0;
+    const int oq = o + (dx * VW + dy) * VW + dz;
+    double gu = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gu += g[c] * (double)s_u[c * VW3 + oq];
+    out[0] += gu * sx * wy * wz;
+    out[1] += gu * wx * sy * wz;
+    out[2] += gu * wx * wy * sz;
+  }
+  for (int a = 0; a < 3; ++a) out[a] *= v.inv_h;
+}
+
 #define ADJ_WAVES 4
 #define ADJ_THREADS (64 * ADJ_WAVES)
 
@@ -191,6 +243,7 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
   __shared__ long long acc[1000];
   __shared__ float coefx[3][512];
   __shared__ float vwin[3 * VW3];
+  __shared__ float s_u[3 * VW3];  // the field's nodes over the window
   __shared__ long long vwin_i[SCHEME == 3 ? 3 * VW3 : 1];
   // records (pass 1 -> pass 2), then the separable contraction stages
   __shared__ float s_raw[REC_WORDS > T_WORDS ? REC_WORDS : T_WORDS];
@@ -215,7 +268,9 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double tf[3] = {P.tf0, P.tf1, P.tf2};
   int li_end, li_step;
-  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
+  // One leaf per block: the grid has at least ceil(n / 8) blocks per XCD.
+  const int li = xcd_first(n, &li_end, &li_step);
+  if (li < li_end) {
     const NanoLeaf& L = g.leaves[li];
     if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
     int total = 0;
@@ -250,6 +305,14 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
       const int bn = f64_to_i32(fl) - wb[a];
       s_bn[a][l] = bn;
       for (int k = 0; k < VW; ++k) s_wt[a][l][k] = (float)axis_weight(k, bn, idx - fl);
+    }
+    {
+      const int nn = P.vel.n;
+      const size_t n3 = (size_t)nn * nn * nn;
+      for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) {
+        const int c = i / VW3, r = i - c * VW3;
+        s_u[i] = dense_at(P.vel.dense + c * n3, nn, wb[0] + r / (VW * VW), wb[1] + (r / VW) % VW, wb[2] + r % VW);
+      }
     }
     stage_interior<ADJ_THREADS>(g, li, brick, tid);
     __syncthreads();
@@ -317,14 +380,14 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
       const int x = off >> 6, y = (off >> 3) & 7, z = off & 7;
       double xw[3], y1[3], y2[3], p[3], u[3];
       index_to_world(g.mat, ox + x, oy + y, oz + z, xw);
-      vel_sample<VK_DENSE>(P.vel, xw, u);
+      win_vel(P.vel, s_u, wb, xw, u);
       for (int a = 0; a < 3; ++a) y1[a] = xw[a] - P.dt * (u[a] * tf[0]);
       if (SCHEME == 1) {
         for (int a = 0; a < 3; ++a) p[a] = y1[a];
       } else {
-        vel_sample<VK_DENSE>(P.vel, y1, u);
+        win_vel(P.vel, s_u, wb, y1, u);
         for (int a = 0; a < 3; ++a) y2[a] = 0.75 * xw[a] + 0.25 * (y1[a] - P.dt * (u[a] * tf[1]));
-        vel_sample<VK_DENSE>(P.vel, y2, u);
+        win_vel(P.vel, s_u, wb, y2, u);
         for (int a = 0; a < 3; ++a) p[a] = (1.0 / 3.0) * xw[a] + (2.0 / 3.0) * (y2[a] - P.dt * (u[a] * tf[2]));
       }
       const double ix = p[0] * g.inv_vs, iy = p[1] * g.inv_vs, iz = p[2] * g.inv_vs;
@@ -368,12 +431,12 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
         for (int a = 0; a < 3; ++a) cself[a] = -P.dt * tf[0] * gp[a];
       } else {
         double jt[3], gy2[3], gy1[3];
-        dense_jt(P.vel, y2, gp, jt);
+        win_jt(P.vel, s_u, wb, y2, gp, jt);
         for (int a = 0; a < 3; ++a) gy2[a] = (2.0 / 3.0) * gp[a] - (2.0 / 3.0) * P.dt * tf[2] * jt[a];
         const double c3[3] = {-(2.0 / 3.0) * P.dt * tf[2] * gp[0], -(2.0 / 3.0) * P.dt * tf[2] * gp[1],
                               -(2.0 / 3.0) * P.dt * tf[2] * gp[2]};
         rec_put(0, off, y2, c3);
-        dense_jt(P.vel, y1, gy2, jt);
+        win_jt(P.vel, s_u, wb, y1, gy2, jt);
         for (int a = 0; a < 3; ++a) gy1[a] = 0.25 * gy2[a] - 0.25 * P.dt * tf[1] * jt[a];
         const double c2[3] = {-0.25 * P.dt * tf[1] * gy2[0], -0.25 * P.dt * tf[1] * gy2[1],
                               -0.25 * P.dt * tf[1] * gy2[2]};
@@ -530,7 +593,9 @@ __global__ __launch_bounds__(256) void k_adj_terminal(GridView g, GridView targe
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = *g.count;
   int li_end, li_step;
-  for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
+  // One leaf per block: the grid has at least ceil(n / 8) blocks per XCD.
+  const int li = xcd_first(n, &li_end, &li_step);
+  if (li < li_end) {
     const NanoLeaf& L = g.leaves[li];
     const int tl = find_leaf(target, L.origin[0], L.origin[1], L.origin[2]);
     double part = 0.0;
@@ -576,6 +641,10 @@ static int grid_blocks(size_t n, int b) {
   size_t g = (n + b - 1) / b;
   return (int)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
 }
+
+// Blocks of the one-leaf-per-block step kernels: at least ceil(L / 8) on
+// each XCD (the leaves are cut into 8 ranges).
+static int leaf_blocks(size_t L) { return (int)(8 * ((L + 7) / 8 + 1)); }
 
 extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_velocity* vel, double dt, int scheme,
                                   int steps, int checkpoint_every, const vr_grid* target, vr_adjoint_stats* stats,
@@ -662,7 +731,7 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   // Terminal adjoint and loss.
   if (rc == VR_OK) {
     view.d_leaves = fin;
-    hipLaunchKernelGGL(k_adj_terminal, dim3(grid_blocks(L, 1)), dim3(256), 0, ctx->stream, make_view(&view),
+    hipLaunchKernelGGL(k_adj_terminal, dim3(leaf_blocks(L)), dim3(256), 0, ctx->stream, make_view(&view),
                        make_view(target), ob[0], leaf_loss);
     hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, ctx->stream, leaf_loss, phi0->topo->d_count, loss_d);
     if ((e = hipGetLastError()) != hipSuccess) rc = vr_hip_fail(ctx, e, "terminal");
@@ -691,9 +760,9 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
       P.tf0 = P.tf1 = P.tf2 = 1.0;
       P.separable = separable;
       if (scheme == VR_SCHEME_RK3)
-        hipLaunchKernelGGL(k_adj_step3, dim3(grid_blocks(L, 1)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+        hipLaunchKernelGGL(k_adj_step3, dim3(leaf_blocks(L)), dim3(ADJ_THREADS), 0, ctx->stream, P);
       else
-        hipLaunchKernelGGL(k_adj_step1, dim3(grid_blocks(L, 1)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+        hipLaunchKernelGGL(k_adj_step1, dim3(leaf_blocks(L)), dim3(ADJ_THREADS), 0, ctx->stream, P);
       if ((e = hipGetLastError()) != hipSuccess) rc = vr_hip_fail(ctx, e, "adjoint step");
       bi ^= 1;
     }

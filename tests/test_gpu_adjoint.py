@@ -47,6 +47,33 @@ def test_adjoint_matches_cpu_adjoint(orc, ctx, vr, scheme, ckpt):
     assert st.forward_ms > 0 and st.adjoint_ms > 0
 
 
+@pytest.mark.parametrize("scheme", [1, 3])
+def test_adjoint_leaf_count_not_multiple_of_8(orc, ctx, vr, scheme):
+    """137 leaves: the one-leaf-per-block kernels (forward, terminal, steps)
+    see XCD ranges of unequal length; two calls, the second on reused memory."""
+    announce(f"adjoint 137 leaves scheme={scheme}")
+    n, steps = 16, 3
+    u, _, tg, dt = setup(orc, ctx, vr, n)
+    g0 = orc.sphere(0.27, (0.5, 0.5, 0.5), 1.0 / 64, 3.0)
+    nano = g0.to_nano()
+    assert ((len(nano) - 320) // 2128) % 8 != 0
+    g = ctx.upload(nano)
+    target = ctx.upload(tg.to_nano())
+    vel = ctx.velocity_dense(u, n, 1.0 / n)
+    fin = orc.rollout_dense(g0, u, n, 1.0 / n, dt, scheme, steps)
+    from util import active_voxels
+    c, v = active_voxels(fin.to_nano())
+    t = np.array([tg.get(*xyz) for xyz in c.tolist()], dtype=np.float64)
+    loss_f32 = 0.5 * float(((v.astype(np.float64) - t) ** 2).sum())
+    loss64, g64 = orc.rollout_adjoint_f64(g0, u.astype(np.float64), n, 1.0 / n, dt, scheme, steps, tg)
+    g64 = g64.reshape(3, n, n, n)
+    for _ in range(2):
+        st, grad = g.adjoint_rollout(vel, n, dt, scheme, steps, 1, target)
+        assert abs(st.loss - loss_f32) <= 1e-12 * loss_f32
+        rel = np.linalg.norm(grad - g64) / np.linalg.norm(g64)
+        assert rel < 1e-3, rel
+
+
 def test_adjoint_matches_central_differences_of_gpu_forward(orc, ctx, vr):
     announce("adjoint vs FD of the GPU forward")
     n, steps, scheme = 16, 6, 1

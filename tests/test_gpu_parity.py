@@ -112,12 +112,14 @@ def test_advect_dense_velocity_large_dt(orc, ctx):
     rng = np.random.default_rng(5)
     n = 16
     u = rng.normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32)
-    sdf = orc.sphere(0.3, (0.5, 0.5, 0.5), 1.0 / 64, 3.0)
-    for dt, scheme in [(0.5 / 64 / 1.5, 1), (6.0 / 64, 1), (3.0 / 64, 3)]:
-        want = orc.advect_dense(sdf, u, n, 1.0 / n, dt, scheme).to_nano()
-        vel = ctx.velocity_dense(u, n, 1.0 / n)
-        got = ctx.upload(sdf.to_nano()).advect(vel, dt, 0.0, scheme).download()
-        assert_nano_equal(got, want, f"dense dt={dt} scheme={scheme}")
+    # r = 0.27: 137 leaves, not a multiple of 8 (XCD ranges of unequal length)
+    for r in (0.3, 0.27):
+        sdf = orc.sphere(r, (0.5, 0.5, 0.5), 1.0 / 64, 3.0)
+        for dt, scheme in [(0.5 / 64 / 1.5, 1), (6.0 / 64, 1), (3.0 / 64, 3)]:
+            want = orc.advect_dense(sdf, u, n, 1.0 / n, dt, scheme).to_nano()
+            vel = ctx.velocity_dense(u, n, 1.0 / n)
+            got = ctx.upload(sdf.to_nano()).advect(vel, dt, 0.0, scheme).download()
+            assert_nano_equal(got, want, f"dense r={r} dt={dt} scheme={scheme}")
 
 
 def test_advect_input_with_empty_leaf(orc, ctx):
