@@ -209,38 +209,29 @@ __device__ __forceinline__ float merged_lambda(const float* ob, const float* far
 // bricks (so no separate merge launch), then:
 //
 // lambda scatter: each contribution (float)(lambda w) is quantised to an
-// int32 with the leaf's scale 2^(30 - E), max |lambda_{n+1}| over the leaf
-// < 2^E, and summed in 64 bits (a brick cell gets at most 512 of them).
-// Integer sums are exact, so lambda_n does not depend on the order of the
-// adds, and integer LDS atomics run far faster than float ones.
+// integer with the leaf's scale 2^(21 - E), max |lambda_{n+1}| over the
+// leaf < 2^E, and summed in 32 bits (a brick cell gets at most one per
+// voxel, 512, so the sum stays below 2^30).  Integer sums are exact, so
+// lambda_n does not depend on the order of the adds, and integer LDS
+// atomics run far faster than float ones.
 //
 // Velocity scatter at the voxel's own position (the RK1 term and the last
 // RK3 term): for a diagonal transform the node weights factor per axis, so
 // the per-voxel coefficients are staged in LDS and contracted axis by axis
-// into the leaf's node window (no atomics).  RK3 stage positions (and the
-// voxel's own position when that form does not apply) are kept as records
-// and added in a second pass as int32-quantised 64-bit sums scaled by the
-// leaf's max |coefficient|; RK1 without the separable form uses float
-// atomics into the window.
-
-// Record of one velocity scatter point of a voxel (RK3 stage positions, and
-// the voxel's own position when the separable form does not apply): base
-// node relative to the leaf's window packed as 3 x 10-bit signed, the three
-// fractions and the three coefficients.  REC_NONE marks no record.
-#define REC_NONE (-2147483647 - 1)
-__device__ __forceinline__ int rec_unpack(int v, int a) {
-  const int s = (v >> (20 - 10 * a)) & 1023;
-  return s >= 512 ? s - 1024 : s;
-}
+// into the leaf's node window (no atomics); otherwise float atomics into the
+// window.  The two RK3 stage positions are kept as records and added in a
+// second pass as int32-quantised 64-bit sums scaled by the leaf's max
+// |coefficient|.
 
 template <int SCHEME>
 __device__ __forceinline__ void adj_step(const AdjStep& P) {
-  // RK3 keeps 3 scatter records per voxel (y2, y1, and x when not separable).
-  constexpr int NREC = SCHEME == 3 ? 3 : 0;
-  constexpr int REC_WORDS = NREC * 7 * 512;
+  // RK3 keeps 2 scatter records per voxel (y2, y1); the scatter at x goes
+  // through the separable contraction, or straight into the window.
+  constexpr int NREC = SCHEME == 3 ? 2 : 0;
+  constexpr int REC_WORDS = NREC * 6 * 512;
   constexpr int T_WORDS = 3 * 64 * VW + 3 * 8 * VW * VW;
   __shared__ float brick[1000];
-  __shared__ long long acc[1000];
+  __shared__ int acc[1000];
   __shared__ float coefx[3][512];
   __shared__ float vwin[3 * VW3];
   __shared__ float s_u[3 * VW3];  // the field's nodes over the window
@@ -249,9 +240,11 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
   __shared__ float s_raw[REC_WORDS > T_WORDS ? REC_WORDS : T_WORDS];
   float* const t1 = s_raw;
   float* const t2 = s_raw + 3 * 64 * VW;
-  float* const rec_c = s_raw;                              // [NREC][3][512]
-  float* const rec_f = s_raw + NREC * 3 * 512;             // [NREC][3][512]
-  int* const rec_b = (int*)(s_raw + NREC * 6 * 512);       // [NREC][512]
+  // record r of a voxel: per axis the coefficient, and the cell's base node
+  // relative to the window (10 bits) with the fraction in 22-bit fixed point
+  // above it; all-zero coefficients mark no record.
+  float* const rec_c = s_raw;                                  // [NREC][3][512]
+  uint32_t* const rec_w = (uint32_t*)(s_raw + NREC * 3 * 512);  // [NREC][3][512]
   __shared__ float s_cmax[ADJ_WAVES];
   __shared__ uint16_t list[512];
   __shared__ int nb[VR_NBR];
@@ -285,7 +278,7 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
     for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin[i] = 0.0f;
     if constexpr (SCHEME == 3) {
       for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin_i[i] = 0;
-      for (int i = tid; i < NREC * 512; i += ADJ_THREADS) rec_b[i] = REC_NONE;
+      for (int i = tid; i < NREC * 3 * 512; i += ADJ_THREADS) rec_c[i] = 0.0f;
     }
     const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
     int wb[3];
@@ -341,10 +334,11 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
     for (int w = 1; w < ADJ_WAVES; ++w) mx = fmaxf(mx, s_max[w]);
     int ex = 0;
     (void)frexpf(mx, &ex);  // mx < 2^ex
-    // contributions |v| < 2^ex quantise to int32 (|q| < 2^30, f32 -> i32 is
-    // one instruction); at most 512 per cell sum below 2^39 in int64
-    const float scale = ldexpf(1.0f, 30 - ex);
-    const double inv_scale = ldexp(1.0, ex - 30);
+    // contributions |v| < 2^ex quantise to |q| < 2^21; a cell gets at most
+    // one per voxel (a voxel's 8 corners are distinct), so at most 512 sum
+    // below 2^30 in int32
+    const float scale = ldexpf(1.0f, 21 - ex);
+    const double inv_scale = ldexp(1.0, ex - 21);
     const bool sep = s_sep != 0;
     float cmax = 0.0f;  // max |coefficient| over this thread's records
     // Stores a scatter record (RK3), or adds far points straight to dL/dU.
@@ -365,9 +359,11 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
         }
         return;
       }
-      rec_b[r * 512 + off] = ((r0 & 1023) << 20) | ((r1 & 1023) << 10) | (r2 & 1023);
+      const int rr[3] = {r0, r1, r2};
       for (int a = 0; a < 3; ++a) {
-        rec_f[(r * 3 + a) * 512 + off] = (float)f[a];
+        // f in [0, 1): 22-bit fixed point, truncated
+        const uint32_t fq = (uint32_t)(f[a] * 4194304.0);
+        rec_w[(r * 3 + a) * 512 + off] = (fq << 10) | (uint32_t)(rr[a] & 1023);
         const float cf = (float)c[a];
         rec_c[(r * 3 + a) * 512 + off] = cf;
         cmax = fmaxf(cmax, fabsf(cf));
@@ -407,7 +403,7 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
           const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
           const double w = (dx ? fx : 1.0 - fx) * (dy ? fy : 1.0 - fy) * (dz ? fz : 1.0 - fz);
           const float v = (float)((double)lam * w);
-          lds_add_i64(&acc[b0 + dx * 100 + dy * 10 + dz], (long long)(int)rintf(v * scale));
+          atomicAdd(&acc[b0 + dx * 100 + dy * 10 + dz], (int)rintf(v * scale));
         }
       } else {
         for (int q = 0; q < 8; ++q) {
@@ -445,8 +441,6 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
       }
       if (sep) {
         for (int a = 0; a < 3; ++a) coefx[a][off] = (float)cself[a];
-      } else if constexpr (SCHEME == 3) {
-        rec_put(2, off, xw, cself);
       } else {
         vel_scatter(P, vwin, wb, xw, cself);
       }
@@ -473,14 +467,15 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
       for (int ii = tid; ii < total; ii += ADJ_THREADS) {
         const int off = list[ii];
         for (int r = 0; r < NREC; ++r) {
-          const int rb = rec_b[r * 512 + off];
-          if (rb == REC_NONE) continue;
-          const int rel[3] = {rec_unpack(rb, 0), rec_unpack(rb, 1), rec_unpack(rb, 2)};
           float f[3], cs[3];
+          int rel[3];
           for (int a = 0; a < 3; ++a) {
-            f[a] = rec_f[(r * 3 + a) * 512 + off];
+            const uint32_t wv = rec_w[(r * 3 + a) * 512 + off];
+            rel[a] = ((int)(wv << 22)) >> 22;
+            f[a] = (float)(wv >> 10) * 2.384185791015625e-07f;  // 2^-22
             cs[a] = rec_c[(r * 3 + a) * 512 + off] * scale_v;
           }
+          if (cs[0] == 0.0f && cs[1] == 0.0f && cs[2] == 0.0f) continue;
           const bool inwin = (unsigned)rel[0] < VW - 1 && (unsigned)rel[1] < VW - 1 && (unsigned)rel[2] < VW - 1 &&
                              (unsigned)(wb[0] + rel[0]) < (unsigned)(nn - 1) &&
                              (unsigned)(wb[1] + rel[1]) < (unsigned)(nn - 1) &&
