@@ -184,9 +184,13 @@ hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* ou
 
 // ------------------------------------------------------------------ zero crossings
 // src/tools/level_set_track.rs:14-47: active voxel with value == 0, or with
-// an active 6-neighbour whose f32 product with it is < 0.
+// an active 6-neighbour whose f32 product with it is < 0.  The leaf and the
+// six face layers of its neighbours are staged in LDS with inactive or
+// absent voxels as +0: a product with +0 is never < 0, which is the same
+// test as skipping them.
 __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, int* any) {
   __shared__ int nb[VR_NBR];
+  __shared__ float brick[1000];  // (x + 1) * 100 + (y + 1) * 10 + (z + 1), x, y, z in -1..8
   const int lane = threadIdx.x;
   const int n = *g.count;
   const int y = lane >> 3, z = lane & 7;
@@ -194,33 +198,44 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
   for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
     if (lane < VR_NBR) nb[lane] = g.nbr[(size_t)li * VR_NBR + lane];
+    const float4* own = reinterpret_cast<const float4*>(L.values);
+    for (int q = lane; q < 128; q += 64) {
+      const float4 v = own[q];
+      const int x = q >> 4, yy = (q >> 1) & 7, zz = (q & 1) * 4;
+      const uint32_t m = (uint32_t)(L.mask[x] >> (yy * 8 + zz)) & 0xFu;
+      float* b = brick + (x + 1) * 100 + (yy + 1) * 10 + (zz + 1);
+      b[0] = (m & 1u) ? v.x : 0.0f;
+      b[1] = (m & 2u) ? v.y : 0.0f;
+      b[2] = (m & 4u) ? v.z : 0.0f;
+      b[3] = (m & 8u) ? v.w : 0.0f;
+    }
+    __syncthreads();
+    // face layers: f = 0..5 for x = -1, x = 8, y = -1, y = 8, z = -1, z = 8
+    for (int h = lane; h < 384; h += 64) {
+      const int f = h >> 6, u = (h >> 3) & 7, w = h & 7;
+      const int side = (f & 1) ? 8 : -1;
+      const int bx = f < 2 ? side : u, by = f < 2 ? u : (f < 4 ? side : w), bz = f < 4 ? w : side;
+      const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
+                dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
+      const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+      const int off = voxel_offset(bx, by, bz);
+      float v = 0.0f;
+      if (src >= 0) {
+        const NanoLeaf& S = g.leaves[src];
+        const float sv = S.values[off];
+        v = ((S.mask[off >> 6] >> (off & 63)) & 1ull) ? sv : 0.0f;
+      }
+      brick[(bx + 1) * 100 + (by + 1) * 10 + (bz + 1)] = v;
+    }
     __syncthreads();
     bool found = false;
     for (int x = 0; x < 8; ++x) {
       const bool act = (L.mask[x] >> lane) & 1ull;
-      bool cr = false;
-      if (act) {
-        const float v = L.values[x * 64 + lane];
-        if (v == 0.0f) {
-          cr = true;
-        } else {
-          const int dd[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-#pragma unroll
-          for (int q = 0; q < 6; ++q) {
-            const int nx = x + dd[q][0], ny = y + dd[q][1], nz = z + dd[q][2];
-            const int ex = nx < 0 ? -1 : (nx > 7 ? 1 : 0), ey = ny < 0 ? -1 : (ny > 7 ? 1 : 0),
-                      ez = nz < 0 ? -1 : (nz > 7 ? 1 : 0);
-            const int src = (ex | ey | ez) ? nb[(ex + 1) * 9 + (ey + 1) * 3 + (ez + 1)] : li;
-            if (src < 0) continue;
-            const int off = voxel_offset(nx, ny, nz);
-            if (!((g.leaves[src].mask[off >> 6] >> (off & 63)) & 1ull)) continue;
-            if (v * g.leaves[src].values[off] < 0.0f) {
-              cr = true;
-              break;
-            }
-          }
-        }
-      }
+      const int c = (x + 1) * 100 + (y + 1) * 10 + (z + 1);
+      const float v = brick[c];
+      const bool cr = act && (v == 0.0f || v * brick[c + 100] < 0.0f || v * brick[c - 100] < 0.0f ||
+                              v * brick[c + 10] < 0.0f || v * brick[c - 10] < 0.0f || v * brick[c + 1] < 0.0f ||
+                              v * brick[c - 1] < 0.0f);
       const uint64_t word = __ballot(cr);
       if (lane == 0) cross[(size_t)li * 8 + x] = word;
       found |= word != 0;
