@@ -250,16 +250,22 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
 // masks of the 27 neighbouring leaves are staged as 24 x 24 rows of 24 z-bits;
 // each active voxel takes the smallest squared lattice distance to a set bit
 // within |d| <= R = ceil(half_width) <= 8, which gives the brute-force result
-// exactly (kept set and values).  No crossings anywhere: the grid is copied
-// unchanged (the reference returns early, :85-87).
+// exactly (kept set and values).  The window is a box and d^2 a sum of
+// squares, so the minimum separates exactly: per row the nearest bit along z
+// (bit scans), then the minimum over dy, then over dx, in small integers.
+// No crossings anywhere: the grid is copied unchanged (the reference returns
+// early, :85-87).
 __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cross, const int* any, NanoLeaf* out,
                                                 int* keep, int R, float vs, float band) {
   __shared__ uint32_t rows[24 * 24];
+  __shared__ uint8_t dz2[24 * 24 * 8];  // row (X, Y), z: dz^2 to the nearest bit, 255 = none within R
+  __shared__ uint8_t dyz2[24 * 8 * 8];  // X, y, z: min over dy of dy^2 + dz2, 255 = none
   __shared__ int nb[VR_NBR];
   const int lane = threadIdx.x;
   const int n = *g.count;
   const bool have = *any != 0;
   const int y = lane >> 3, z = lane & 7;
+  const int lo = 8 - R, span = 8 + 2 * R;  // rows X, Y in [lo, lo + span)
   int li_end, li_step;
   for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
@@ -286,8 +292,36 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
       rows[r] = bits;
     }
     __syncthreads();
-    bool any_left = false;
     const uint32_t win = (2u << (2 * R)) - 1u;  // 2R+1 bits
+    for (int i = lane; i < span * span * 8; i += 64) {
+      const int zz = i & 7, rc = i >> 3, X = lo + rc / span, Y = lo + rc % span;
+      const uint32_t b = (rows[X * 24 + Y] >> (8 + zz - R)) & win;
+      int d = 255;
+      if (b) {
+        const uint32_t below = b & ((2u << R) - 1u);  // positions 0..R (at or below z)
+        const uint32_t above = b >> R;                // positions R..2R (at or above z)
+        int dz = 64;
+        if (below) dz = R - (31 - __clz(below));
+        if (above) {
+          const int h = __ffs(above) - 1;
+          dz = h < dz ? h : dz;
+        }
+        d = dz * dz;
+      }
+      dz2[(X * 24 + Y) * 8 + zz] = (uint8_t)d;
+    }
+    __syncthreads();
+    for (int i = lane; i < span * 64; i += 64) {
+      const int X = lo + (i >> 6), yy = (i >> 3) & 7, zz = i & 7;
+      int m = 255;
+      for (int dy = -R; dy <= R; ++dy) {
+        const int v = dz2[(X * 24 + 8 + yy + dy) * 8 + zz];
+        if (v != 255) m = min(m, v + dy * dy);
+      }
+      dyz2[(X * 8 + yy) * 8 + zz] = (uint8_t)m;
+    }
+    __syncthreads();
+    bool any_left = false;
     for (int x = 0; x < 8; ++x) {
       const bool act = (L.mask[x] >> lane) & 1ull;
       const float old = L.values[x * 64 + lane];
@@ -295,26 +329,9 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
       float nv = old;
       if (act) {
         int best = 0x7fffffff;
-        const int X = 8 + x, Y = 8 + y, Z = 8 + z;
         for (int dx = -R; dx <= R; ++dx) {
-          const int dx2 = dx * dx;
-          if (dx2 >= best) continue;
-          for (int dy = -R; dy <= R; ++dy) {
-            const int dxy = dx2 + dy * dy;
-            if (dxy >= best) continue;
-            const uint32_t b = (rows[(X + dx) * 24 + (Y + dy)] >> (Z - R)) & win;
-            if (!b) continue;
-            const uint32_t lo = b & ((2u << R) - 1u);  // positions 0..R (at or below z)
-            const uint32_t hi = b >> R;                // positions R..2R (at or above z)
-            int dz = 64;
-            if (lo) dz = R - (31 - __clz(lo));
-            if (hi) {
-              const int h = __ffs(hi) - 1;
-              dz = h < dz ? h : dz;
-            }
-            const int d2 = dxy + dz * dz;
-            best = d2 < best ? d2 : best;
-          }
+          const int v = dyz2[((8 + x + dx) * 8 + y) * 8 + z];
+          if (v != 255) best = min(best, v + dx * dx);
         }
         if (best != 0x7fffffff) {
           const float sign = old >= 0.0f ? 1.0f : -1.0f;
