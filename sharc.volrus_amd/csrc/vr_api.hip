@@ -233,7 +233,7 @@ void copy_header_into(vr_grid* dst, const vr_grid* src) {
 // in order, as g's new leaf set, and rebuilds g's lookup arrays.  Mirrors
 // remove_empty_leaves (src/tree/internal.rs:231-247) followed by the sorted
 // leaf order of NanoGrid::from_grid.
-int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, bool has_empty_after) {
+int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, bool has_empty_after, bool exact) {
   hipError_t e;
   const int n = g->topo->count;
   DevBuf& pos = ctx->scratch[SCR_POS];
@@ -247,16 +247,21 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
     // enough (the count never grows), so reuse them.
     Topo* t = g->topo.get();
     if ((e = tot.ensure(sizeof(int) * 4)) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
-    if ((e = vr_exclusive_scan(keep, pos.as<int>(), (size_t)n, sums.as<int>(), tot.as<int>(), ctx->stream)) !=
+    // With the host count an upper bound, the flags past the device count
+    // are cleared so the scan sees only live leaves.
+    if ((!exact && (e = vr_zero_tail(const_cast<int*>(keep), t->d_count, n, ctx->stream)) != hipSuccess) ||
+        (e = vr_exclusive_scan(keep, pos.as<int>(), (size_t)n, sums.as<int>(), tot.as<int>(), ctx->stream)) !=
             hipSuccess ||
         (e = vr_launch_compact(src, t->d_count, n, keep, pos.as<int>(), g->d_leaves, ctx->stream)) != hipSuccess ||
         (e = vr_copy_int(tot.as<int>(), t->d_count, ctx->stream)) != hipSuccess)
       return vr_hip_fail(ctx, e, "compact");
-    int cnt = 0;
-    if ((e = hipMemcpyAsync(&cnt, tot.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-      return vr_hip_fail(ctx, e, "count readback");
-    t->count = cnt;
+    if (exact) {
+      int cnt = 0;
+      if ((e = hipMemcpyAsync(&cnt, tot.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+          (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return vr_hip_fail(ctx, e, "count readback");
+      t->count = cnt;
+    }
     t->has_empty = has_empty_after;
     const size_t R = t->roots.size();
     if ((e = vr_fill_int(t->d_child, -1, 4096 * (R ? R : 1), ctx->stream)) != hipSuccess ||
@@ -557,13 +562,13 @@ int advect_into(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel, double d
   vr_stage_factors(vel->period, t, dt, scheme, tf);
   // Algorithmic bytes: read and write every leaf record once, plus the
   // velocity data (three leaf sets, or the dense array; 0 when analytic).
-  double bytes = 2.0 * VR_LEAF_BYTES * in->topo->count;
+  double vel_bytes = 0.0;
   if (vel->kind == VK_GRIDS3)
-    for (int c = 0; c < 3; ++c) bytes += (double)VR_LEAF_BYTES * vel->g[c]->topo->count;
-  if (vel->kind == VK_DENSE) bytes += 12.0 * vel->n * vel->n * vel->n;
+    for (int c = 0; c < 3; ++c) vel_bytes += (double)VR_LEAF_BYTES * vel->g[c]->topo->count;
+  if (vel->kind == VK_DENSE) vel_bytes += 12.0 * vel->n * vel->n * vel->n;
   hipError_t e;
   {
-    TimedLaunch tl(ctx, TC_ADVECT, bytes);
+    TimedLaunch tl(ctx, TC_ADVECT, 2.0 * VR_LEAF_BYTES, vel_bytes, in->topo->count);
     e = vr_launch_advect(make_view(in), make_vel_view(vel), o->d_leaves, in->topo->cap, dt, tf, scheme,
                          vr_advect_waves(in->topo->count, ctx->cus), ctx->stream);
   }
@@ -614,7 +619,7 @@ int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any) {
     return vr_hip_fail(ctx, e, "scratch");
   if ((e = vr_fill_int(ab.as<int>(), 0, 1, ctx->stream)) != hipSuccess) return vr_hip_fail(ctx, e, "crossings");
   {
-    TimedLaunch tl(ctx, TC_CROSSINGS, (double)(VR_LEAF_BYTES + 64) * n);
+    TimedLaunch tl(ctx, TC_CROSSINGS, (double)(VR_LEAF_BYTES + 64), 0.0, n);
     e = vr_launch_crossings(make_view(g), g->topo->cap, cb.as<uint64_t>(), ab.as<int>(), ctx->stream);
   }
   if (e != hipSuccess) return vr_hip_fail(ctx, e, "crossings");
@@ -687,7 +692,7 @@ extern "C" int vr_trim_band(vr_ctx* ctx, vr_grid* g, float half_width) {
 
 // Crossings, bounded-radius redistance into `tmp` (>= count leaves), then
 // compaction back into g.
-int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp) {
+int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact) {
   const double rr = std::ceil((double)half_width);
   if (!(rr <= 8.0)) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "rebuild supports ceil(half_width) <= 8");
   const int R = rr < 0.0 ? 0 : (int)rr;
@@ -702,7 +707,7 @@ int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp) {
   DevBuf& keep = ctx->scratch[SCR_KEEP];
   if ((e = keep.ensure(sizeof(int) * (size_t)(n + 1))) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
   {
-    TimedLaunch tl(ctx, TC_REBUILD, (double)(2 * VR_LEAF_BYTES + 64) * n);
+    TimedLaunch tl(ctx, TC_REBUILD, (double)(2 * VR_LEAF_BYTES + 64), 0.0, n);
     e = vr_launch_rebuild(make_view(g), g->topo->cap, cross, any, tmp, keep.as<int>(), R, vs, band, ctx->stream);
   }
   if (e != hipSuccess) return vr_hip_fail(ctx, e, "rebuild");
@@ -714,7 +719,7 @@ int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp) {
       ((e = hipMemcpyAsync(&anyh, any, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess))
     return vr_hip_fail(ctx, e, "readback");
-  return compact_into(ctx, g, tmp, keep.as<int>(), anyh ? false : g->topo->has_empty);
+  return compact_into(ctx, g, tmp, keep.as<int>(), anyh ? false : g->topo->has_empty, exact);
 }
 
 extern "C" int vr_rebuild_band(vr_ctx* ctx, vr_grid* g, float half_width) {

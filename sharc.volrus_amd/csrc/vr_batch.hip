@@ -313,7 +313,7 @@ static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int schem
   const bool one = !persistent && b->lpg == 1;
   double bytes = 2.0 * VR_LEAF_BYTES * (double)b->ltot;
   {
-    TimedLaunch tl(ctx, TC_ADVECT, bytes);
+    TimedLaunch tl(ctx, TC_ADVECT, 0.0, bytes, 0.0);
     const double* dts = b->has_dts ? b->d_dts.as<double>() : nullptr;
     if (scheme == VR_SCHEME_RK3)
       launch_batch_s<3>(b->vk, nw, one, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
@@ -342,7 +342,7 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
     b->tcap = nt;
   }
   rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->d_leaf_base.as<int>(), b->G, b->ltot, b->tcap,
-                      iso, (double)VR_LEAF_BYTES * b->ltot);
+                      iso);
   if (rc != VR_OK) return rc;
   if (sizing) {
     int th[3];
@@ -364,7 +364,8 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
       b->tricap = th[2] + 1;
     }
   }
-  return mc_emit_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->tcap, iso, b->d_verts, b->d_tris, 0.0);
+  return mc_emit_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->tcap, iso, b->d_verts, b->d_tris, 0.0,
+                       (long long)b->vcap, (long long)b->tricap);
 }
 
 static int batch_read_ranges(vr_ctx* ctx, vr_batch* b) {
@@ -404,9 +405,15 @@ extern "C" int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int 
     }
     return (e = hipStreamSynchronize(ctx->stream)) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "sync");
   }
+  // Mesh capacities come from a host-synchronised run with the same
+  // parameters (the tile count depends on the topology only, but vertex and
+  // triangle counts depend on dt, t and iso).
+  if (b->sized && b->exec &&
+      (b->g_dt != dt || b->g_t != t || b->g_scheme != scheme || b->g_iso != iso))
+    b->sized = false;
   if (!b->sized || mode == VR_BATCH_EAGER) {
     int rc = batch_advect(ctx, b, dt, t, scheme);
-    if (rc == VR_OK) rc = batch_mc(ctx, b, iso, !b->sized);
+    if (rc == VR_OK) rc = batch_mc(ctx, b, iso, true);
     if (rc == VR_OK) rc = batch_read_ranges(ctx, b);
     if (rc != VR_OK) return rc;
     b->sized = true;
@@ -446,7 +453,7 @@ extern "C" int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int 
     b->g_wpc = b->wpc;
   }
   {
-    TimedLaunch tl(ctx, TC_GRAPH, 2.0 * VR_LEAF_BYTES * (double)b->ltot);
+    TimedLaunch tl(ctx, TC_GRAPH, 2.0 * VR_LEAF_BYTES, 0.0, (double)b->ltot);
     if ((e = hipGraphLaunch(b->exec, ctx->stream)) != hipSuccess) return vr_hip_fail(ctx, e, "hipGraphLaunch");
   }
   return VR_OK;

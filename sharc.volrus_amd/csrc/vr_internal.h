@@ -46,7 +46,7 @@ struct Topo {
 enum ScratchSlot {
   SCR_POS = 0, SCR_SUMS, SCR_SUMS2, SCR_TOTAL, SCR_KEEP, SCR_CROSS, SCR_ANY, SCR_WORDS, SCR_COORDS,
   SCR_MC_KEEP, SCR_MC_POS, SCR_MC_TILES, SCR_MC_TLEAF, SCR_MC_TTILE, SCR_MC_META, SCR_MC_TV, SCR_MC_TT,
-  SCR_MC_VB, SCR_MC_TB, SCR_NEWCOUNT, SCR_TMP_LEAVES, SCR_ACTIVE, SCR_PING,
+  SCR_MC_VB, SCR_MC_TB, SCR_NEWCOUNT, SCR_TMP_LEAVES, SCR_ACTIVE, SCR_PING, SCR_ROLL_LEAVES, SCR_ROLL_MESH,
   SCR_COUNT
 };
 
@@ -58,10 +58,15 @@ struct MCMulti {
 // Kernel classes timed with HIP events when timing is on.
 enum TimingClass { TC_ADVECT = 0, TC_CROSSINGS, TC_REBUILD, TC_COMPACT, TC_MC, TC_GRAPH, TC_COUNT };
 
+// bytes = per_leaf * (leaf count) + fixed.  A rollout keeps leaf counts on
+// the device; its records carry the segment (and, for mesh writes, the mesh)
+// they belong to and get their bytes once the counts are read back.
 struct TimingRec {
   hipEvent_t a, b;
   int cls;
   double bytes;
+  double per_leaf, fixed;
+  int seg, mesh;
 };
 
 struct vr_ctx {
@@ -74,6 +79,8 @@ struct vr_ctx {
   bool timing = false;
   std::vector<TimingRec> pending;
   std::vector<hipEvent_t> event_pool;
+  int roll_seg = -1;   // rollout segment of the launches being recorded (-1: none)
+  int roll_mesh = -1;  // rollout mesh index of the launches being recorded
   double t_ms[TC_COUNT] = {0, 0, 0, 0, 0, 0};
   double t_bytes[TC_COUNT] = {0, 0, 0, 0, 0, 0};
   uint64_t t_n[TC_COUNT] = {0, 0, 0, 0, 0, 0};
@@ -84,9 +91,10 @@ struct vr_ctx {
 struct TimedLaunch {
   vr_ctx* ctx;
   int cls;
-  double bytes;
+  double per_leaf, fixed, leaves;
+  bool mesh;  // fixed = bytes of the mesh written (known later in a rollout)
   hipEvent_t a = nullptr;
-  TimedLaunch(vr_ctx* c, int k, double b);
+  TimedLaunch(vr_ctx* c, int k, double per_leaf_bytes, double fixed_bytes, double leaf_count, bool mesh_bytes = false);
   ~TimedLaunch();
 };
 
@@ -137,17 +145,24 @@ int vr_hip_fail(vr_ctx* ctx, hipError_t e, const char* where);
 GridView make_view(const vr_grid* g);
 VelView make_vel_view(const vr_velocity* v);
 int build_topology(vr_ctx* ctx, vr_grid* g, std::shared_ptr<Topo> t);
-int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, bool has_empty_after);
+// exact = false: the host leaf count stays an upper bound (no readback; the
+// device count is exact) -- rollouts only, which read the counts back once.
+int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, bool has_empty_after,
+                 bool exact = true);
 int advect_into(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel, double dt, double t, int scheme,
                 vr_grid* o);
 int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any);
-int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* s, vr_mesh* m);
+// deferred: no host synchronisation; the leaf count is the host upper bound,
+// the mesh is written up to its capacity and its counts are copied to
+// d_counts[0..1] (vertices, triangles) instead of being read back.
+int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* s, vr_mesh* m, bool deferred = false,
+           int* d_counts = nullptr);
 int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot);
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
-                   int tcap, float iso, double algorithmic_bytes);
+                   int tcap, float iso);
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, float iso, double* verts, uint32_t* tris,
-                  double algorithmic_bytes);
-int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp);
+                  double algorithmic_bytes, long long vcap = -1, long long tricap = -1);
+int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact = true);
 int count_active_into(vr_ctx* ctx, const vr_grid* g, unsigned long long* slot);
 void copy_header_into(vr_grid* dst, const vr_grid* src);
 double vr_time_factor(double period, double t);

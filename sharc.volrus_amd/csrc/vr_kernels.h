@@ -29,6 +29,7 @@ hipError_t vr_launch_nonempty(const NanoLeaf* leaves, const int* count, int cap,
 hipError_t vr_launch_compact(const NanoLeaf* src, const int* count, int cap, const int* keep, const int* pos,
                              NanoLeaf* dst, hipStream_t s);
 hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s);
+hipError_t vr_zero_tail(int* p, const int* count, int n, hipStream_t s);
 hipError_t vr_launch_popcount_words(const uint64_t* w, const int* count, int cap, int* out, hipStream_t s);
 hipError_t vr_launch_crossing_coords(const int4* origins, const int* count, int cap, const uint64_t* cross,
                                      const int* pos, int* xyz, hipStream_t s);

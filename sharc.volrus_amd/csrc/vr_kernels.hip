@@ -403,6 +403,12 @@ __global__ __launch_bounds__(256) void k_compact_leaves(const NanoLeaf* src, con
 
 __global__ void k_copy_int(const int* src, int* dst) { *dst = *src; }
 
+// p[i] = 0 for *count <= i < n (flags past a device-side count, before a
+// scan whose length is a host upper bound).
+__global__ void k_zero_tail(int* p, const int* count, int n) {
+  for (int i = *count + (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
 // ------------------------------------------------------------------ crossing list
 __global__ void k_crossing_coords(const int4* origins, const int* count, const uint64_t* cross, const int* pos,
                                   int* xyz) {
@@ -488,6 +494,10 @@ hipError_t vr_launch_compact(const NanoLeaf* src, const int* count, int cap, con
 }
 hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, s, src, dst);
+  return hipGetLastError();
+}
+hipError_t vr_zero_tail(int* p, const int* count, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_tail, dim3(grid_for((size_t)(n < 1 ? 1 : n), 256)), dim3(256), 0, s, p, count, n);
   return hipGetLastError();
 }
 hipError_t vr_launch_popcount_words(const uint64_t* w, const int* count, int cap, int* out, hipStream_t s) {

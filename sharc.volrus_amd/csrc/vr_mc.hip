@@ -135,8 +135,13 @@ __device__ __forceinline__ int segment_of(const int* base, int G, long long i) {
 }
 
 // ------------------------------------------------------------------ tile set
-__global__ void k_mc_keep(const GridView* views, const int* leaf_base, int G, int* keep) {
+// Entries [7 * leaf_base[G], n7cap) (a host upper bound on the length) are
+// cleared for the scan.
+__global__ void k_mc_keep(const GridView* views, const int* leaf_base, int G, int* keep, long long n7cap) {
   const long long total = 7LL * leaf_base[G];
+  for (long long t = total + blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n7cap;
+       t += (long long)gridDim.x * blockDim.x)
+    keep[t] = 0;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     const int g = segment_of(leaf_base, G, t / 7);  // 7 * leaf_base is monotone like leaf_base
@@ -434,7 +439,7 @@ __device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves
 // neighbour lists load while the current one is processed.
 __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
                                                          const int* tleaf, const int* ttile, float iso,
-                                                         uint32_t* meta, int* tv, int* tt) {
+                                                         uint32_t* meta, int* tv, int* tt, int tcap) {
   __shared__ uint16_t s_emn[256];
   __shared__ uint32_t s_before[27 * 12];
   __shared__ MCTileLDS s_w[MC_WAVES];
@@ -444,6 +449,11 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   __syncthreads();
   MCTileLDS& w = s_w[wv];
   const int nt = *ntiles;
+  // Totals past the tile count (tcap is a host upper bound) are cleared for the scans.
+  for (int i = nt + (int)(blockIdx.x * MC_THREADS) + tid; i < tcap; i += gridDim.x * MC_THREADS) {
+    tv[i] = 0;
+    tt[i] = 0;
+  }
   int t_end, t_step;
   int t = xcd_first_sub(nt, wv, MC_WAVES, &t_end, &t_step);
   int ntl = -1, ntt = -1;
@@ -515,7 +525,8 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
                                                          const int* tleaf, const int* ttile, const int* tile_base,
                                                          float iso, const uint32_t* meta, const int* tv, const int* tt,
                                                          const int* vbase, const int* tbase, const int* totals,
-                                                         double* verts, uint32_t* tris) {
+                                                         double* verts, uint32_t* tris, long long vcap,
+                                                         long long tricap) {
   __shared__ uint16_t s_emn[256];
   __shared__ __align__(16) int8_t s_tri[256 * 16];  // TRI_TABLE rows
   __shared__ uint32_t s_ownsort[27 * 12];
@@ -537,7 +548,8 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
   MCTileLDS& w = s_w[wv];
   MCEmitLDS& el = s_e[wv];
   const int nt = *ntiles;
-  const int vmax = totals[1], tmax = totals[2];
+  // writes stop at the counts or at the output capacity, whichever is lower
+  const int vmax = (int)min((long long)totals[1], vcap), tmax = (int)min((long long)totals[2], tricap);
   int t_end, t_step;
   int t = xcd_first_sub(nt, wv, MC_EWAVES, &t_end, &t_step);
   // Header of the next tile, loaded while the current one is processed.
@@ -695,7 +707,7 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
       (e = s.tile_base.ensure(sizeof(int) * (size_t)(G + 1))) != hipSuccess)
     return vr_hip_fail(ctx, e, "scratch");
   hipLaunchKernelGGL(k_mc_keep, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views, d_leaf_base, G,
-                     s.keep.as<int>());
+                     s.keep.as<int>(), (long long)n7);
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = vr_exclusive_scan(s.keep.as<int>(), s.pos.as<int>(), n7, s.sums.as<int>(), s.totals.as<int>(),
                              ctx->stream)) != hipSuccess)
@@ -707,7 +719,7 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
 
 // Phase B (tile capacity tcap known): scatter, neighbours, count, scans.
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
-                   int tcap, float iso, double algorithmic_bytes) {
+                   int tcap, float iso) {
   hipError_t e;
   const size_t n7 = (size_t)(7 * ltot);
   const size_t tc = (size_t)(tcap + 1);
@@ -726,10 +738,10 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
                      s.tiles.as<int4>(), totals, s.tile_base.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>());
   if ((e = hipGetLastError()) != hipSuccess) return vr_hip_fail(ctx, e, "mc tiles");
   {
-    TimedLaunch tl(ctx, TC_MC, algorithmic_bytes);
+    TimedLaunch tl(ctx, TC_MC, (double)VR_LEAF_BYTES, 0.0, (double)ltot);
     hipLaunchKernelGGL(k_mc_count, dim3(mc_blocks(ctx, (const void*)k_mc_count, MC_WAVES, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        totals, s.tleaf.as<int>(), s.ttile.as<int>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(),
-                       s.tt.as<int>());
+                       s.tt.as<int>(), tcap);
   }
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = vr_exclusive_scan(s.tv.as<int>(), s.vb.as<int>(), (size_t)tcap, s.sums2.as<int>(), totals + 1,
@@ -744,38 +756,74 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
 
 // Phase C: emit into verts/tris (capacities from the counts).
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, float iso, double* verts, uint32_t* tris,
-                  double algorithmic_bytes) {
+                  double algorithmic_bytes, long long vcap, long long tricap) {
   hipError_t e;
+  if (vcap < 0) vcap = 0x7fffffffLL;
+  if (tricap < 0) tricap = 0x7fffffffLL;
   {
-    TimedLaunch tl(ctx, TC_MC, algorithmic_bytes);
+    TimedLaunch tl(ctx, TC_MC, 0.0, algorithmic_bytes, 0.0, true);
     hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, (const void*)k_mc_emit, MC_EWAVES, tcap)), dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(), iso,
                        s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
-                       s.totals.as<int>(), verts, tris);
+                       s.totals.as<int>(), verts, tris, vcap, tricap);
   }
   return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc emit");
 }
 
+// The view and the leaf range [0, *count) of a single grid, written on the
+// device (no host copy).
+__global__ void k_mc_setup(GridView v, GridView* views, const int* count, int* leaf_base) {
+  if (threadIdx.x == 0) {
+    views[0] = v;
+    leaf_base[0] = 0;
+    leaf_base[1] = *count;
+  }
+}
+
 // Single grid (volume_to_mesh) through the multi-grid passes with G = 1.
-int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh* m) {
+int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh* m, bool deferred, int* d_counts) {
   (void)unused;
   hipError_t e;
-  const int n = g->topo->count;
+  const int n = g->topo->count;  // exact, or an upper bound when deferred
   MCMulti& s = ctx->mc;
   if ((e = s.views.ensure(sizeof(GridView))) != hipSuccess || (e = s.leaf_base.ensure(sizeof(int) * 2)) != hipSuccess)
     return vr_hip_fail(ctx, e, "scratch");
-  const GridView v = make_view(g);
-  const int lb[2] = {0, n};
-  if ((e = hipMemcpyAsync(s.views.p, &v, sizeof(GridView), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(s.leaf_base.p, lb, sizeof(lb), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
-    return vr_hip_fail(ctx, e, "hipMemcpyAsync");
+  hipLaunchKernelGGL(k_mc_setup, dim3(1), dim3(64), 0, ctx->stream, make_view(g), s.views.as<GridView>(),
+                     (const int*)g->topo->d_count, s.leaf_base.as<int>());
+  if ((e = hipGetLastError()) != hipSuccess) return vr_hip_fail(ctx, e, "mc setup");
   int rc = mc_tiles_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n);
   if (rc != VR_OK) return rc;
+  if (deferred) {
+    // Capacities from the bounds: at most 7 tiles per leaf; the mesh keeps
+    // the capacity it has (a first call takes a guess); the counts go to
+    // d_counts and the caller checks them against the capacity.
+    const int tcap = 7 * (n < 1 ? 1 : n);
+    rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, tcap, iso);
+    if (rc != VR_OK) return rc;
+    if ((e = vr_copy_int(s.totals.as<int>() + 1, d_counts, ctx->stream)) != hipSuccess ||
+        (e = vr_copy_int(s.totals.as<int>() + 2, d_counts + 1, ctx->stream)) != hipSuccess)
+      return vr_hip_fail(ctx, e, "mc counts");
+    if (m->cap_v < 1 || m->cap_t < 1) {
+      const uint64_t cv = (uint64_t)16 * (uint64_t)(n < 256 ? 256 : n);
+      if (m->d_v) (void)hipFree(m->d_v);
+      if (m->d_t) (void)hipFree(m->d_t);
+      m->d_v = nullptr;
+      m->d_t = nullptr;
+      m->cap_v = m->cap_t = 0;
+      if ((e = hipMalloc(&m->d_v, sizeof(double) * 3 * cv)) != hipSuccess ||
+          (e = hipMalloc(&m->d_t, sizeof(uint32_t) * 3 * 2 * cv)) != hipSuccess)
+        return vr_hip_fail(ctx, e, "hipMalloc mesh");
+      m->cap_v = cv;
+      m->cap_t = 2 * cv;
+    }
+    return mc_emit_multi(ctx, s, s.views.as<GridView>(), tcap, iso, m->d_v, m->d_t, 0.0, (long long)m->cap_v,
+                         (long long)m->cap_t);
+  }
   int nt = 0;
   if ((e = hipMemcpyAsync(&nt, s.totals.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
     return vr_hip_fail(ctx, e, "readback");
-  rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, nt, iso, (double)VR_LEAF_BYTES * n);
+  rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, nt, iso);
   if (rc != VR_OK) return rc;
   int th[3] = {0, 0, 0};
   if ((e = hipMemcpyAsync(th, s.totals.p, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||

@@ -9,7 +9,9 @@
 #include "vr_internal.h"
 
 // ------------------------------------------------------------------ event timing
-TimedLaunch::TimedLaunch(vr_ctx* c, int k, double b) : ctx(c), cls(k), bytes(b) {
+TimedLaunch::TimedLaunch(vr_ctx* c, int k, double per_leaf_bytes, double fixed_bytes, double leaf_count,
+                         bool mesh_bytes)
+    : ctx(c), cls(k), per_leaf(per_leaf_bytes), fixed(fixed_bytes), leaves(leaf_count), mesh(mesh_bytes) {
   if (!ctx || !ctx->timing) return;
   if (ctx->event_pool.empty()) {
     if (hipEventCreate(&a) != hipSuccess) {
@@ -33,7 +35,8 @@ TimedLaunch::~TimedLaunch() {
     ctx->event_pool.pop_back();
   }
   (void)hipEventRecord(b, ctx->stream);
-  ctx->pending.push_back({a, b, cls, bytes});
+  ctx->pending.push_back({a, b, cls, per_leaf * leaves + fixed, per_leaf, fixed, ctx->roll_seg,
+                          mesh ? ctx->roll_mesh : -1});
 }
 
 static void drain_timing(vr_ctx* ctx) {
@@ -190,15 +193,30 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
     if (rc0 != VR_OK) return rc0;
     g->topo = t;
   }
-  // Active voxels of each topology segment (between rebuilds) are counted on
-  // the device into one slot per segment and read back once at the end.
+  // Counts stay on the device until the end: per topology segment (between
+  // rebuilds) the active voxels and the leaf count, per mesh its vertex and
+  // triangle counts.  The host leaf count is an upper bound meanwhile (it
+  // only sizes buffers and launch grids; kernels read the device count).
   const int nseg = 1 + (rebuild_every > 0 ? steps / rebuild_every : 0);
+  const int nmesh = mesh_every > 0 ? steps / mesh_every : 0;
   std::vector<int> seg_steps((size_t)nseg, 0);
   DevBuf& slots = ctx->scratch[SCR_ACTIVE];
-  if ((e = slots.ensure(sizeof(unsigned long long) * (size_t)nseg)) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
-  int seg = 0;
+  DevBuf& seg_leaves = ctx->scratch[SCR_ROLL_LEAVES];
+  DevBuf& mesh_counts = ctx->scratch[SCR_ROLL_MESH];
+  if ((e = slots.ensure(sizeof(unsigned long long) * (size_t)nseg)) != hipSuccess ||
+      (e = seg_leaves.ensure(sizeof(int) * (size_t)nseg)) != hipSuccess ||
+      (e = mesh_counts.ensure(sizeof(int) * 2 * (size_t)(nmesh + 1))) != hipSuccess)
+    return vr_hip_fail(ctx, e, "scratch");
+  const size_t rec0 = ctx->pending.size();  // timing records of this rollout start here
+  int seg = 0, imesh = -1;
+  ctx->roll_seg = 0;
   int rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg);
-  if (rc != VR_OK) return rc;
+  if (rc == VR_OK && (e = vr_copy_int(g->topo->d_count, seg_leaves.as<int>(), ctx->stream)) != hipSuccess)
+    rc = vr_hip_fail(ctx, e, "count");
+  if (rc != VR_OK) {
+    ctx->roll_seg = -1;
+    return rc;
+  }
   // Second leaf buffer (context scratch; ownership is swapped back at the end);
   // the advected grid shares g's topology.
   const int cap = g->topo->cap < 1 ? 1 : g->topo->cap;
@@ -242,43 +260,104 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
         break;
       }
       std::swap(g->d_leaves, other);
-      rc = compact_into(ctx, g, other, keep.as<int>(), false);
+      rc = compact_into(ctx, g, other, keep.as<int>(), false, false);
       if (rc != VR_OK) break;
     }
     if (rebuild_every > 0 && (s + 1) % rebuild_every == 0) {
-      rc = rebuild_impl(ctx, g, half_width, other);
+      rc = rebuild_impl(ctx, g, half_width, other, false);
       if (rc != VR_OK) break;
       st.rebuilds += 1;
       if (seg + 1 < nseg) {
         ++seg;
+        ctx->roll_seg = seg;
         rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg);
         if (rc != VR_OK) break;
+        if ((e = vr_copy_int(g->topo->d_count, seg_leaves.as<int>() + seg, ctx->stream)) != hipSuccess) {
+          rc = vr_hip_fail(ctx, e, "count");
+          break;
+        }
       }
     }
-    if (mesh_every > 0 && (s + 1) % mesh_every == 0) {
-      if (g->topo->count > 0) {
+    if (mesh_every > 0 && (s + 1) % mesh_every == 0 && imesh + 1 < nmesh) {
+      ++imesh;
+      if (g->topo->count > 0) {  // an upper bound: 0 means no leaves
+        ctx->roll_mesh = imesh;
         MCScratch sc;
-        rc = mc_run(ctx, g, iso, &sc, mesh);
-        if (rc != VR_OK) break;
-      } else {
-        mesh->nv = mesh->nt = 0;
+        rc = mc_run(ctx, g, iso, &sc, mesh, true, mesh_counts.as<int>() + 2 * imesh);
+        ctx->roll_mesh = -1;
+      } else if ((e = vr_fill_int(mesh_counts.as<int>() + 2 * imesh, 0, 2, ctx->stream)) != hipSuccess) {
+        rc = vr_hip_fail(ctx, e, "mesh counts");
       }
+      if (rc != VR_OK) break;
       st.meshes += 1;
     }
   }
+  ctx->roll_seg = -1;
+  ctx->roll_mesh = -1;
   // The grid keeps whichever buffer holds its data; the scratch slot takes
   // the spare one (`other` is never freed by the steps above).
   ping.p = other;
   ping.bytes = other == pbuf ? pbuf_bytes : own_bytes;
   if (g->d_leaves == pbuf) g->cap = (int)(pbuf_bytes / sizeof(NanoLeaf));
   std::vector<unsigned long long> act((size_t)nseg, 0);
+  std::vector<int> leaves((size_t)nseg, 0), mc((size_t)2 * (nmesh + 1), 0);
   if (rc == VR_OK &&
       ((e = hipMemcpyAsync(act.data(), slots.p, sizeof(unsigned long long) * nseg, hipMemcpyDeviceToHost,
                            ctx->stream)) != hipSuccess ||
+       (e = hipMemcpyAsync(leaves.data(), seg_leaves.p, sizeof(int) * nseg, hipMemcpyDeviceToHost, ctx->stream)) !=
+           hipSuccess ||
+       (e = hipMemcpyAsync(mc.data(), mesh_counts.p, sizeof(int) * mc.size(), hipMemcpyDeviceToHost, ctx->stream)) !=
+           hipSuccess ||
        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess))
     rc = vr_hip_fail(ctx, e, "readback");
-  if (rc != VR_OK) return rc;
+  if (rc != VR_OK) {
+    // leave the host count exact for later calls, if the device still answers
+    int cnt = 0;
+    if (hipMemcpy(&cnt, g->topo->d_count, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess) g->topo->count = cnt;
+    return rc;
+  }
+  g->topo->count = leaves[seg];  // exact again
+  // Algorithmic bytes of this rollout's timed launches, from the real counts.
+  for (size_t i = rec0; i < ctx->pending.size(); ++i) {
+    TimingRec& r = ctx->pending[i];
+    if (r.seg < 0) continue;
+    const double mesh_bytes = r.mesh >= 0 ? 24.0 * mc[2 * r.mesh] + 12.0 * mc[2 * r.mesh + 1] : r.fixed;
+    r.bytes = r.per_leaf * leaves[r.seg] + mesh_bytes;
+  }
   for (int k = 0; k <= seg; ++k) st.voxel_updates += act[k] * (uint64_t)seg_steps[k];
+  if (mesh && imesh >= 0) {
+    // Meshes past the capacity were written up to it: grow the capacity to
+    // the largest count, and mesh the final state again if it was cut.
+    uint64_t mv = 0, mt = 0;
+    for (int k = 0; k <= imesh; ++k) {
+      mv = std::max<uint64_t>(mv, (uint64_t)mc[2 * k]);
+      mt = std::max<uint64_t>(mt, (uint64_t)mc[2 * k + 1]);
+    }
+    mesh->nv = (uint64_t)mc[2 * imesh];
+    mesh->nt = (uint64_t)mc[2 * imesh + 1];
+    if (mv > mesh->cap_v || mt > mesh->cap_t) {
+      if (mv > mesh->cap_v) {
+        if (mesh->d_v) (void)hipFree(mesh->d_v);
+        mesh->d_v = nullptr;
+        mesh->cap_v = 0;
+        if ((e = hipMalloc(&mesh->d_v, sizeof(double) * 3 * (size_t)(mv + 1))) != hipSuccess)
+          return vr_hip_fail(ctx, e, "hipMalloc mesh");
+        mesh->cap_v = mv + 1;
+      }
+      if (mt > mesh->cap_t) {
+        if (mesh->d_t) (void)hipFree(mesh->d_t);
+        mesh->d_t = nullptr;
+        mesh->cap_t = 0;
+        if ((e = hipMalloc(&mesh->d_t, sizeof(uint32_t) * 3 * (size_t)(mt + 1))) != hipSuccess)
+          return vr_hip_fail(ctx, e, "hipMalloc mesh");
+        mesh->cap_t = mt + 1;
+      }
+      // new buffers: mesh the final state into them (host-synchronised)
+      MCScratch sc;
+      rc = mc_run(ctx, g, iso, &sc, mesh);
+      if (rc != VR_OK) return rc;
+    }
+  }
   if (mesh) {
     st.last_vertex_count = mesh->nv;
     st.last_triangle_count = mesh->nt;

@@ -316,6 +316,32 @@ def test_rollout_matches_oracle_composition(orc, ctx, vr, shared):
         assert (st.last_vertex_count, st.last_triangle_count) == (len(wmesh[0]), len(wmesh[1]))
 
 
+def test_rollout_mesh_capacity_regrowth(orc, ctx, vr):
+    """A mesh larger than a fresh mesh's first capacity guess (16 vertices per
+    leaf): the rollout grows the buffers and meshes the final state again, and
+    the next rollout fits; both equal the oracle."""
+    announce("rollout mesh regrowth")
+    og = sphere128(orc)
+    nano = og.to_nano()
+    leaves = (len(nano) - 320) // 2128
+    params = (0.0, 0.0, 2 * math.pi, 0.5, 0.5, 0.5)
+    dt = sphere128_rigid_dt()
+    steps, rb, me = 4, 2, 2
+    want, updates, wmesh = oracle_rollout(orc, og, 3, params, 0.0, 0.0, dt, 3, steps, rb, 3.0, me)
+    assert len(wmesh[0]) > 16 * max(leaves, 256)
+    vel = ctx.velocity_analytic(vr.VEL_RIGID, params, 0.0)
+    base = ctx.upload(nano)
+    mesh = vr.Mesh(ctx)
+    for rep in range(2):
+        g = base.clone()
+        st = g.rollout(vel, 0.0, dt, vr.RK3, steps, rb, 3.0, me, 0.0, mesh)
+        assert_nano_equal(g.download(), want.to_nano(), f"regrowth rollout rep {rep}")
+        assert st.voxel_updates == updates
+        vg, tg = mesh.arrays()
+        mesh_equal(vg, tg, wmesh[0], wmesh[1], 1.0 / 128, f"regrowth mesh rep {rep}")
+        assert (st.last_vertex_count, st.last_triangle_count) == (len(wmesh[0]), len(wmesh[1]))
+
+
 # ---------------------------------------------------------------- FP32 error against the FP64 oracle
 @pytest.mark.parametrize("kind,params,scheme,steps", [
     (3, (0, 0, 2 * math.pi, 0.5, 0.5, 0.5), 1, 1),   # rigid rotation, one RK1 step
