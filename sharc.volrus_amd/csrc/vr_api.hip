@@ -233,7 +233,8 @@ void copy_header_into(vr_grid* dst, const vr_grid* src) {
 // in order, as g's new leaf set, and rebuilds g's lookup arrays.  Mirrors
 // remove_empty_leaves (src/tree/internal.rs:231-247) followed by the sorted
 // leaf order of NanoGrid::from_grid.
-int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, bool has_empty_after, bool exact) {
+int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, bool has_empty_after, bool exact,
+                 bool tail_clear, int* count_copy) {
   hipError_t e;
   const int n = g->topo->count;
   DevBuf& pos = ctx->scratch[SCR_POS];
@@ -249,11 +250,15 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
     if ((e = tot.ensure(sizeof(int) * 4)) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
     // With the host count an upper bound, the flags past the device count
     // are cleared so the scan sees only live leaves.
-    if ((!exact && (e = vr_zero_tail(const_cast<int*>(keep), t->d_count, n, ctx->stream)) != hipSuccess) ||
+    // The compaction also resets the internal nodes' leaf slots for the
+    // lookup rebuild below.
+    const size_t R = t->roots.size();
+    if ((!exact && !tail_clear && (e = vr_zero_tail(const_cast<int*>(keep), t->d_count, n, ctx->stream)) != hipSuccess) ||
         (e = vr_exclusive_scan(keep, pos.as<int>(), (size_t)n, sums.as<int>(), tot.as<int>(), ctx->stream)) !=
             hipSuccess ||
-        (e = vr_launch_compact(src, t->d_count, n, keep, pos.as<int>(), g->d_leaves, ctx->stream)) != hipSuccess ||
-        (e = vr_copy_int(tot.as<int>(), t->d_count, ctx->stream)) != hipSuccess)
+        (e = vr_launch_compact(src, t->d_count, n, keep, pos.as<int>(), g->d_leaves, ctx->stream, t->d_child,
+                               4096 * (R ? R : 1))) != hipSuccess ||
+        (e = vr_copy_int(tot.as<int>(), t->d_count, ctx->stream, count_copy)) != hipSuccess)
       return vr_hip_fail(ctx, e, "compact");
     if (exact) {
       int cnt = 0;
@@ -263,9 +268,7 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
       t->count = cnt;
     }
     t->has_empty = has_empty_after;
-    const size_t R = t->roots.size();
-    if ((e = vr_fill_int(t->d_child, -1, 4096 * (R ? R : 1), ctx->stream)) != hipSuccess ||
-        (e = vr_build_child(g->d_leaves, t->d_count, t->cap < 1 ? 1 : t->cap, t->d_roots, (int)R, t->d_child,
+    if ((e = vr_build_child(g->d_leaves, t->d_count, t->cap < 1 ? 1 : t->cap, t->d_roots, (int)R, t->d_child,
                             t->d_origins, ctx->stream)) != hipSuccess ||
         (e = vr_build_nbr(make_view(g), t->cap < 1 ? 1 : t->cap, t->d_nbr, ctx->stream)) != hipSuccess)
       return vr_hip_fail(ctx, e, "topology");
@@ -292,6 +295,10 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
   }
   t->count = cnt;
   t->has_empty = has_empty_after;
+  if (count_copy && (e = vr_copy_int(t->d_count, count_copy, ctx->stream)) != hipSuccess) {
+    (void)hipFree(dst);
+    return vr_hip_fail(ctx, e, "count");
+  }
   if (g->d_leaves) (void)hipFree(g->d_leaves);
   g->d_leaves = dst;
   g->cap = n;
@@ -692,7 +699,7 @@ extern "C" int vr_trim_band(vr_ctx* ctx, vr_grid* g, float half_width) {
 
 // Crossings, bounded-radius redistance into `tmp` (>= count leaves), then
 // compaction back into g.
-int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact) {
+int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact, int* count_copy) {
   const double rr = std::ceil((double)half_width);
   if (!(rr <= 8.0)) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "rebuild supports ceil(half_width) <= 8");
   const int R = rr < 0.0 ? 0 : (int)rr;
@@ -708,7 +715,7 @@ int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool 
   if ((e = keep.ensure(sizeof(int) * (size_t)(n + 1))) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
   {
     TimedLaunch tl(ctx, TC_REBUILD, (double)(2 * VR_LEAF_BYTES + 64), 0.0, n);
-    e = vr_launch_rebuild(make_view(g), g->topo->cap, cross, any, tmp, keep.as<int>(), R, vs, band, ctx->stream);
+    e = vr_launch_rebuild(make_view(g), g->topo->cap, cross, any, tmp, keep.as<int>(), n, R, vs, band, ctx->stream);
   }
   if (e != hipSuccess) return vr_hip_fail(ctx, e, "rebuild");
   // Leaves without active voxels survive only when the grid has no crossing
@@ -719,7 +726,7 @@ int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool 
       ((e = hipMemcpyAsync(&anyh, any, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess))
     return vr_hip_fail(ctx, e, "readback");
-  return compact_into(ctx, g, tmp, keep.as<int>(), anyh ? false : g->topo->has_empty, exact);
+  return compact_into(ctx, g, tmp, keep.as<int>(), anyh ? false : g->topo->has_empty, exact, true, count_copy);
 }
 
 extern "C" int vr_rebuild_band(vr_ctx* ctx, vr_grid* g, float half_width) {

@@ -147,8 +147,10 @@ VelView make_vel_view(const vr_velocity* v);
 int build_topology(vr_ctx* ctx, vr_grid* g, std::shared_ptr<Topo> t);
 // exact = false: the host leaf count stays an upper bound (no readback; the
 // device count is exact) -- rollouts only, which read the counts back once.
+// tail_clear: the caller already cleared the flags past the device count.
+// count_copy (optional): also receives the new device count.
 int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, bool has_empty_after,
-                 bool exact = true);
+                 bool exact = true, bool tail_clear = false, int* count_copy = nullptr);
 int advect_into(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel, double dt, double t, int scheme,
                 vr_grid* o);
 int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any);
@@ -159,11 +161,13 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* s, vr_mesh* m, b
            int* d_counts = nullptr);
 int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot);
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
-                   int tcap, float iso);
+                   int tcap, float iso, int* counts_out = nullptr);
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, float iso, double* verts, uint32_t* tris,
                   double algorithmic_bytes, long long vcap = -1, long long tricap = -1);
-int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact = true);
-int count_active_into(vr_ctx* ctx, const vr_grid* g, unsigned long long* slot);
+int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact = true,
+                 int* count_copy = nullptr);
+// zero = false: *slot is already 0.
+int count_active_into(vr_ctx* ctx, const vr_grid* g, unsigned long long* slot, bool zero = true);
 void copy_header_into(vr_grid* dst, const vr_grid* src);
 double vr_time_factor(double period, double t);
 void vr_stage_factors(double period, double t, double dt, int scheme, double tf[3]);

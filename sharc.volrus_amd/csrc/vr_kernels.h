@@ -23,12 +23,12 @@ hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* ou
 inline int vr_advect_waves(long long leaves, int cus) { return leaves >= 16LL * (cus > 0 ? cus : 256) ? 1 : 4; }
 hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, hipStream_t s);
 hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, NanoLeaf* out,
-                             int* keep, int R, float vs, float band, hipStream_t s);
+                             int* keep, int keep_len, int R, float vs, float band, hipStream_t s);
 hipError_t vr_launch_trim(const GridView& g, int cap, NanoLeaf* out, int* keep, float threshold, hipStream_t s);
 hipError_t vr_launch_nonempty(const NanoLeaf* leaves, const int* count, int cap, int* keep, hipStream_t s);
 hipError_t vr_launch_compact(const NanoLeaf* src, const int* count, int cap, const int* keep, const int* pos,
-                             NanoLeaf* dst, hipStream_t s);
-hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s);
+                             NanoLeaf* dst, hipStream_t s, int* child_fill = nullptr, size_t nchild = 0);
+hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s, int* dst2 = nullptr);
 hipError_t vr_zero_tail(int* p, const int* count, int n, hipStream_t s);
 hipError_t vr_launch_popcount_words(const uint64_t* w, const int* count, int cap, int* out, hipStream_t s);
 hipError_t vr_launch_crossing_coords(const int4* origins, const int* count, int cap, const uint64_t* cross,

@@ -262,14 +262,17 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
 // (bit scans), then the minimum over dy, then over dx, in small integers.
 // No crossings anywhere: the grid is copied unchanged (the reference returns
 // early, :85-87).
+// keep[*count .. keep_len) is cleared (block 0) for a scan of keep_len flags.
 __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cross, const int* any, NanoLeaf* out,
-                                                int* keep, int R, float vs, float band) {
+                                                int* keep, int keep_len, int R, float vs, float band) {
   __shared__ uint32_t rows[24 * 24];
   __shared__ uint8_t dz2[24 * 24 * 8];  // row (X, Y), z: dz^2 to the nearest bit, 255 = none within R
   __shared__ uint8_t dyz2[24 * 8 * 8];  // X, y, z: min over dy of dy^2 + dz2, 255 = none
   __shared__ int nb[VR_NBR];
   const int lane = threadIdx.x;
   const int n = *g.count;
+  if (blockIdx.x == 0)
+    for (int i = n + lane; i < keep_len; i += 64) keep[i] = 0;
   const bool have = *any != 0;
   const int y = lane >> 3, z = lane & 7;
   const int lo = 8 - R, span = 8 + 2 * R;  // rows X, Y in [lo, lo + span)
@@ -396,8 +399,14 @@ __global__ void k_nonempty(const NanoLeaf* leaves, const int* count, int* keep) 
 }
 
 // Stable compaction of leaves: dst[pos[i]] = src[i] where keep[i].
+// child_fill (optional): the leaf slots of the internal nodes, reset to -1
+// for the lookup rebuild that follows.
 __global__ __launch_bounds__(256) void k_compact_leaves(const NanoLeaf* src, const int* count, const int* keep,
-                                                        const int* pos, NanoLeaf* dst) {
+                                                        const int* pos, NanoLeaf* dst, int* child_fill,
+                                                        size_t nchild) {
+  if (child_fill)
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nchild; i += (size_t)gridDim.x * blockDim.x)
+      child_fill[i] = -1;
   const int n = *count;
   const int sub = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = blockIdx.x * 4 + sub; i < n; i += gridDim.x * 4) {
@@ -408,7 +417,11 @@ __global__ __launch_bounds__(256) void k_compact_leaves(const NanoLeaf* src, con
   }
 }
 
-__global__ void k_copy_int(const int* src, int* dst) { *dst = *src; }
+__global__ void k_copy_int(const int* src, int* dst, int* dst2) {
+  const int v = *src;
+  *dst = v;
+  if (dst2) *dst2 = v;
+}
 
 // p[i] = 0 for *count <= i < n (flags past a device-side count, before a
 // scan whose length is a host upper bound).
@@ -482,8 +495,9 @@ hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int*
   return hipGetLastError();
 }
 hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, NanoLeaf* out,
-                             int* keep, int R, float vs, float band, hipStream_t s) {
-  hipLaunchKernelGGL(k_rebuild, dim3(leaf_blocks(cap)), dim3(64), 0, s, g, cross, any, out, keep, R, vs, band);
+                             int* keep, int keep_len, int R, float vs, float band, hipStream_t s) {
+  hipLaunchKernelGGL(k_rebuild, dim3(leaf_blocks(cap)), dim3(64), 0, s, g, cross, any, out, keep, keep_len, R, vs,
+                     band);
   return hipGetLastError();
 }
 hipError_t vr_launch_trim(const GridView& g, int cap, NanoLeaf* out, int* keep, float threshold, hipStream_t s) {
@@ -495,12 +509,13 @@ hipError_t vr_launch_nonempty(const NanoLeaf* leaves, const int* count, int cap,
   return hipGetLastError();
 }
 hipError_t vr_launch_compact(const NanoLeaf* src, const int* count, int cap, const int* keep, const int* pos,
-                             NanoLeaf* dst, hipStream_t s) {
-  hipLaunchKernelGGL(k_compact_leaves, dim3(grid_for(cap, 4)), dim3(256), 0, s, src, count, keep, pos, dst);
+                             NanoLeaf* dst, hipStream_t s, int* child_fill, size_t nchild) {
+  hipLaunchKernelGGL(k_compact_leaves, dim3(grid_for(cap, 4)), dim3(256), 0, s, src, count, keep, pos, dst,
+                     child_fill, nchild);
   return hipGetLastError();
 }
-hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, s, src, dst);
+hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s, int* dst2) {
+  hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(1), 0, s, src, dst, dst2);
   return hipGetLastError();
 }
 hipError_t vr_zero_tail(int* p, const int* count, int n, hipStream_t s) {

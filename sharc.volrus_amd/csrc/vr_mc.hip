@@ -664,9 +664,14 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
 
 // Per-grid vertex and triangle ranges: out[4 g] = first vertex, [4 g + 1] =
 // vertex count, [4 g + 2] = first triangle, [4 g + 3] = triangle count.
+// counts_out (optional): the vertex and triangle totals.
 __global__ void k_mc_ranges(const int* tile_base, int G, const int* ntiles, const int* vbase, const int* tbase,
-                            const int* totals, int* out) {
+                            const int* totals, int* out, int* counts_out) {
   const int nt = *ntiles;
+  if (counts_out && blockIdx.x == 0 && threadIdx.x == 0) {
+    counts_out[0] = totals[1];
+    counts_out[1] = totals[2];
+  }
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
     const int a = tile_base[g], b = tile_base[g + 1];
     const int va = a < nt ? vbase[a] : totals[1], vb = b < nt ? vbase[b] : totals[1];
@@ -719,7 +724,7 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
 
 // Phase B (tile capacity tcap known): scatter, neighbours, count, scans.
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
-                   int tcap, float iso) {
+                   int tcap, float iso, int* counts_out) {
   hipError_t e;
   const size_t n7 = (size_t)(7 * ltot);
   const size_t tc = (size_t)(tcap + 1);
@@ -750,7 +755,7 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
                              ctx->stream)) != hipSuccess)
     return vr_hip_fail(ctx, e, "mc count");
   hipLaunchKernelGGL(k_mc_ranges, dim3(blocks_for(G, 256)), dim3(256), 0, ctx->stream, s.tile_base.as<int>(), G, totals,
-                     s.vb.as<int>(), s.tb.as<int>(), totals, s.ranges.as<int>());
+                     s.vb.as<int>(), s.tb.as<int>(), totals, s.ranges.as<int>(), counts_out);
   return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc ranges");
 }
 
@@ -798,11 +803,8 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
     // the capacity it has (a first call takes a guess); the counts go to
     // d_counts and the caller checks them against the capacity.
     const int tcap = 7 * (n < 1 ? 1 : n);
-    rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, tcap, iso);
+    rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, tcap, iso, d_counts);
     if (rc != VR_OK) return rc;
-    if ((e = vr_copy_int(s.totals.as<int>() + 1, d_counts, ctx->stream)) != hipSuccess ||
-        (e = vr_copy_int(s.totals.as<int>() + 2, d_counts + 1, ctx->stream)) != hipSuccess)
-      return vr_hip_fail(ctx, e, "mc counts");
     if (m->cap_v < 1 || m->cap_t < 1) {
       const uint64_t cv = (uint64_t)16 * (uint64_t)(n < 256 ? 256 : n);
       if (m->d_v) (void)hipFree(m->d_v);

@@ -99,11 +99,11 @@ __global__ __launch_bounds__(256) void k_active_sum(const NanoLeaf* __restrict__
 }
 
 // Device-side active-voxel count of g into *slot (no host synchronisation).
-int count_active_into(vr_ctx* ctx, const vr_grid* g, unsigned long long* slot) {
+int count_active_into(vr_ctx* ctx, const vr_grid* g, unsigned long long* slot, bool zero) {
   const int cap = g->topo->cap < 1 ? 1 : g->topo->cap;
   size_t blocks = ((size_t)cap * 8 + 255) / 256;
   if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, slot);
+  if (zero) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, slot);
   hipLaunchKernelGGL(k_active_sum, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, g->d_leaves, g->topo->d_count,
                      slot);
   hipError_t e = hipGetLastError();
@@ -210,9 +210,13 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
   const size_t rec0 = ctx->pending.size();  // timing records of this rollout start here
   int seg = 0, imesh = -1;
   ctx->roll_seg = 0;
-  int rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg);
-  if (rc == VR_OK && (e = vr_copy_int(g->topo->d_count, seg_leaves.as<int>(), ctx->stream)) != hipSuccess)
-    rc = vr_hip_fail(ctx, e, "count");
+  // all segment slots zeroed at once; each rebuild's compaction copies the
+  // new leaf count into the next segment's slot
+  int rc = VR_OK;
+  if ((e = vr_fill_int(slots.as<int>(), 0, 2 * (size_t)nseg, ctx->stream)) != hipSuccess ||
+      (e = vr_copy_int(g->topo->d_count, seg_leaves.as<int>(), ctx->stream)) != hipSuccess)
+    rc = vr_hip_fail(ctx, e, "counts");
+  if (rc == VR_OK) rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg, false);
   if (rc != VR_OK) {
     ctx->roll_seg = -1;
     return rc;
@@ -264,18 +268,15 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
       if (rc != VR_OK) break;
     }
     if (rebuild_every > 0 && (s + 1) % rebuild_every == 0) {
-      rc = rebuild_impl(ctx, g, half_width, other, false);
+      const bool next = seg + 1 < nseg;
+      rc = rebuild_impl(ctx, g, half_width, other, false, next ? seg_leaves.as<int>() + seg + 1 : nullptr);
       if (rc != VR_OK) break;
       st.rebuilds += 1;
-      if (seg + 1 < nseg) {
+      if (next) {
         ++seg;
         ctx->roll_seg = seg;
-        rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg);
+        rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg, false);
         if (rc != VR_OK) break;
-        if ((e = vr_copy_int(g->topo->d_count, seg_leaves.as<int>() + seg, ctx->stream)) != hipSuccess) {
-          rc = vr_hip_fail(ctx, e, "count");
-          break;
-        }
       }
     }
     if (mesh_every > 0 && (s + 1) % mesh_every == 0 && imesh + 1 < nmesh) {
