@@ -616,7 +616,7 @@ extern "C" int vr_advect(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel,
 }
 
 // ------------------------------------------------------------------ band tools
-int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any) {
+int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any, int* gen) {
   hipError_t e;
   const int n = g->topo->count;
   DevBuf& cb = ctx->scratch[SCR_CROSS];
@@ -624,14 +624,22 @@ int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any) {
   if ((e = cb.ensure(sizeof(uint64_t) * 8 * (size_t)(n + 1))) != hipSuccess ||
       (e = ab.ensure(sizeof(int) * 4)) != hipSuccess)
     return vr_hip_fail(ctx, e, "scratch");
-  if ((e = vr_fill_int(ab.as<int>(), 0, 1, ctx->stream)) != hipSuccess) return vr_hip_fail(ctx, e, "crossings");
+  // The mark is the largest generation that found a crossing, so it needs
+  // no reset between passes: only a new buffer or a wrapped counter clears it.
+  if (ctx->cross_any != ab.p || ctx->cross_gen >= 0x7ffffff0) {
+    if ((e = vr_fill_int(ab.as<int>(), 0, 1, ctx->stream)) != hipSuccess) return vr_hip_fail(ctx, e, "crossings");
+    ctx->cross_any = ab.p;
+    ctx->cross_gen = 0;
+  }
+  const int gn = ++ctx->cross_gen;
   {
     TimedLaunch tl(ctx, TC_CROSSINGS, (double)(VR_LEAF_BYTES + 64), 0.0, n);
-    e = vr_launch_crossings(make_view(g), g->topo->cap, cb.as<uint64_t>(), ab.as<int>(), ctx->stream);
+    e = vr_launch_crossings(make_view(g), g->topo->cap, cb.as<uint64_t>(), ab.as<int>(), gn, ctx->stream);
   }
   if (e != hipSuccess) return vr_hip_fail(ctx, e, "crossings");
   *cross = cb.as<uint64_t>();
   *any = ab.as<int>();
+  *gen = gn;
   return VR_OK;
 }
 
@@ -641,7 +649,8 @@ extern "C" int vr_zero_crossings(vr_ctx* ctx, const vr_grid* g, int32_t* xyz, si
   if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
   uint64_t* cross;
   int* any;
-  int rc = crossings_into(ctx, g, &cross, &any);
+  int gen = 0;
+  int rc = crossings_into(ctx, g, &cross, &any, &gen);
   if (rc != VR_OK) return rc;
   const int n = g->topo->count;
   const size_t nw = (size_t)n * 8;
@@ -699,7 +708,8 @@ extern "C" int vr_trim_band(vr_ctx* ctx, vr_grid* g, float half_width) {
 
 // Crossings, bounded-radius redistance into `tmp` (>= count leaves), then
 // compaction back into g.
-int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact, int* count_copy) {
+int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact, int* count_copy,
+                 unsigned long long* active_out) {
   const double rr = std::ceil((double)half_width);
   if (!(rr <= 8.0)) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "rebuild supports ceil(half_width) <= 8");
   const int R = rr < 0.0 ? 0 : (int)rr;
@@ -709,24 +719,27 @@ int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool 
   const float band = half_width * vs;
   uint64_t* cross;
   int* any;
-  int rc = crossings_into(ctx, g, &cross, &any);
+  int gen = 0;
+  int rc = crossings_into(ctx, g, &cross, &any, &gen);
   if (rc != VR_OK) return rc;
   DevBuf& keep = ctx->scratch[SCR_KEEP];
   if ((e = keep.ensure(sizeof(int) * (size_t)(n + 1))) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
   {
     TimedLaunch tl(ctx, TC_REBUILD, (double)(2 * VR_LEAF_BYTES + 64), 0.0, n);
-    e = vr_launch_rebuild(make_view(g), g->topo->cap, cross, any, tmp, keep.as<int>(), n, R, vs, band, ctx->stream);
+    e = vr_launch_rebuild(make_view(g), g->topo->cap, cross, any, gen, tmp, keep.as<int>(), n, R, vs, band,
+                          active_out, ctx->stream);
   }
   if (e != hipSuccess) return vr_hip_fail(ctx, e, "rebuild");
   // Leaves without active voxels survive only when the grid has no crossing
   // at all (every leaf is then kept as it was); the flag is read back only
   // when the grid can hold such leaves.
-  int anyh = 1;
+  int anyh = gen;
   if (g->topo->has_empty &&
       ((e = hipMemcpyAsync(&anyh, any, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess))
     return vr_hip_fail(ctx, e, "readback");
-  return compact_into(ctx, g, tmp, keep.as<int>(), anyh ? false : g->topo->has_empty, exact, true, count_copy);
+  return compact_into(ctx, g, tmp, keep.as<int>(), anyh == gen ? false : g->topo->has_empty, exact, true,
+                      count_copy);
 }
 
 extern "C" int vr_rebuild_band(vr_ctx* ctx, vr_grid* g, float half_width) {

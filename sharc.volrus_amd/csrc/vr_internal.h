@@ -79,6 +79,8 @@ struct vr_ctx {
   bool timing = false;
   std::vector<TimingRec> pending;
   std::vector<hipEvent_t> event_pool;
+  int cross_gen = 0;   // generation of the last crossings pass (its "any crossing" mark)
+  const void* cross_any = nullptr;  // the mark's buffer that generation counts on
   int roll_seg = -1;   // rollout segment of the launches being recorded (-1: none)
   int roll_mesh = -1;  // rollout mesh index of the launches being recorded
   double t_ms[TC_COUNT] = {0, 0, 0, 0, 0, 0};
@@ -153,7 +155,8 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
                  bool exact = true, bool tail_clear = false, int* count_copy = nullptr);
 int advect_into(vr_ctx* ctx, const vr_grid* in, const vr_velocity* vel, double dt, double t, int scheme,
                 vr_grid* o);
-int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any);
+// *any == *gen after the pass when some voxel is a crossing.
+int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any, int* gen);
 // deferred: no host synchronisation; the leaf count is the host upper bound,
 // the mesh is written up to its capacity and its counts are copied to
 // d_counts[0..1] (vertices, triangles) instead of being read back.
@@ -164,8 +167,9 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
                    int tcap, float iso, int* counts_out = nullptr);
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, float iso, double* verts, uint32_t* tris,
                   double algorithmic_bytes, long long vcap = -1, long long tricap = -1);
+// active_out (optional): receives += the active voxels after the rebuild.
 int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact = true,
-                 int* count_copy = nullptr);
+                 int* count_copy = nullptr, unsigned long long* active_out = nullptr);
 // zero = false: *slot is already 0.
 int count_active_into(vr_ctx* ctx, const vr_grid* g, unsigned long long* slot, bool zero = true);
 void copy_header_into(vr_grid* dst, const vr_grid* src);

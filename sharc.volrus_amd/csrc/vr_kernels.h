@@ -21,9 +21,10 @@ hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* ou
 // one (fewest instructions per leaf) once there are enough leaves to give
 // every CU 16 waves, else four (a leaf finishes sooner).
 inline int vr_advect_waves(long long leaves, int cus) { return leaves >= 16LL * (cus > 0 ? cus : 256) ? 1 : 4; }
-hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, hipStream_t s);
-hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, NanoLeaf* out,
-                             int* keep, int keep_len, int R, float vs, float band, hipStream_t s);
+hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, int gen, hipStream_t s);
+hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, int gen, NanoLeaf* out,
+                             int* keep, int keep_len, int R, float vs, float band, unsigned long long* active_out,
+                             hipStream_t s);
 hipError_t vr_launch_trim(const GridView& g, int cap, NanoLeaf* out, int* keep, float threshold, hipStream_t s);
 hipError_t vr_launch_nonempty(const NanoLeaf* leaves, const int* count, int cap, int* keep, hipStream_t s);
 hipError_t vr_launch_compact(const NanoLeaf* src, const int* count, int cap, const int* keep, const int* pos,

@@ -195,7 +195,8 @@ hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* ou
 // six face layers of its neighbours are staged in LDS with inactive or
 // absent voxels as +0: a product with +0 is never < 0, which is the same
 // test as skipping them.
-__global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, int* any) {
+// *any is raised to gen when the grid has a crossing.
+__global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, int* any, int gen) {
   __shared__ int nb[VR_NBR];
   __shared__ float brick[1000];  // (x + 1) * 100 + (y + 1) * 10 + (z + 1), x, y, z in -1..8
   const int lane = threadIdx.x;
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
       if (lane == 0) cross[(size_t)li * 8 + x] = word;
       found |= word != 0;
     }
-    if (found && lane == 0) atomicOr(any, 1);
+    if (found && lane == 0) atomicMax(any, gen);
     __syncthreads();
   }
 }
@@ -263,8 +264,10 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
 // No crossings anywhere: the grid is copied unchanged (the reference returns
 // early, :85-87).
 // keep[*count .. keep_len) is cleared (block 0) for a scan of keep_len flags.
-__global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cross, const int* any, NanoLeaf* out,
-                                                int* keep, int keep_len, int R, float vs, float band) {
+// active_out (optional) gets += the active voxels of the output.
+__global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cross, const int* any, int gen,
+                                                NanoLeaf* out, int* keep, int keep_len, int R, float vs, float band,
+                                                unsigned long long* active_out) {
   __shared__ uint32_t rows[24 * 24];
   __shared__ uint8_t dz2[24 * 24 * 8];  // row (X, Y), z: dz^2 to the nearest bit, 255 = none within R
   __shared__ uint8_t dyz2[24 * 8 * 8];  // X, y, z: min over dy of dy^2 + dz2, 255 = none
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
   const int n = *g.count;
   if (blockIdx.x == 0)
     for (int i = n + lane; i < keep_len; i += 64) keep[i] = 0;
-  const bool have = *any != 0;
+  const bool have = *any == gen;
   const int y = lane >> 3, z = lane & 7;
   const int lo = 8 - R, span = 8 + 2 * R;  // rows X, Y in [lo, lo + span)
   int li_end, li_step;
@@ -285,7 +288,14 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
     if (!have) {
       if (lane < 8) O.mask[lane] = L.mask[lane];
       for (int x = 0; x < 8; ++x) O.values[x * 64 + lane] = L.values[x * 64 + lane];
-      if (lane == 0) keep[li] = 1;
+      if (lane == 0) {
+        keep[li] = 1;
+        if (active_out) {
+          unsigned long long c = 0;
+          for (int w = 0; w < 8; ++w) c += (unsigned long long)__popcll(L.mask[w]);
+          atomicAdd(active_out, c);
+        }
+      }
       continue;
     }
     if (lane < VR_NBR) nb[lane] = g.nbr[(size_t)li * VR_NBR + lane];
@@ -332,6 +342,7 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
     }
     __syncthreads();
     bool any_left = false;
+    unsigned long long n_active = 0;
     for (int x = 0; x < 8; ++x) {
       const bool act = (L.mask[x] >> lane) & 1ull;
       const float old = L.values[x * 64 + lane];
@@ -357,8 +368,12 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
       const uint64_t word = __ballot(act ? kept : false);
       if (lane == 0) O.mask[x] = word;
       any_left |= word != 0;
+      n_active += __popcll(word);
     }
-    if (lane == 0) keep[li] = any_left ? 1 : 0;
+    if (lane == 0) {
+      keep[li] = any_left ? 1 : 0;
+      if (active_out && n_active) atomicAdd(active_out, n_active);
+    }
     __syncthreads();
   }
 }
@@ -490,14 +505,15 @@ hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, in
   hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, scratch, n);
   return hipGetLastError();
 }
-hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, hipStream_t s) {
-  hipLaunchKernelGGL(k_crossings, dim3(leaf_blocks(cap)), dim3(64), 0, s, g, cross, any);
+hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, int gen, hipStream_t s) {
+  hipLaunchKernelGGL(k_crossings, dim3(leaf_blocks(cap)), dim3(64), 0, s, g, cross, any, gen);
   return hipGetLastError();
 }
-hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, NanoLeaf* out,
-                             int* keep, int keep_len, int R, float vs, float band, hipStream_t s) {
-  hipLaunchKernelGGL(k_rebuild, dim3(leaf_blocks(cap)), dim3(64), 0, s, g, cross, any, out, keep, keep_len, R, vs,
-                     band);
+hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, int gen, NanoLeaf* out,
+                             int* keep, int keep_len, int R, float vs, float band, unsigned long long* active_out,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_rebuild, dim3(leaf_blocks(cap)), dim3(64), 0, s, g, cross, any, gen, out, keep, keep_len, R,
+                     vs, band, active_out);
   return hipGetLastError();
 }
 hipError_t vr_launch_trim(const GridView& g, int cap, NanoLeaf* out, int* keep, float threshold, hipStream_t s) {

@@ -269,14 +269,14 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
     }
     if (rebuild_every > 0 && (s + 1) % rebuild_every == 0) {
       const bool next = seg + 1 < nseg;
-      rc = rebuild_impl(ctx, g, half_width, other, false, next ? seg_leaves.as<int>() + seg + 1 : nullptr);
+      // the rebuild writes the next segment's leaf and active counts
+      rc = rebuild_impl(ctx, g, half_width, other, false, next ? seg_leaves.as<int>() + seg + 1 : nullptr,
+                        next ? slots.as<unsigned long long>() + seg + 1 : nullptr);
       if (rc != VR_OK) break;
       st.rebuilds += 1;
       if (next) {
         ++seg;
         ctx->roll_seg = seg;
-        rc = count_active_into(ctx, g, slots.as<unsigned long long>() + seg, false);
-        if (rc != VR_OK) break;
       }
     }
     if (mesh_every > 0 && (s + 1) % mesh_every == 0 && imesh + 1 < nmesh) {
