@@ -254,8 +254,8 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
     // lookup rebuild below.
     const size_t R = t->roots.size();
     if ((!exact && !tail_clear && (e = vr_zero_tail(const_cast<int*>(keep), t->d_count, n, ctx->stream)) != hipSuccess) ||
-        (e = vr_exclusive_scan(keep, pos.as<int>(), (size_t)n, sums.as<int>(), tot.as<int>(), ctx->stream)) !=
-            hipSuccess ||
+        (e = vr_exclusive_scan(keep, pos.as<int>(), (size_t)n, sums.as<int>(), tot.as<int>(), ctx->stream,
+                               t->d_count)) != hipSuccess ||
         (e = vr_launch_compact(src, t->d_count, n, keep, pos.as<int>(), g->d_leaves, ctx->stream, t->d_child,
                                4096 * (R ? R : 1))) != hipSuccess ||
         (e = vr_copy_int(tot.as<int>(), t->d_count, ctx->stream, count_copy)) != hipSuccess)

@@ -12,7 +12,13 @@ hipError_t vr_build_child(const NanoLeaf* leaves, const int* count, int cap, con
 hipError_t vr_build_nbr(const GridView& g, int cap, int* nbr, hipStream_t s);
 
 size_t vr_scan_scratch_ints(size_t n);
-hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, int* total, hipStream_t s);
+// n_dev (optional, single-block sizes only): scan min(n, *n_dev * n_mult)
+// elements, a device-side count below the host bound n.
+hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, int* total, hipStream_t s,
+                             const int* n_dev = nullptr, int n_mult = 1);
+// Two arrays of the same length in one pass when they fit one block.
+hipError_t vr_exclusive_scan2(const int* in0, int* out0, int* total0, const int* in1, int* out1, int* total1,
+                              size_t n, int* scratch, hipStream_t s, const int* n_dev = nullptr);
 
 // nw = waves per leaf (1 or 4), see vr_advect_waves.
 hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* out, int cap, double dt,

@@ -67,38 +67,59 @@ __device__ __forceinline__ int block_excl_scan_256(int v, int* s_wave, int* tota
 
 // Small inputs (n <= VR_SCAN_SINGLE_MAX) in one launch: one block of 1024
 // threads walks the input in tiles of 4096 (4 consecutive elements per
-// thread, so a wave reads 1 KB contiguous), with a running carry.
+// thread) with a running carry; NA = 2 scans a second array of the same
+// length in the same pass.  n_dev (optional): the live length is
+// min(n, *n_dev * n_mult), a device-side count below the host bound n.
 #define VR_SCAN_SINGLE_MAX 32768
-__global__ __launch_bounds__(1024) void k_scan_single(const int* in, int* out, int n, int* total) {
-  __shared__ int s_wave[16];
+template <int NA>
+__global__ __launch_bounds__(1024) void k_scan_single(const int* in0, int* out0, int* total0, const int* in1,
+                                                      int* out1, int* total1, int n, const int* n_dev, int n_mult) {
+  __shared__ int s_wave[NA][16];
   const int lane = lane_id(), wave = threadIdx.x >> 6;
-  int carry = 0;
+  if (n_dev) {
+    const long long live = (long long)*n_dev * n_mult;
+    if (live < n) n = (int)live;
+  }
+  const int* in[2] = {in0, in1};
+  int* out[2] = {out0, out1};
+  int carry[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) carry[a] = 0;
   for (int base = 0; base < n; base += 4096) {
     const int i0 = base + (int)threadIdx.x * 4;
-    int v[4];
+    int v[NA][4], sum[NA], incl[NA];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? in[i0 + k] : 0;
-    const int sum = (v[0] + v[1]) + (v[2] + v[3]);
-    const int incl = wave_incl_scan(sum);
-    if (lane == 63) s_wave[wave] = incl;
+    for (int a = 0; a < NA; ++a) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[a][k] = i0 + k < n ? in[a][i0 + k] : 0;
+      sum[a] = (v[a][0] + v[a][1]) + (v[a][2] + v[a][3]);
+      incl[a] = wave_incl_scan(sum[a]);
+      if (lane == 63) s_wave[a][wave] = incl[a];
+    }
     __syncthreads();
-    int off = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) {
-      const int ws = s_wave[w];
-      if (w < wave) off += ws;
-      tot += ws;
-    }
-    int run = carry + off + incl - sum;
+    for (int a = 0; a < NA; ++a) {
+      int off = 0, tot = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (i0 + k < n) out[i0 + k] = run;
-      run += v[k];
+      for (int w = 0; w < 16; ++w) {
+        const int ws = s_wave[a][w];
+        if (w < wave) off += ws;
+        tot += ws;
+      }
+      int run = carry[a] + off + incl[a] - sum[a];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (i0 + k < n) out[a][i0 + k] = run;
+        run += v[a][k];
+      }
+      carry[a] += tot;
     }
-    carry += tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0 && total) *total = carry;
+  if (threadIdx.x == 0) {
+    if (total0) *total0 = carry[0];
+    if (NA == 2 && total1) *total1 = carry[NA - 1];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_scan_blocks(const int* in, int* out, int* block_sums, size_t n) {
@@ -195,21 +216,23 @@ hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* ou
 // six face layers of its neighbours are staged in LDS with inactive or
 // absent voxels as +0: a product with +0 is never < 0, which is the same
 // test as skipping them.
-// *any is raised to gen when the grid has a crossing.
-__global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, int* any, int gen) {
+// *any is raised to gen when the grid has a crossing.  One leaf per block of
+// four waves (wave w takes x = w and w + 4).
+__global__ __launch_bounds__(256) void k_crossings(GridView g, uint64_t* cross, int* any, int gen) {
   __shared__ int nb[VR_NBR];
   __shared__ float brick[1000];  // (x + 1) * 100 + (y + 1) * 10 + (z + 1), x, y, z in -1..8
-  const int lane = threadIdx.x;
+  __shared__ int s_found;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = *g.count;
   const int y = lane >> 3, z = lane & 7;
   int li_end, li_step;
   for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
-    if (lane < VR_NBR) nb[lane] = g.nbr[(size_t)li * VR_NBR + lane];
-    const float4* own = reinterpret_cast<const float4*>(L.values);
-    for (int q = lane; q < 128; q += 64) {
-      const float4 v = own[q];
-      const int x = q >> 4, yy = (q >> 1) & 7, zz = (q & 1) * 4;
+    if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
+    if (tid == 0) s_found = 0;
+    if (tid < 128) {
+      const float4 v = reinterpret_cast<const float4*>(L.values)[tid];
+      const int x = tid >> 4, yy = (tid >> 1) & 7, zz = (tid & 1) * 4;
       const uint32_t m = (uint32_t)(L.mask[x] >> (yy * 8 + zz)) & 0xFu;
       float* b = brick + (x + 1) * 100 + (yy + 1) * 10 + (zz + 1);
       b[0] = (m & 1u) ? v.x : 0.0f;
@@ -219,7 +242,7 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
     }
     __syncthreads();
     // face layers: f = 0..5 for x = -1, x = 8, y = -1, y = 8, z = -1, z = 8
-    for (int h = lane; h < 384; h += 64) {
+    for (int h = tid; h < 384; h += 256) {
       const int f = h >> 6, u = (h >> 3) & 7, w = h & 7;
       const int side = (f & 1) ? 8 : -1;
       const int bx = f < 2 ? side : u, by = f < 2 ? u : (f < 4 ? side : w), bz = f < 4 ? w : side;
@@ -237,7 +260,7 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
     }
     __syncthreads();
     bool found = false;
-    for (int x = 0; x < 8; ++x) {
+    for (int x = wave; x < 8; x += 4) {
       const bool act = (L.mask[x] >> lane) & 1ull;
       const int c = (x + 1) * 100 + (y + 1) * 10 + (z + 1);
       const float v = brick[c];
@@ -248,8 +271,9 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
       if (lane == 0) cross[(size_t)li * 8 + x] = word;
       found |= word != 0;
     }
-    if (found && lane == 0) atomicMax(any, gen);
+    if (found && lane == 0) s_found = 1;
     __syncthreads();
+    if (tid == 0 && s_found) atomicMax(any, gen);
   }
 }
 
@@ -264,18 +288,21 @@ __global__ __launch_bounds__(64) void k_crossings(GridView g, uint64_t* cross, i
 // No crossings anywhere: the grid is copied unchanged (the reference returns
 // early, :85-87).
 // keep[*count .. keep_len) is cleared (block 0) for a scan of keep_len flags.
-// active_out (optional) gets += the active voxels of the output.
-__global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cross, const int* any, int gen,
+// active_out (optional) gets += the active voxels of the output.  One leaf
+// per block of four waves (wave w takes x = w and w + 4 in the last pass).
+__global__ __launch_bounds__(256) void k_rebuild(GridView g, const uint64_t* cross, const int* any, int gen,
                                                 NanoLeaf* out, int* keep, int keep_len, int R, float vs, float band,
                                                 unsigned long long* active_out) {
   __shared__ uint32_t rows[24 * 24];
   __shared__ uint8_t dz2[24 * 24 * 8];  // row (X, Y), z: dz^2 to the nearest bit, 255 = none within R
   __shared__ uint8_t dyz2[24 * 8 * 8];  // X, y, z: min over dy of dy^2 + dz2, 255 = none
   __shared__ int nb[VR_NBR];
-  const int lane = threadIdx.x;
+  __shared__ int s_left[4];
+  __shared__ unsigned long long s_cnt[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = *g.count;
   if (blockIdx.x == 0)
-    for (int i = n + lane; i < keep_len; i += 64) keep[i] = 0;
+    for (int i = n + tid; i < keep_len; i += 256) keep[i] = 0;
   const bool have = *any == gen;
   const int y = lane >> 3, z = lane & 7;
   const int lo = 8 - R, span = 8 + 2 * R;  // rows X, Y in [lo, lo + span)
@@ -283,12 +310,12 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
   for (int li = xcd_first(n, &li_end, &li_step); li < li_end; li += li_step) {
     const NanoLeaf& L = g.leaves[li];
     NanoLeaf& O = out[li];
-    if (lane < 3) O.origin[lane] = L.origin[lane];
-    if (lane == 3) O.pad = 0;
+    if (tid < 3) O.origin[tid] = L.origin[tid];
+    if (tid == 3) O.pad = 0;
     if (!have) {
-      if (lane < 8) O.mask[lane] = L.mask[lane];
-      for (int x = 0; x < 8; ++x) O.values[x * 64 + lane] = L.values[x * 64 + lane];
-      if (lane == 0) {
+      if (tid < 8) O.mask[tid] = L.mask[tid];
+      for (int i = tid; i < 512; i += 256) O.values[i] = L.values[i];
+      if (tid == 0) {
         keep[li] = 1;
         if (active_out) {
           unsigned long long c = 0;
@@ -298,9 +325,9 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
       }
       continue;
     }
-    if (lane < VR_NBR) nb[lane] = g.nbr[(size_t)li * VR_NBR + lane];
+    if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
     __syncthreads();
-    for (int r = lane; r < 576; r += 64) {
+    for (int r = tid; r < 576; r += 256) {
       const int X = r / 24, Y = r % 24;
       const int nx = X >> 3, ny = Y >> 3;
       uint32_t bits = 0;
@@ -313,7 +340,7 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
     }
     __syncthreads();
     const uint32_t win = (2u << (2 * R)) - 1u;  // 2R+1 bits
-    for (int i = lane; i < span * span * 8; i += 64) {
+    for (int i = tid; i < span * span * 8; i += 256) {
       const int zz = i & 7, rc = i >> 3, X = lo + rc / span, Y = lo + rc % span;
       const uint32_t b = (rows[X * 24 + Y] >> (8 + zz - R)) & win;
       int d = 255;
@@ -331,7 +358,7 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
       dz2[(X * 24 + Y) * 8 + zz] = (uint8_t)d;
     }
     __syncthreads();
-    for (int i = lane; i < span * 64; i += 64) {
+    for (int i = tid; i < span * 64; i += 256) {
       const int X = lo + (i >> 6), yy = (i >> 3) & 7, zz = i & 7;
       int m = 255;
       for (int dy = -R; dy <= R; ++dy) {
@@ -343,7 +370,7 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
     __syncthreads();
     bool any_left = false;
     unsigned long long n_active = 0;
-    for (int x = 0; x < 8; ++x) {
+    for (int x = wave; x < 8; x += 4) {
       const bool act = (L.mask[x] >> lane) & 1ull;
       const float old = L.values[x * 64 + lane];
       bool kept = false;
@@ -371,8 +398,14 @@ __global__ __launch_bounds__(64) void k_rebuild(GridView g, const uint64_t* cros
       n_active += __popcll(word);
     }
     if (lane == 0) {
-      keep[li] = any_left ? 1 : 0;
-      if (active_out && n_active) atomicAdd(active_out, n_active);
+      s_left[wave] = any_left ? 1 : 0;
+      s_cnt[wave] = n_active;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      keep[li] = (s_left[0] | s_left[1] | s_left[2] | s_left[3]) ? 1 : 0;
+      const unsigned long long c = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+      if (active_out && c) atomicAdd(active_out, c);
     }
     __syncthreads();
   }
@@ -493,11 +526,13 @@ hipError_t vr_build_nbr(const GridView& g, int cap, int* nbr, hipStream_t s) {
   return hipGetLastError();
 }
 size_t vr_scan_scratch_ints(size_t n) { return (n + 1023) / 1024 + 1; }
-hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, int* total, hipStream_t s) {
+hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, int* total, hipStream_t s,
+                             const int* n_dev, int n_mult) {
   const int nb = (int)((n + 1023) / 1024);
   if (nb == 0) return total ? vr_fill_int(total, 0, 1, s) : hipSuccess;
   if (n <= VR_SCAN_SINGLE_MAX) {
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, in, out, (int)n, total);
+    hipLaunchKernelGGL(k_scan_single<1>, dim3(1), dim3(1024), 0, s, in, out, total, (const int*)nullptr,
+                       (int*)nullptr, (int*)nullptr, (int)n, n_dev, n_mult);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, in, out, scratch, n);
@@ -505,14 +540,24 @@ hipError_t vr_exclusive_scan(const int* in, int* out, size_t n, int* scratch, in
   hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, scratch, n);
   return hipGetLastError();
 }
+hipError_t vr_exclusive_scan2(const int* in0, int* out0, int* total0, const int* in1, int* out1, int* total1,
+                              size_t n, int* scratch, hipStream_t s, const int* n_dev) {
+  if (n > 0 && n <= VR_SCAN_SINGLE_MAX) {
+    hipLaunchKernelGGL(k_scan_single<2>, dim3(1), dim3(1024), 0, s, in0, out0, total0, in1, out1, total1, (int)n,
+                       n_dev, 1);
+    return hipGetLastError();
+  }
+  hipError_t e = vr_exclusive_scan(in0, out0, n, scratch, total0, s);
+  return e != hipSuccess ? e : vr_exclusive_scan(in1, out1, n, scratch, total1, s);
+}
 hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, int gen, hipStream_t s) {
-  hipLaunchKernelGGL(k_crossings, dim3(leaf_blocks(cap)), dim3(64), 0, s, g, cross, any, gen);
+  hipLaunchKernelGGL(k_crossings, dim3(leaf_blocks(cap)), dim3(256), 0, s, g, cross, any, gen);
   return hipGetLastError();
 }
 hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, int gen, NanoLeaf* out,
                              int* keep, int keep_len, int R, float vs, float band, unsigned long long* active_out,
                              hipStream_t s) {
-  hipLaunchKernelGGL(k_rebuild, dim3(leaf_blocks(cap)), dim3(64), 0, s, g, cross, any, gen, out, keep, keep_len, R,
+  hipLaunchKernelGGL(k_rebuild, dim3(leaf_blocks(cap)), dim3(256), 0, s, g, cross, any, gen, out, keep, keep_len, R,
                      vs, band, active_out);
   return hipGetLastError();
 }

@@ -715,7 +715,7 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
                      s.keep.as<int>(), (long long)n7);
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = vr_exclusive_scan(s.keep.as<int>(), s.pos.as<int>(), n7, s.sums.as<int>(), s.totals.as<int>(),
-                             ctx->stream)) != hipSuccess)
+                             ctx->stream, d_leaf_base + G, 7)) != hipSuccess)
     return vr_hip_fail(ctx, e, "mc tiles");
   hipLaunchKernelGGL(k_mc_tile_base, dim3(blocks_for(G + 1, 256)), dim3(256), 0, ctx->stream, d_leaf_base, G,
                      s.pos.as<int>(), s.totals.as<int>(), s.tile_base.as<int>());
@@ -749,10 +749,8 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
                        s.tt.as<int>(), tcap);
   }
   if ((e = hipGetLastError()) != hipSuccess ||
-      (e = vr_exclusive_scan(s.tv.as<int>(), s.vb.as<int>(), (size_t)tcap, s.sums2.as<int>(), totals + 1,
-                             ctx->stream)) != hipSuccess ||
-      (e = vr_exclusive_scan(s.tt.as<int>(), s.tb.as<int>(), (size_t)tcap, s.sums2.as<int>(), totals + 2,
-                             ctx->stream)) != hipSuccess)
+      (e = vr_exclusive_scan2(s.tv.as<int>(), s.vb.as<int>(), totals + 1, s.tt.as<int>(), s.tb.as<int>(),
+                              totals + 2, (size_t)tcap, s.sums2.as<int>(), ctx->stream, totals)) != hipSuccess)
     return vr_hip_fail(ctx, e, "mc count");
   hipLaunchKernelGGL(k_mc_ranges, dim3(blocks_for(G, 256)), dim3(256), 0, ctx->stream, s.tile_base.as<int>(), G, totals,
                      s.vb.as<int>(), s.tb.as<int>(), totals, s.ranges.as<int>(), counts_out);
