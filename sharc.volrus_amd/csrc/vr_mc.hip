@@ -186,10 +186,26 @@ __global__ void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, 
     const GridView& v = views[g];
     const int4 o = v.origins[i];
     const int ex = o.x + c_dir7[k][0], ey = o.y + c_dir7[k][1], ez = o.z + c_dir7[k][2];
+    // The 7 lower bounds in lockstep (the halving sequence depends on n
+    // only), so each step issues 7 independent loads.
+    int b[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) b[j] = 0;
+    for (int len = n; len > 1;) {
+      const int half = len >> 1;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const int4 r = v.origins[b[j] + half];
+        if (coord_less(r.x, r.y, r.z, ex - c_dir7[j][0], ey - c_dir7[j][1], ez - c_dir7[j][2])) b[j] += half;
+      }
+      len -= half;
+    }
     long long rank = pos[seg];
+#pragma unroll
     for (int j = 0; j < 7; ++j) {
+      const int4 r = v.origins[b[j]];
+      const int lb = b[j] + (coord_less(r.x, r.y, r.z, ex - c_dir7[j][0], ey - c_dir7[j][1], ez - c_dir7[j][2]) ? 1 : 0);
       const long long lj = seg + (long long)j * n;
-      const int lb = lower_bound_origin(v.origins, 0, n, ex - c_dir7[j][0], ey - c_dir7[j][1], ez - c_dir7[j][2]);
       rank += (lb < n ? pos[lj + lb] : pos[lj + n - 1] + keep[lj + n - 1]) - pos[lj];
     }
     if (rank < nt) tiles[rank] = make_int4(ex, ey, ez, g);
