@@ -140,7 +140,11 @@ int vr_mesh_create(vr_ctx* ctx, vr_mesh** out);
  * `steps` advection steps from time t0 (step s starts at t0 + s * dt), calls
  * rebuild_narrow_band(half_width) after every `rebuild_every`-th step and
  * volume_to_mesh(iso) after every `mesh_every`-th step (0 disables either),
- * in place on g.  The last mesh is left in `mesh` (may be NULL). */
+ * in place on g.  The last mesh is left in `mesh` (may be NULL).  The call
+ * synchronises with the host once, at the end; meshes on the way are written
+ * into the mesh's buffers up to their capacity, and when a mesh is larger
+ * the buffers grow and the final state is meshed again, so the returned mesh
+ * is always whole (pass the same mesh to later calls to keep the capacity). */
 typedef struct vr_rollout_stats {
   uint64_t voxel_updates; /* sum over steps of the active voxels advected */
   uint64_t advect_steps;
