@@ -73,6 +73,28 @@ def test_batch_graph_replays_are_stable(orc, ctx, vr):
             assert np.array_equal(t, ref[i][1][1]) and np.array_equal(v, ref[i][1][0]), f"replay {rep} mesh {i}"
 
 
+def test_batch_graph_new_parameters_resize(orc, ctx, vr):
+    """Graph steps whose new dt or iso gives a larger mesh than the first
+    capture: the batch sizes its mesh again for each parameter set."""
+    announce("batch graph new parameters")
+    rng = np.random.default_rng(13)
+    vs = 1.0 / 128
+    spec = scenes(vr, rng, 3, vs)
+    grids = [ctx.from_primitives(p, vs, 3.0) for p, _ in spec]
+    vels = [ctx.velocity_analytic(vr.VEL_UNIFORM, v) for _, v in spec]
+    b = vr.Batch(ctx, grids, vels)
+    for dt, iso in [(0.3 * vs, 0.0), (0.3 * vs, 2.0 * vs), (0.6 * vs, 2.0 * vs), (0.3 * vs, 0.0)]:
+        b.step(dt, 0.0, vr.RK3, iso, vr.BATCH_GRAPH)
+        ctx.synchronize()
+        for i, (_, vel) in enumerate(spec):
+            og = orc.from_nano(grids[i].download())
+            want = orc.advect_analytic(og, vr.VEL_UNIFORM, vel, dt, 3)
+            vw, tw, _ = orc.mesh(want, iso)
+            vg, tg = b.mesh(i, vr.BATCH_GRAPH)
+            assert tg.shape == tw.shape and np.array_equal(tg, tw), f"dt {dt} iso {iso} grid {i} triangles"
+            assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"dt {dt} iso {iso} grid {i} vertices"
+
+
 @pytest.mark.parametrize("kinds,tuning", [
     ("mixed", (1, 0)),      # per-leaf velocity switch, one block per chunk
     ("mixed", (3, 8)),      # persistent, chunk size not dividing the grids
