@@ -10,12 +10,27 @@
 // 10^3) when they all lie in it; otherwise each goes through the
 // root/internal lookup (any dt), one corner at a time to keep that rare path
 // small in registers.
+// vtab (used when tab: single-vortex field on a diagonal transform): per
+// local x and y, sin and cos of pi times the voxel row's world coordinate,
+// so the velocity at the voxel itself (the first stage) takes no sincospi.
 template <int VK, int SCHEME>
 __device__ __forceinline__ float advect_voxel(const GridView& in, const VelView& vel, const float* brick, int ox,
-                                              int oy, int oz, int x, int y, int z, double dt, const double tf[3]) {
+                                              int oy, int oz, int x, int y, int z, double dt, const double tf[3],
+                                              bool tab, const double* vtab) {
   double w[3], p[3];
   view_index_to_world(in, ox + x, oy + y, oz + z, w);
-  departure<VK, SCHEME>(vel, w, dt, tf, p);
+  if constexpr (VK == VK_VORTEX) {
+    if (tab) {
+      // vel_sample<VK_VORTEX> at w from the row tables: the same values
+      const double sx = vtab[x], cx = vtab[8 + x], sy = vtab[16 + y], cy = vtab[24 + y];
+      const double u0[3] = {(sx * sx) * (2.0 * sy * cy), -((sy * sy) * (2.0 * sx * cx)), 0.0};
+      departure_from<VK, SCHEME>(vel, w, u0, dt, tf, p);
+    } else {
+      departure<VK, SCHEME>(vel, w, dt, tf, p);
+    }
+  } else {
+    departure<VK, SCHEME>(vel, w, dt, tf, p);
+  }
   const double ix = p[0] * in.inv_vs, iy = p[1] * in.inv_vs, iz = p[2] * in.inv_vs;
   const double flx = floor(ix), fly = floor(iy), flz = floor(iz);
   const float fx = (float)(ix - flx), fy = (float)(iy - fly), fz = (float)(iz - flz);
@@ -94,6 +109,7 @@ struct AdvectLDS {
   float brick[1000];
   uint16_t list[512];
   int nb[VR_NBR];
+  double vtab[32];  // single-vortex row tables: sin x, cos x, sin y, cos y (8 each)
 };
 
 // Advects leaf li of `in` into out[li] with NW waves (the whole block,
@@ -128,14 +144,25 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
     }
     total += __popcll(m);
   }
+  const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
+  const bool tab = VK == VK_VORTEX && in.diag;
+  if (tab && tid < 16) {
+    // row coordinates as view_index_to_world's diagonal form computes them
+    const int a = tid >> 3, l = tid & 7;
+    const double wc = a == 0 ? in.mat[0] * (double)(ox + l) + in.mat[3] : in.mat[5] * (double)(oy + l) + in.mat[7];
+    double sn, cs;
+    sincospi_det(wc, &sn, &cs);
+    s.vtab[16 * a + l] = sn;
+    s.vtab[16 * a + 8 + l] = cs;
+  }
   stage_interior<NT>(in, li, s.brick, tid);
   __syncthreads();
   stage_halo<NT>(in, s.nb, s.brick, tid);
   __syncthreads();
-  const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
   for (int i = tid; i < total; i += NT) {
     const int off = s.list[i];
-    O.values[off] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt, tf);
+    O.values[off] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt,
+                                             tf, tab, s.vtab);
   }
   __syncthreads();
 }

@@ -254,10 +254,9 @@ __device__ __forceinline__ void vel_sample(const VelView& v, const double p[3], 
 // x - dt v(x), src/tools/advection.rs:40-48; RK3: SSP Shu-Osher on the
 // backward characteristic).  tf = stage time factors.
 template <int VK, int SCHEME>
-__device__ __forceinline__ void departure(const VelView& v, const double x[3], double dt, const double tf[3],
-                                         double out[3]) {
-  double u[3];
-  vel_sample<VK>(v, x, u);
+__device__ __forceinline__ void departure_from(const VelView& v, const double x[3], const double u0[3], double dt,
+                                              const double tf[3], double out[3]) {
+  double u[3] = {u0[0], u0[1], u0[2]};
   if constexpr (SCHEME == 1) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a] = x[a] - dt * (u[a] * tf[0]);
@@ -272,6 +271,15 @@ __device__ __forceinline__ void departure(const VelView& v, const double x[3], d
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a] = (1.0 / 3.0) * x[a] + (2.0 / 3.0) * (y2[a] - dt * (u[a] * tf[2]));
   }
+}
+
+// The same, with the velocity at x sampled here (u0 = v(x)).
+template <int VK, int SCHEME>
+__device__ __forceinline__ void departure(const VelView& v, const double x[3], double dt, const double tf[3],
+                                         double out[3]) {
+  double u0[3];
+  vel_sample<VK>(v, x, u0);
+  departure_from<VK, SCHEME>(v, x, u0, dt, tf, out);
 }
 
 // ------------------------------------------------------------------ XCD-aware item order
