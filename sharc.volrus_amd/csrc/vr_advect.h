@@ -118,7 +118,9 @@ struct AdvectLDS {
 // (one pass for a band leaf with 4 waves); each result is stored straight
 // into the output leaf, whose inactive positions get the background.
 // Output origin and mask are the input's (the topology does not change,
-// src/tools/advection.rs:34-53).
+// src/tools/advection.rs:34-53).  The background, mask and origin stores
+// follow the results, so each output cache line is written within a short
+// window and leaves the L2 once.
 template <int VK, int SCHEME, int NW>
 __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
                                             AdvectLDS& s, int tid, double dt, const double tf[3]) {
@@ -129,19 +131,11 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   NanoLeaf& O = out[li];
   const float bg = in.background;
   if (tid < VR_NBR) s.nb[tid] = in.nbr[(size_t)li * VR_NBR + tid];
-  if (tid < 3) O.origin[tid] = L.origin[tid];
-  if (tid == 3) O.pad = 0;
   int total = 0;
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
     const uint64_t m = L.mask[x];
-    if ((x % NW) == wave) {
-      if (lane == 0) O.mask[x] = m;
-      if ((m >> lane) & 1ull)
-        s.list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
-      else
-        O.values[x * 64 + lane] = bg;
-    }
+    if ((x % NW) == wave && ((m >> lane) & 1ull)) s.list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
     total += __popcll(m);
   }
   const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
@@ -164,5 +158,15 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
     O.values[off] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt,
                                              tf, tab, s.vtab);
   }
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    if ((x % NW) == wave) {
+      const uint64_t m = L.mask[x];
+      if (lane == 0) O.mask[x] = m;
+      if (!((m >> lane) & 1ull)) O.values[x * 64 + lane] = bg;
+    }
+  }
+  if (tid < 3) O.origin[tid] = L.origin[tid];
+  if (tid == 3) O.pad = 0;
   __syncthreads();
 }

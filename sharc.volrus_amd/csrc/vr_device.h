@@ -98,11 +98,11 @@ __device__ __forceinline__ bool grid_is_active(const GridView& g, int x, int y, 
 }
 
 // Rust `f64 as i32`: saturating, NaN -> 0.
+// Branch-free: the clamp keeps the conversion in range (a NaN clamps to
+// -2^31 and is then replaced by 0).
 __device__ __forceinline__ int f64_to_i32(double v) {
-  if (v != v) return 0;
-  if (v >= 2147483647.0) return 2147483647;
-  if (v <= -2147483648.0) return (-2147483647 - 1);
-  return (int)v;
+  const int t = (int)fmin(fmax(v, -2147483648.0), 2147483647.0);
+  return v != v ? 0 : t;
 }
 
 // The 7-lerp blend of src/tools/interpolation.rs:39-47, in f32, in that order.
