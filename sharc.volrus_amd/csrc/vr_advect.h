@@ -109,6 +109,7 @@ struct AdvectLDS {
   float brick[1000];
   uint16_t list[512];
   int nb[VR_NBR];
+  uint64_t act[8];  // the leaf's active mask
   double vtab[32];  // single-vortex row tables: sin x, cos x, sin y, cos y (8 each)
 };
 
@@ -118,9 +119,9 @@ struct AdvectLDS {
 // (one pass for a band leaf with 4 waves); each result is stored straight
 // into the output leaf, whose inactive positions get the background.
 // Output origin and mask are the input's (the topology does not change,
-// src/tools/advection.rs:34-53).  The background, mask and origin stores
-// follow the results, so each output cache line is written within a short
-// window and leaves the L2 once.
+// src/tools/advection.rs:34-53).  The background stores follow the
+// results, so each output cache line is written within a short window and
+// leaves the L2 once.
 template <int VK, int SCHEME, int NW>
 __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
                                             AdvectLDS& s, int tid, double dt, const double tf[3]) {
@@ -130,11 +131,24 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   const NanoLeaf& L = in.leaves[li];
   NanoLeaf& O = out[li];
   const float bg = in.background;
-  if (tid < VR_NBR) s.nb[tid] = in.nbr[(size_t)li * VR_NBR + tid];
+  // The independent loads go out first (every lane loads, so no branch
+  // waits on them): the neighbour list entry and the leaf's own values.
+  constexpr int OQ = (128 + NT - 1) / NT;
+  const int nbv = in.nbr[(size_t)li * VR_NBR + (tid < VR_NBR ? tid : 0)];
+  const uint64_t mword = L.mask[tid & 7];  // copied to the output by threads 0..7
+  float4 own[OQ];
+#pragma unroll
+  for (int q = 0; q < OQ; ++q) {
+    const int e = tid + q * NT;
+    own[q] = reinterpret_cast<const float4*>(L.values)[e < 128 ? e : 0];
+  }
+  uint64_t mk[8];  // all eight mask words in one round trip
+#pragma unroll
+  for (int x = 0; x < 8; ++x) mk[x] = L.mask[x];
   int total = 0;
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
-    const uint64_t m = L.mask[x];
+    const uint64_t m = mk[x];
     if ((x % NW) == wave && ((m >> lane) & 1ull)) s.list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
     total += __popcll(m);
   }
@@ -149,8 +163,27 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
     s.vtab[16 * a + l] = sn;
     s.vtab[16 * a + 8 + l] = cs;
   }
-  stage_interior<NT>(in, li, s.brick, tid);
+  if (tid < VR_NBR) s.nb[tid] = nbv;
+#pragma unroll
+  for (int q = 0; q < OQ; ++q) {
+    const int e = tid + q * NT;
+    if (e < 128) {
+      const int x = e >> 4, y = (e >> 1) & 7, z = (e & 1) * 4;
+      float* b = s.brick + (x + 1) * 100 + (y + 1) * 10 + (z + 1);
+      b[0] = own[q].x;
+      b[1] = own[q].y;
+      b[2] = own[q].z;
+      b[3] = own[q].w;
+    }
+  }
   __syncthreads();
+  // Output header: origin and mask are the input's.
+  if (tid < 8) {
+    O.mask[tid] = mword;
+    s.act[tid] = mword;
+  }
+  if (tid < 3) O.origin[tid] = tid == 0 ? ox : (tid == 1 ? oy : oz);
+  if (tid == 3) O.pad = 0;
   stage_halo<NT>(in, s.nb, s.brick, tid);
   __syncthreads();
   for (int i = tid; i < total; i += NT) {
@@ -159,14 +192,7 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
                                              tf, tab, s.vtab);
   }
 #pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    if ((x % NW) == wave) {
-      const uint64_t m = L.mask[x];
-      if (lane == 0) O.mask[x] = m;
-      if (!((m >> lane) & 1ull)) O.values[x * 64 + lane] = bg;
-    }
-  }
-  if (tid < 3) O.origin[tid] = L.origin[tid];
-  if (tid == 3) O.pad = 0;
+  for (int x = 0; x < 8; ++x)
+    if ((x % NW) == wave && !((s.act[x] >> lane) & 1ull)) O.values[x * 64 + lane] = bg;
   __syncthreads();
 }
