@@ -92,16 +92,30 @@ constexpr AdvectHalo make_advect_halo() {
 static __constant__ AdvectHalo c_advect_halo = make_advect_halo();
 
 // Absent neighbours read the background (leaf 0 is read and discarded, so
-// the load does not wait on the branch).
+// the load does not wait on the branch).  Each thread's table entries, then
+// its values, are all loaded before any is used (every thread loads, past
+// the end of the table at entry 0), so the round trips overlap instead of
+// following one another.
 template <int NT>
 __device__ __forceinline__ void stage_halo(const GridView& in, const int* nb, float* brick, int tid) {
+  constexpr int HQ = (488 + NT - 1) / NT;
   const float bg = in.background;
-  for (int h = tid; h < 488; h += NT) {
-    const uint32_t e = c_advect_halo.e[h];
-    const int src = nb[(e >> 10) & 31u];
-    const float v = in.leaves[src < 0 ? 0 : src].values[e >> 15];
-    brick[e & 1023u] = src < 0 ? bg : v;
+  uint32_t e[HQ];
+#pragma unroll
+  for (int q = 0; q < HQ; ++q) {
+    const int h = tid + q * NT;
+    e[q] = c_advect_halo.e[h < 488 ? h : 0];
   }
+  int src[HQ];
+  float v[HQ];
+#pragma unroll
+  for (int q = 0; q < HQ; ++q) {
+    src[q] = nb[(e[q] >> 10) & 31u];
+    v[q] = in.leaves[src[q] < 0 ? 0 : src[q]].values[e[q] >> 15];
+  }
+#pragma unroll
+  for (int q = 0; q < HQ; ++q)
+    if (tid + q * NT < 488) brick[e[q] & 1023u] = src[q] < 0 ? bg : v[q];
 }
 
 // LDS of one advecting block.
