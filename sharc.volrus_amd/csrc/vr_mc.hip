@@ -152,12 +152,25 @@ __global__ void k_mc_keep(const GridView* views, const int* leaf_base, int G, in
       keep[t] = 1;
       continue;
     }
+    // Element = leaf i + D[k]; it and each leaf + D[k] - D[j] (j < k) are
+    // looked up in leaf i's neighbour list, except leaf + 2 D[k] (the
+    // opposite direction on k's axis, j = k - 1 for even k), which goes
+    // through the tree.  All lookups are issued together.
     const GridView& v = views[g];
-    const int4 o = v.origins[i];
-    const int cx = o.x + c_dir7[k][0], cy = o.y + c_dir7[k][1], cz = o.z + c_dir7[k][2];
-    int kp = find_leaf(v, cx, cy, cz) < 0;
-    for (int j = 1; j < k && kp; ++j)
-      if (find_leaf(v, cx - c_dir7[j][0], cy - c_dir7[j][1], cz - c_dir7[j][2]) >= 0) kp = 0;
+    const VR_GLOBAL int* nb = v.nbr + (size_t)i * VR_NBR;
+    const int kx = c_dir7[k][0] >> 3, ky = c_dir7[k][1] >> 3, kz = c_dir7[k][2] >> 3;
+    int far = -1;
+    if ((k & 1) == 0) {
+      const int4 o = v.origins[i];
+      far = find_leaf(v, o.x + 2 * c_dir7[k][0], o.y + 2 * c_dir7[k][1], o.z + 2 * c_dir7[k][2]);
+    }
+    int kp = nb[(kx + 1) * 9 + (ky + 1) * 3 + (kz + 1)] < 0;
+    for (int j = 1; j < k; ++j) {
+      const int dx = kx - (c_dir7[j][0] >> 3), dy = ky - (c_dir7[j][1] >> 3), dz = kz - (c_dir7[j][2] >> 3);
+      const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+      if (near && nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] >= 0) kp = 0;
+    }
+    if (far >= 0) kp = 0;
     keep[t] = kp;
   }
 }
@@ -308,17 +321,40 @@ struct MCTileLDS {
 // corner signs (the reference's corner test, volume_to_mesh.rs:115-123);
 // the two halves sit in adjacent lanes and combine in registers, so lanes
 // h and h ^ 1 must both be active.  Absent leaves read the background.
-__device__ __forceinline__ void mc_stage_half(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, float iso,
-                                              int h, uint32_t* rows) {
+// In two steps, so that a lane's three half rows load together:
+// mc_half_load issues the loads (an absent leaf's slot reads a present
+// leaf or leaf 0, and the value is replaced afterwards), mc_half_bits
+// combines.
+struct MCHalf {
+  float4 a;
+  float c;
+  int la, lb;
+};
+__device__ __forceinline__ MCHalf mc_half_load(const VR_GLOBAL NanoLeaf* leaves, const int* tl, int h) {
   const int r = h >> 1, odd = h & 1;
   const int x = (r * 57) >> 9, y = r - 9 * x;  // r / 9 for r < 81
   const int d = ((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + 1;
   const int row = (x & 7) * 64 + (y & 7) * 8;
-  const int la = tl[d], lb = odd ? tl[d + 1] : -1;
-  float4 a = make_float4(bg, bg, bg, bg);
-  float c = bg;
-  if (la >= 0) a = *reinterpret_cast<const VR_GLOBAL float4*>(&leaves[la].values[row + 4 * odd]);
-  if (lb >= 0) c = leaves[lb].values[row];
+  MCHalf q;
+  q.la = tl[d];
+  q.lb = odd ? tl[d + 1] : -1;
+  q.a = *reinterpret_cast<const VR_GLOBAL float4*>(&leaves[q.la < 0 ? 0 : q.la].values[row + 4 * odd]);
+  q.c = leaves[q.lb >= 0 ? q.lb : (q.la >= 0 ? q.la : 0)].values[row];
+  return q;
+}
+
+// Waits here for the loads of six values at once (an empty asm that uses
+// them all), so loads issued together are not each waited for at their
+// first use.
+template <class A, class B, class C, class D, class E, class F>
+__device__ __forceinline__ void mc_use6(A a, B b, C c, D d, E e, F f) {
+  asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d), "v"(e), "v"(f));
+}
+
+__device__ __forceinline__ void mc_half_bits(const MCHalf& q, float bg, float iso, int h, uint32_t* rows) {
+  const int r = h >> 1, odd = h & 1;
+  const float4 a = q.la >= 0 ? q.a : make_float4(bg, bg, bg, bg);
+  const float c = q.lb >= 0 ? q.c : bg;
   uint32_t bits = (uint32_t)(a.x < iso) | ((uint32_t)(a.y < iso) << 1) | ((uint32_t)(a.z < iso) << 2) |
                   ((uint32_t)(a.w < iso) << 3);
   if (odd) bits = (bits << 4) | ((uint32_t)(c < iso) << 8);
@@ -340,9 +376,12 @@ __device__ __forceinline__ int mc_tile_setup(const VR_GLOBAL NanoLeaf* leaves, f
     w.ttile[lane] = tt;
   }
   wave_lds_sync();
-  mc_stage_half(leaves, bg, w.tleaf, iso, lane, w.rows);
-  mc_stage_half(leaves, bg, w.tleaf, iso, lane + 64, w.rows);
-  if (lane < 162 - 128) mc_stage_half(leaves, bg, w.tleaf, iso, lane + 128, w.rows);
+  const MCHalf q0 = mc_half_load(leaves, w.tleaf, lane), q1 = mc_half_load(leaves, w.tleaf, lane + 64);
+  const MCHalf q2 = mc_half_load(leaves, w.tleaf, lane < 162 - 128 ? lane + 128 : 161);
+  mc_use6(q0.a.w, q1.a.w, q2.a.w, q0.c, q1.c, q2.c);
+  mc_half_bits(q0, bg, iso, lane, w.rows);
+  mc_half_bits(q1, bg, iso, lane + 64, w.rows);
+  if (lane < 162 - 128) mc_half_bits(q2, bg, iso, lane + 128, w.rows);
   if (lane < 27) {
     uint32_t m = 0;
 #pragma unroll
