@@ -184,8 +184,9 @@ __global__ void k_mc_tile_base(const int* leaf_base, int G, const int* pos, cons
   }
 }
 
+// tanchor[rank] = i << 3 | k: the tile is leaf i of its grid plus D[k].
 __global__ void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, int G, const int* keep, const int* pos,
-                                   const int* ntiles, int4* tiles) {
+                                   const int* ntiles, int4* tiles, int* tanchor) {
   const long long total = 7LL * leaf_base[G];
   const int nt = *ntiles;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
@@ -221,25 +222,149 @@ __global__ void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, 
       const long long lj = seg + (long long)j * n;
       rank += (lb < n ? pos[lj + lb] : pos[lj + n - 1] + keep[lj + n - 1]) - pos[lj];
     }
-    if (rank < nt) tiles[rank] = make_int4(ex, ey, ez, g);
+    if (rank < nt) {
+      tiles[rank] = make_int4(ex, ey, ez, g);
+      tanchor[rank] = (i << 3) | k;
+    }
   }
 }
 
-// Per tile: the 27 neighbouring leaves and the 27 neighbouring tiles (-1 when
-// that tile is not processed).
-__global__ void k_mc_tile_nbr(const GridView* views, const int4* tiles, const int* ntiles, const int* tile_base,
-                              int* tleaf, int* ttile) {
+// Per tile, over its 27 neighbour slots d = (dx + 1) * 9 + (dy + 1) * 3 +
+// (dz + 1): the leaves at offsets with every component 0 or +1 (the 9^3
+// corner values of the tile's cells lie in them) and the tiles at d <= 13
+// (the tile itself and those before it in visiting order: every owner of
+// an edge of the tile's cells lies in one; -1 when not processed).  The
+// other slots are not read and hold -1.  A leaf slot within one leaf of
+// the tile's anchor leaf comes from that leaf's neighbour list (all of them
+// for a tile that is a leaf or lies on a leaf's -x, -y or -z side), the
+// rest from the tree.
+//
+// A block takes MC_TN_TILES tiles, one per lane, and its four waves split
+// the slots: wave w searches the tiles at d = w, w + 4, w + 8 with three
+// lower bounds in lockstep over the grid's sorted tiles (neighbouring
+// lanes search neighbouring origins, so the halvings share cache lines),
+// and looks up two of the eight leaf slots; d = 12 (0, 0, -1) can only be
+// the tile just before.  The slots are gathered in LDS and written out
+// tile by tile.
+// One thread per slot: the lower-latency form for small tile lists (no
+// block barriers; each slot's lookup runs on its own).
+__global__ void k_mc_tile_nbr_slots(const GridView* views, const int4* tiles, const int* tanchor, const int* ntiles,
+                                    const int* tile_base, int* tleaf, int* ttile) {
   const int nt = *ntiles;
   const size_t total = (size_t)nt * VR_NBR;
   for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
     const int t = (int)(q / VR_NBR), d = (int)(q % VR_NBR);
+    const int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
+    const bool want_leaf = dx >= 0 && dy >= 0 && dz >= 0, want_tile = d <= 13;
+    if (!want_leaf && !want_tile) {
+      tleaf[q] = -1;
+      ttile[q] = -1;
+      continue;
+    }
     const int4 o = tiles[t];
     const int g = o.w;
-    const int x = o.x + 8 * (d / 9 - 1), y = o.y + 8 * ((d / 3) % 3 - 1), z = o.z + 8 * (d % 3 - 1);
-    tleaf[q] = find_leaf(views[g], x, y, z);
-    const int lo = tile_base[g], hi = tile_base[g + 1];
-    const int lb = lower_bound_origin(tiles, lo, hi, x, y, z);
-    ttile[q] = (lb < hi && tiles[lb].x == x && tiles[lb].y == y && tiles[lb].z == z) ? lb : -1;
+    const int x = o.x + 8 * dx, y = o.y + 8 * dy, z = o.z + 8 * dz;
+    int tl = -1;
+    if (want_leaf) {
+      const GridView& v = views[g];
+      const int a = tanchor[t], k = a & 7;
+      const int ax = dx + (c_dir7[k][0] >> 3), ay = dy + (c_dir7[k][1] >> 3), az = dz + (c_dir7[k][2] >> 3);
+      if (ax <= 1 && ay <= 1 && az <= 1)
+        tl = v.nbr[(size_t)(a >> 3) * VR_NBR + (ax + 1) * 9 + (ay + 1) * 3 + (az + 1)];
+      else
+        tl = find_leaf(v, x, y, z);
+    }
+    tleaf[q] = tl;
+    int tt = -1;
+    if (d == 13) {
+      tt = t;
+    } else if (want_tile) {
+      const int lo = tile_base[g], hi = tile_base[g + 1];
+      const int lb = lower_bound_origin(tiles, lo, hi, x, y, z);
+      tt = (lb < hi && tiles[lb].x == x && tiles[lb].y == y && tiles[lb].z == z) ? lb : -1;
+    }
+    ttile[q] = tt;
+  }
+}
+
+#define MC_TN_TILES 64
+#define MC_TN_BIG 32768  // tile capacity from which the tile-per-lane form is used
+__constant__ int c_tn_leaf[8] = {13, 14, 16, 17, 22, 23, 25, 26};
+__global__ __launch_bounds__(256) void k_mc_tile_nbr(const GridView* views, const int4* tiles, const int* tanchor,
+                                                     const int* ntiles, const int* tile_base, int* tleaf, int* ttile) {
+  __shared__ int s_tl[MC_TN_TILES * VR_NBR], s_tt[MC_TN_TILES * VR_NBR];
+  const int nt = *ntiles;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t0 = blockIdx.x * MC_TN_TILES; t0 < nt; t0 += gridDim.x * MC_TN_TILES) {
+    for (int i = threadIdx.x; i < MC_TN_TILES * VR_NBR; i += 256) {
+      s_tl[i] = -1;
+      s_tt[i] = -1;
+    }
+    __syncthreads();
+    const int t = t0 + lane;
+    if (t < nt) {
+      const int4 o = tiles[t];
+      const int g = o.w;
+      const GridView& v = views[g];
+      int* stl = s_tl + lane * VR_NBR;
+      int* stt = s_tt + lane * VR_NBR;
+      const int a = tanchor[t], k = a & 7;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int d = c_tn_leaf[2 * wave + j];
+        const int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
+        const int ax = dx + (c_dir7[k][0] >> 3), ay = dy + (c_dir7[k][1] >> 3), az = dz + (c_dir7[k][2] >> 3);
+        stl[d] = (ax <= 1 && ay <= 1 && az <= 1)
+                     ? v.nbr[(size_t)(a >> 3) * VR_NBR + (ax + 1) * 9 + (ay + 1) * 3 + (az + 1)]
+                     : find_leaf(v, o.x + 8 * dx, o.y + 8 * dy, o.z + 8 * dz);
+      }
+      const int lo = tile_base[g], n = tile_base[g + 1] - lo;
+      int tx[3], ty[3], tz[3], b[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int d = wave + 4 * j;
+        tx[j] = o.x + 8 * (d / 9 - 1);
+        ty[j] = o.y + 8 * ((d / 3) % 3 - 1);
+        tz[j] = o.z + 8 * (d % 3 - 1);
+        b[j] = 0;
+      }
+      for (int len = n; len > 1;) {
+        const int half = len >> 1;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const int4 r = tiles[lo + b[j] + half];
+          if (coord_less(r.x, r.y, r.z, tx[j], ty[j], tz[j])) b[j] += half;
+        }
+        len -= half;
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int4 r = tiles[lo + b[j]];
+        const int lb = b[j] + (coord_less(r.x, r.y, r.z, tx[j], ty[j], tz[j]) ? 1 : 0);
+        bool hit = false;
+        if (lb < n) {
+          const int4 h = lb == b[j] ? r : tiles[lo + lb];
+          hit = h.x == tx[j] && h.y == ty[j] && h.z == tz[j];
+        }
+        stt[wave + 4 * j] = hit ? lo + lb : -1;
+      }
+      if (wave == 0) {
+        bool prev = false;
+        if (t > lo) {
+          const int4 p = tiles[t - 1];
+          prev = p.x == o.x && p.y == o.y && p.z == o.z - 8;
+        }
+        stt[12] = prev ? t - 1 : -1;
+        stt[13] = t;
+      }
+    }
+    __syncthreads();
+    const int cnt = min(MC_TN_TILES, nt - t0) * VR_NBR;
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+      tleaf[(size_t)t0 * VR_NBR + i] = s_tl[i];
+      ttile[(size_t)t0 * VR_NBR + i] = s_tt[i];
+    }
+    __syncthreads();
   }
 }
 
@@ -783,7 +908,8 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
   hipError_t e;
   const size_t n7 = (size_t)(7 * ltot);
   const size_t tc = (size_t)(tcap + 1);
-  if ((e = s.tiles.ensure(sizeof(int4) * tc)) != hipSuccess || (e = s.tleaf.ensure(sizeof(int) * VR_NBR * tc)) != hipSuccess ||
+  if ((e = s.tiles.ensure(sizeof(int4) * tc)) != hipSuccess || (e = s.tanchor.ensure(sizeof(int) * tc)) != hipSuccess ||
+      (e = s.tleaf.ensure(sizeof(int) * VR_NBR * tc)) != hipSuccess ||
       (e = s.ttile.ensure(sizeof(int) * VR_NBR * tc)) != hipSuccess ||
       (e = s.meta.ensure(sizeof(uint32_t) * MC_FACE * tc)) != hipSuccess || (e = s.tv.ensure(sizeof(int) * tc)) != hipSuccess ||
       (e = s.tt.ensure(sizeof(int) * tc)) != hipSuccess || (e = s.vb.ensure(sizeof(int) * tc)) != hipSuccess ||
@@ -793,9 +919,15 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
     return vr_hip_fail(ctx, e, "scratch");
   int* totals = s.totals.as<int>();
   hipLaunchKernelGGL(k_mc_scatter_tiles, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views, d_leaf_base, G,
-                     s.keep.as<int>(), s.pos.as<int>(), totals, s.tiles.as<int4>());
-  hipLaunchKernelGGL(k_mc_tile_nbr, dim3(blocks_for((size_t)tcap * VR_NBR, 256)), dim3(256), 0, ctx->stream, d_views,
-                     s.tiles.as<int4>(), totals, s.tile_base.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>());
+                     s.keep.as<int>(), s.pos.as<int>(), totals, s.tiles.as<int4>(), s.tanchor.as<int>());
+  if (tcap >= MC_TN_BIG)
+    hipLaunchKernelGGL(k_mc_tile_nbr, dim3(blocks_for((size_t)tcap, MC_TN_TILES)), dim3(256), 0, ctx->stream, d_views,
+                       s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(), s.tleaf.as<int>(),
+                       s.ttile.as<int>());
+  else
+    hipLaunchKernelGGL(k_mc_tile_nbr_slots, dim3(blocks_for((size_t)tcap * VR_NBR, 256)), dim3(256), 0, ctx->stream,
+                       d_views, s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(),
+                       s.tleaf.as<int>(), s.ttile.as<int>());
   if ((e = hipGetLastError()) != hipSuccess) return vr_hip_fail(ctx, e, "mc tiles");
   {
     TimedLaunch tl(ctx, TC_MC, (double)VR_LEAF_BYTES, 0.0, (double)ltot);

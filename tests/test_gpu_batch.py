@@ -130,3 +130,32 @@ def test_batch_velocity_kinds_and_tunings(orc, ctx, vr, kinds, tuning):
             vg, tg = b.mesh(i, mode)
             assert tg.shape == tw.shape and np.array_equal(tg, tw), f"mode {mode} grid {i} triangles"
             assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"mode {mode} grid {i} vertices"
+
+
+def test_batch_mesh_large_tile_list(orc, ctx, vr):
+    """A batch whose tile list (at least its leaf count) takes the
+    tile-per-lane neighbour kernel (32768 tiles and up), while each grid
+    alone (at most 7 x its leaves) takes the per-slot one: every grid's mesh
+    from the batch equals its own mesh, and grid 0's equals the oracle's."""
+    announce("batch mesh large tile list")
+    rng = np.random.default_rng(31)
+    vs, hw = 1.0 / 256, 3.0
+    spec = []
+    for _ in range(40):
+        c = rng.uniform(0.3, 0.7, 3)
+        r = float(rng.uniform(0.15, 0.3))
+        spec.append([(vr.PRIM_SPHERE, vr.CSG_UNION, (float(c[0]), float(c[1]), float(c[2]), r))])
+    grids = [ctx.from_primitives(p, vs, hw) for p in spec]
+    leaves = sum(g.counts()[0] for g in grids)
+    assert leaves >= 32768 and all(7 * g.counts()[0] < 32768 for g in grids), leaves
+    vels = [ctx.velocity_analytic(vr.VEL_UNIFORM, (0.0, 0.0, 0.0)) for _ in grids]
+    b = vr.Batch(ctx, grids, vels)
+    b.step(0.25 * vs, 0.0, vr.RK1, 0.0, vr.BATCH_EAGER)
+    for i in range(len(grids)):
+        vg, tg = b.mesh(i, vr.BATCH_EAGER)
+        vs1, ts1 = ctx.upload(b.output_nano(i)).mesh(0.0)
+        assert np.array_equal(tg, ts1), f"grid {i} triangles"
+        assert np.array_equal(vg.view(np.uint64), vs1.view(np.uint64)), f"grid {i} vertices"
+        if i == 0:
+            vw, tw, _ = orc.mesh(orc.from_nano(b.output_nano(0)), 0.0)
+            assert np.array_equal(tg, tw) and np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), "oracle"
