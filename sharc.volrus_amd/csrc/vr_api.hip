@@ -257,8 +257,7 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
         (e = vr_exclusive_scan(keep, pos.as<int>(), (size_t)n, sums.as<int>(), tot.as<int>(), ctx->stream,
                                t->d_count)) != hipSuccess ||
         (e = vr_launch_compact(src, t->d_count, n, keep, pos.as<int>(), g->d_leaves, ctx->stream, t->d_child,
-                               4096 * (R ? R : 1))) != hipSuccess ||
-        (e = vr_copy_int(tot.as<int>(), t->d_count, ctx->stream, count_copy)) != hipSuccess)
+                               4096 * (R ? R : 1))) != hipSuccess)
       return vr_hip_fail(ctx, e, "compact");
     if (exact) {
       int cnt = 0;
@@ -268,8 +267,10 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
       t->count = cnt;
     }
     t->has_empty = has_empty_after;
-    if ((e = vr_build_child(g->d_leaves, t->d_count, t->cap < 1 ? 1 : t->cap, t->d_roots, (int)R, t->d_child,
-                            t->d_origins, ctx->stream)) != hipSuccess ||
+    // The new count (in tot) reaches the topology's count, and count_copy,
+    // through the lookup rebuild: the compaction above read the old count.
+    if ((e = vr_build_child(g->d_leaves, tot.as<int>(), t->cap < 1 ? 1 : t->cap, t->d_roots, (int)R, t->d_child,
+                            t->d_origins, ctx->stream, t->d_count, count_copy)) != hipSuccess ||
         (e = vr_build_nbr(make_view(g), t->cap < 1 ? 1 : t->cap, t->d_nbr, ctx->stream)) != hipSuccess)
       return vr_hip_fail(ctx, e, "topology");
     return VR_OK;
