@@ -364,7 +364,7 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
       b->tricap = th[2] + 1;
     }
   }
-  return mc_emit_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->tcap, iso, b->d_verts, b->d_tris, 0.0,
+  return mc_emit_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->G, b->tcap, iso, b->d_verts, b->d_tris, 0.0,
                        (long long)b->vcap, (long long)b->tricap);
 }
 

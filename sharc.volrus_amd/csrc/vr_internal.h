@@ -163,11 +163,13 @@ int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any, i
 // d_counts[0..1] (vertices, triangles) instead of being read back.
 int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* s, vr_mesh* m, bool deferred = false,
            int* d_counts = nullptr);
-int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot);
+int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
+                   const GridView* single = nullptr);
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
-                   int tcap, float iso, int* counts_out = nullptr);
-int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, float iso, double* verts, uint32_t* tris,
-                  double algorithmic_bytes, long long vcap = -1, long long tricap = -1);
+                   int tcap, float iso);
+int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int tcap, float iso, double* verts,
+                  uint32_t* tris, double algorithmic_bytes, long long vcap = -1, long long tricap = -1,
+                  int* counts_out = nullptr);
 // active_out (optional): receives += the active voxels after the rebuild.
 int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact = true,
                  int* count_copy = nullptr, unsigned long long* active_out = nullptr);

@@ -137,16 +137,27 @@ __device__ __forceinline__ int segment_of(const int* base, int G, long long i) {
 // ------------------------------------------------------------------ tile set
 // Entries [7 * leaf_base[G], n7cap) (a host upper bound on the length) are
 // cleared for the scan.
-__global__ void k_mc_keep(const GridView* views, const int* leaf_base, int G, int* keep, long long n7cap) {
-  const long long total = 7LL * leaf_base[G];
+// SINGLE: one grid given by value (sv, its leaf count on the device), whose
+// view and leaf range [0, count) this launch also writes to views[0] and
+// leaf_base[0..1] for the later passes; it reads neither.
+template <bool SINGLE>
+__global__ void k_mc_keep(const GridView* views, const int* leaf_base, int G, int* keep, long long n7cap, GridView sv,
+                          GridView* views_out, int* leaf_base_out) {
+  const int n1 = SINGLE ? *sv.count : 0;
+  if (SINGLE && blockIdx.x == 0 && threadIdx.x == 0) {
+    views_out[0] = sv;
+    leaf_base_out[0] = 0;
+    leaf_base_out[1] = n1;
+  }
+  const long long total = 7LL * (SINGLE ? n1 : leaf_base[G]);
   for (long long t = total + blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n7cap;
        t += (long long)gridDim.x * blockDim.x)
     keep[t] = 0;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
-    const int g = segment_of(leaf_base, G, t / 7);  // 7 * leaf_base is monotone like leaf_base
-    const long long loc = t - 7LL * leaf_base[g];
-    const int n = leaf_base[g + 1] - leaf_base[g];
+    const int g = SINGLE ? 0 : segment_of(leaf_base, G, t / 7);  // 7 * leaf_base is monotone like leaf_base
+    const long long loc = SINGLE ? t : t - 7LL * leaf_base[g];
+    const int n = SINGLE ? n1 : leaf_base[g + 1] - leaf_base[g];
     const int k = (int)(loc / n), i = (int)(loc % n);
     if (k == 0) {
       keep[t] = 1;
@@ -156,7 +167,7 @@ __global__ void k_mc_keep(const GridView* views, const int* leaf_base, int G, in
     // looked up in leaf i's neighbour list, except leaf + 2 D[k] (the
     // opposite direction on k's axis, j = k - 1 for even k), which goes
     // through the tree.  All lookups are issued together.
-    const GridView& v = views[g];
+    const GridView& v = SINGLE ? sv : views[g];
     const VR_GLOBAL int* nb = v.nbr + (size_t)i * VR_NBR;
     const int kx = c_dir7[k][0] >> 3, ky = c_dir7[k][1] >> 3, kz = c_dir7[k][2] >> 3;
     int far = -1;
@@ -175,20 +186,23 @@ __global__ void k_mc_keep(const GridView* views, const int* leaf_base, int G, in
   }
 }
 
-// tile_base[g] = first tile of grid g (G + 1 entries; tile_base[G] = total).
-__global__ void k_mc_tile_base(const int* leaf_base, int G, const int* pos, const int* ntiles, int* tile_base) {
+// tile_base[g] = first tile of grid g (G + 1 entries; tile_base[G] = total),
+// written by the first G + 1 threads of the scatter launch.
+__device__ __forceinline__ void mc_tile_base(const int* leaf_base, int G, const int* pos, int nt, int* tile_base) {
   const long long total7 = 7LL * leaf_base[G];
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) {
     const long long s = 7LL * leaf_base[g];
-    tile_base[g] = s < total7 ? pos[s] : *ntiles;
+    tile_base[g] = s < total7 ? pos[s] : nt;
   }
 }
 
 // tanchor[rank] = i << 3 | k: the tile is leaf i of its grid plus D[k].
+// Also writes tile_base.
 __global__ void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, int G, const int* keep, const int* pos,
-                                   const int* ntiles, int4* tiles, int* tanchor) {
+                                   const int* ntiles, int4* tiles, int* tanchor, int* tile_base) {
   const long long total = 7LL * leaf_base[G];
   const int nt = *ntiles;
+  mc_tile_base(leaf_base, G, pos, nt, tile_base);
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     if (!keep[t]) continue;
@@ -680,6 +694,27 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   }
 }
 
+// Per-grid vertex and triangle ranges: out[4 g] = first vertex, [4 g + 1] =
+// vertex count, [4 g + 2] = first triangle, [4 g + 3] = triangle count.
+// counts_out (optional): the vertex and triangle totals.  Written by the
+// first block of the emit launch (the scans are complete by then).
+__device__ __forceinline__ void mc_ranges(const int* tile_base, int G, int nt, const int* vbase, const int* tbase,
+                                          const int* totals, int* out, int* counts_out) {
+  if (counts_out && threadIdx.x == 0) {
+    counts_out[0] = totals[1];
+    counts_out[1] = totals[2];
+  }
+  for (int g = threadIdx.x; g < G; g += blockDim.x) {
+    const int a = tile_base[g], b = tile_base[g + 1];
+    const int va = a < nt ? vbase[a] : totals[1], vb = b < nt ? vbase[b] : totals[1];
+    const int ta = a < nt ? tbase[a] : totals[2], tb = b < nt ? tbase[b] : totals[2];
+    out[4 * g] = va;
+    out[4 * g + 1] = vb - va;
+    out[4 * g + 2] = ta;
+    out[4 * g + 3] = tb - ta;
+  }
+}
+
 // ------------------------------------------------------------------ pass 2
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
@@ -706,7 +741,7 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
                                                          float iso, const uint32_t* meta, const int* tv, const int* tt,
                                                          const int* vbase, const int* tbase, const int* totals,
                                                          double* verts, uint32_t* tris, long long vcap,
-                                                         long long tricap) {
+                                                         long long tricap, int G, int* ranges, int* counts_out) {
   __shared__ uint16_t s_emn[256];
   __shared__ __align__(16) int8_t s_tri[256 * 16];  // TRI_TABLE rows
   __shared__ uint32_t s_ownsort[27 * 12];
@@ -728,6 +763,7 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
   MCTileLDS& w = s_w[wv];
   MCEmitLDS& el = s_e[wv];
   const int nt = *ntiles;
+  if (blockIdx.x == 0) mc_ranges(tile_base, G, nt, vbase, tbase, totals, ranges, counts_out);
   // writes stop at the counts or at the output capacity, whichever is lower
   const int vmax = (int)min((long long)totals[1], vcap), tmax = (int)min((long long)totals[2], tricap);
   int t_end, t_step;
@@ -842,26 +878,6 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
   }
 }
 
-// Per-grid vertex and triangle ranges: out[4 g] = first vertex, [4 g + 1] =
-// vertex count, [4 g + 2] = first triangle, [4 g + 3] = triangle count.
-// counts_out (optional): the vertex and triangle totals.
-__global__ void k_mc_ranges(const int* tile_base, int G, const int* ntiles, const int* vbase, const int* tbase,
-                            const int* totals, int* out, int* counts_out) {
-  const int nt = *ntiles;
-  if (counts_out && blockIdx.x == 0 && threadIdx.x == 0) {
-    counts_out[0] = totals[1];
-    counts_out[1] = totals[2];
-  }
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
-    const int a = tile_base[g], b = tile_base[g + 1];
-    const int va = a < nt ? vbase[a] : totals[1], vb = b < nt ? vbase[b] : totals[1];
-    const int ta = a < nt ? tbase[a] : totals[2], tb = b < nt ? tbase[b] : totals[2];
-    out[4 * g] = va;
-    out[4 * g + 1] = vb - va;
-    out[4 * g + 2] = ta;
-    out[4 * g + 3] = tb - ta;
-  }
-}
 
 // ------------------------------------------------------------------ host driver
 static int blocks_for(size_t n, int b) {
@@ -882,8 +898,11 @@ static int mc_blocks(const vr_ctx* ctx, const void* kernel, int waves, int tcap)
 }
 
 // Phase A: tile list for G grids (leaf counts known on the host: ltot).
-// Leaves the tile count in s.totals[0] and tile_base; caller reads nt.
-int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot) {
+// Leaves the tile count in s.totals[0]; caller reads nt.  single (G = 1):
+// the grid's view on the host; the keep launch then writes d_views[0] and
+// d_leaf_base[0..1] itself.
+int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
+                   const GridView* single) {
   hipError_t e;
   const size_t n7 = (size_t)(7 * ltot);
   if ((e = s.keep.ensure(sizeof(int) * (n7 + 1))) != hipSuccess || (e = s.pos.ensure(sizeof(int) * (n7 + 1))) != hipSuccess ||
@@ -891,20 +910,24 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
       (e = s.totals.ensure(sizeof(int) * 4)) != hipSuccess ||
       (e = s.tile_base.ensure(sizeof(int) * (size_t)(G + 1))) != hipSuccess)
     return vr_hip_fail(ctx, e, "scratch");
-  hipLaunchKernelGGL(k_mc_keep, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views, d_leaf_base, G,
-                     s.keep.as<int>(), (long long)n7);
+  if (single)
+    hipLaunchKernelGGL(k_mc_keep<true>, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views, d_leaf_base, G,
+                       s.keep.as<int>(), (long long)n7, *single, const_cast<GridView*>(d_views),
+                       const_cast<int*>(d_leaf_base));
+  else
+    hipLaunchKernelGGL(k_mc_keep<false>, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views, d_leaf_base, G,
+                       s.keep.as<int>(), (long long)n7, GridView{}, (GridView*)nullptr, (int*)nullptr);
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = vr_exclusive_scan(s.keep.as<int>(), s.pos.as<int>(), n7, s.sums.as<int>(), s.totals.as<int>(),
                              ctx->stream, d_leaf_base + G, 7)) != hipSuccess)
     return vr_hip_fail(ctx, e, "mc tiles");
-  hipLaunchKernelGGL(k_mc_tile_base, dim3(blocks_for(G + 1, 256)), dim3(256), 0, ctx->stream, d_leaf_base, G,
-                     s.pos.as<int>(), s.totals.as<int>(), s.tile_base.as<int>());
-  return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc tile base");
+  return VR_OK;
 }
 
-// Phase B (tile capacity tcap known): scatter, neighbours, count, scans.
+// Phase B (tile capacity tcap known): scatter (and tile_base), neighbours,
+// count, scans.
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
-                   int tcap, float iso, int* counts_out) {
+                   int tcap, float iso) {
   hipError_t e;
   const size_t n7 = (size_t)(7 * ltot);
   const size_t tc = (size_t)(tcap + 1);
@@ -919,7 +942,8 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
     return vr_hip_fail(ctx, e, "scratch");
   int* totals = s.totals.as<int>();
   hipLaunchKernelGGL(k_mc_scatter_tiles, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views, d_leaf_base, G,
-                     s.keep.as<int>(), s.pos.as<int>(), totals, s.tiles.as<int4>(), s.tanchor.as<int>());
+                     s.keep.as<int>(), s.pos.as<int>(), totals, s.tiles.as<int4>(), s.tanchor.as<int>(),
+                     s.tile_base.as<int>());
   if (tcap >= MC_TN_BIG)
     hipLaunchKernelGGL(k_mc_tile_nbr, dim3(blocks_for((size_t)tcap, MC_TN_TILES)), dim3(256), 0, ctx->stream, d_views,
                        s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(), s.tleaf.as<int>(),
@@ -939,14 +963,13 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
       (e = vr_exclusive_scan2(s.tv.as<int>(), s.vb.as<int>(), totals + 1, s.tt.as<int>(), s.tb.as<int>(),
                               totals + 2, (size_t)tcap, s.sums2.as<int>(), ctx->stream, totals)) != hipSuccess)
     return vr_hip_fail(ctx, e, "mc count");
-  hipLaunchKernelGGL(k_mc_ranges, dim3(blocks_for(G, 256)), dim3(256), 0, ctx->stream, s.tile_base.as<int>(), G, totals,
-                     s.vb.as<int>(), s.tb.as<int>(), totals, s.ranges.as<int>(), counts_out);
-  return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc ranges");
+  return VR_OK;
 }
 
-// Phase C: emit into verts/tris (capacities from the counts).
-int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, float iso, double* verts, uint32_t* tris,
-                  double algorithmic_bytes, long long vcap, long long tricap) {
+// Phase C: emit into verts/tris (capacities from the counts), and the
+// per-grid ranges (s.ranges; counts_out, optional: the totals).
+int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int tcap, float iso, double* verts,
+                  uint32_t* tris, double algorithmic_bytes, long long vcap, long long tricap, int* counts_out) {
   hipError_t e;
   if (vcap < 0) vcap = 0x7fffffffLL;
   if (tricap < 0) tricap = 0x7fffffffLL;
@@ -955,19 +978,9 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int tcap, fl
     hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, (const void*)k_mc_emit, MC_EWAVES, tcap)), dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(), iso,
                        s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
-                       s.totals.as<int>(), verts, tris, vcap, tricap);
+                       s.totals.as<int>(), verts, tris, vcap, tricap, G, s.ranges.as<int>(), counts_out);
   }
   return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc emit");
-}
-
-// The view and the leaf range [0, *count) of a single grid, written on the
-// device (no host copy).
-__global__ void k_mc_setup(GridView v, GridView* views, const int* count, int* leaf_base) {
-  if (threadIdx.x == 0) {
-    views[0] = v;
-    leaf_base[0] = 0;
-    leaf_base[1] = *count;
-  }
 }
 
 // Single grid (volume_to_mesh) through the multi-grid passes with G = 1.
@@ -978,17 +991,16 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
   MCMulti& s = ctx->mc;
   if ((e = s.views.ensure(sizeof(GridView))) != hipSuccess || (e = s.leaf_base.ensure(sizeof(int) * 2)) != hipSuccess)
     return vr_hip_fail(ctx, e, "scratch");
-  hipLaunchKernelGGL(k_mc_setup, dim3(1), dim3(64), 0, ctx->stream, make_view(g), s.views.as<GridView>(),
-                     (const int*)g->topo->d_count, s.leaf_base.as<int>());
-  if ((e = hipGetLastError()) != hipSuccess) return vr_hip_fail(ctx, e, "mc setup");
-  int rc = mc_tiles_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n);
+  // the view and the leaf range [0, count) go to the device with the first pass
+  const GridView v = make_view(g);
+  int rc = mc_tiles_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, &v);
   if (rc != VR_OK) return rc;
   if (deferred) {
     // Capacities from the bounds: at most 7 tiles per leaf; the mesh keeps
     // the capacity it has (a first call takes a guess); the counts go to
     // d_counts and the caller checks them against the capacity.
     const int tcap = 7 * (n < 1 ? 1 : n);
-    rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, tcap, iso, d_counts);
+    rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, tcap, iso);
     if (rc != VR_OK) return rc;
     if (m->cap_v < 1 || m->cap_t < 1) {
       const uint64_t cv = (uint64_t)16 * (uint64_t)(n < 256 ? 256 : n);
@@ -1003,8 +1015,8 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
       m->cap_v = cv;
       m->cap_t = 2 * cv;
     }
-    return mc_emit_multi(ctx, s, s.views.as<GridView>(), tcap, iso, m->d_v, m->d_t, 0.0, (long long)m->cap_v,
-                         (long long)m->cap_t);
+    return mc_emit_multi(ctx, s, s.views.as<GridView>(), 1, tcap, iso, m->d_v, m->d_t, 0.0, (long long)m->cap_v,
+                         (long long)m->cap_t, d_counts);
   }
   int nt = 0;
   if ((e = hipMemcpyAsync(&nt, s.totals.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
@@ -1034,5 +1046,5 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
       return vr_hip_fail(ctx, e, "hipMalloc mesh");
     m->cap_t = (uint64_t)th[2] + 1;
   }
-  return mc_emit_multi(ctx, s, s.views.as<GridView>(), nt, iso, m->d_v, m->d_t, 24.0 * th[1] + 12.0 * th[2]);
+  return mc_emit_multi(ctx, s, s.views.as<GridView>(), 1, nt, iso, m->d_v, m->d_t, 24.0 * th[1] + 12.0 * th[2]);
 }
