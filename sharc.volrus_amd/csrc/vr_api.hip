@@ -57,6 +57,8 @@ Topo::~Topo() {
   if (d_roots) (void)hipFree(d_roots);
   if (d_child) (void)hipFree(d_child);
   if (d_nbr) (void)hipFree(d_nbr);
+  if (d_count_alt) (void)hipFree(d_count_alt);
+  if (d_nbr_alt) (void)hipFree(d_nbr_alt);
 }
 
 vr_grid::~vr_grid() {
@@ -250,15 +252,23 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
     if ((e = tot.ensure(sizeof(int) * 4)) != hipSuccess) return vr_hip_fail(ctx, e, "scratch");
     // With the host count an upper bound, the flags past the device count
     // are cleared so the scan sees only live leaves.
-    // The compaction also resets the internal nodes' leaf slots for the
-    // lookup rebuild below.
+    // The compaction updates the lookup arrays too: the new count and
+    // neighbour table go to the second buffers, which then become the
+    // topology's.
     const size_t R = t->roots.size();
+    if ((!t->d_count_alt && (e = hipMalloc(&t->d_count_alt, sizeof(int))) != hipSuccess) ||
+        (!t->d_nbr_alt &&
+         (e = hipMalloc(&t->d_nbr_alt, sizeof(int) * VR_NBR * (size_t)(t->cap < 1 ? 1 : t->cap))) != hipSuccess))
+      return vr_hip_fail(ctx, e, "hipMalloc");
     if ((!exact && !tail_clear && (e = vr_zero_tail(const_cast<int*>(keep), t->d_count, n, ctx->stream)) != hipSuccess) ||
         (e = vr_exclusive_scan(keep, pos.as<int>(), (size_t)n, sums.as<int>(), tot.as<int>(), ctx->stream,
                                t->d_count)) != hipSuccess ||
-        (e = vr_launch_compact(src, t->d_count, n, keep, pos.as<int>(), g->d_leaves, ctx->stream, t->d_child,
-                               4096 * (R ? R : 1))) != hipSuccess)
+        (e = vr_launch_compact_topo(src, t->d_count, n, keep, pos.as<int>(), tot.as<int>(), g->d_leaves, t->d_roots,
+                                    (int)R, t->d_child, t->d_origins, t->d_nbr, t->d_nbr_alt, t->d_count_alt,
+                                    count_copy, ctx->stream)) != hipSuccess)
       return vr_hip_fail(ctx, e, "compact");
+    std::swap(t->d_count, t->d_count_alt);
+    std::swap(t->d_nbr, t->d_nbr_alt);
     if (exact) {
       int cnt = 0;
       if ((e = hipMemcpyAsync(&cnt, tot.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
@@ -267,12 +277,6 @@ int compact_into(vr_ctx* ctx, vr_grid* g, const NanoLeaf* src, const int* keep, 
       t->count = cnt;
     }
     t->has_empty = has_empty_after;
-    // The new count (in tot) reaches the topology's count, and count_copy,
-    // through the lookup rebuild: the compaction above read the old count.
-    if ((e = vr_build_child(g->d_leaves, tot.as<int>(), t->cap < 1 ? 1 : t->cap, t->d_roots, (int)R, t->d_child,
-                            t->d_origins, ctx->stream, t->d_count, count_copy)) != hipSuccess ||
-        (e = vr_build_nbr(make_view(g), t->cap < 1 ? 1 : t->cap, t->d_nbr, ctx->stream)) != hipSuccess)
-      return vr_hip_fail(ctx, e, "topology");
     return VR_OK;
   }
   std::shared_ptr<Topo> t(new Topo());

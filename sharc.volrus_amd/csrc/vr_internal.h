@@ -37,6 +37,10 @@ struct Topo {
   int4* d_roots = nullptr;
   int* d_child = nullptr;
   int* d_nbr = nullptr;
+  // Second count and neighbour table: an in-place compaction writes the new
+  // ones there while it reads the old, and the pointers are then swapped.
+  int* d_count_alt = nullptr;
+  int* d_nbr_alt = nullptr;
   Topo() = default;
   Topo(const Topo&) = delete;
   Topo& operator=(const Topo&) = delete;

@@ -38,6 +38,10 @@ hipError_t vr_launch_nonempty(const NanoLeaf* leaves, const int* count, int cap,
 hipError_t vr_launch_compact(const NanoLeaf* src, const int* count, int cap, const int* keep, const int* pos,
                              NanoLeaf* dst, hipStream_t s, int* child_fill = nullptr, size_t nchild = 0);
 hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s, int* dst2 = nullptr);
+hipError_t vr_launch_compact_topo(const NanoLeaf* src, const int* count_old, int cap, const int* keep, const int* pos,
+                                  const int* total, NanoLeaf* dst, const int4* roots, int root_count, int* child,
+                                  int4* origins, const int* nbr_old, int* nbr_new, int* count_out, int* count_out2,
+                                  hipStream_t s);
 hipError_t vr_zero_tail(int* p, const int* count, int n, hipStream_t s);
 hipError_t vr_launch_popcount_words(const uint64_t* w, const int* count, int cap, int* out, hipStream_t s);
 hipError_t vr_launch_crossing_coords(const int4* origins, const int* count, int cap, const uint64_t* cross,

@@ -471,6 +471,54 @@ __global__ __launch_bounds__(256) void k_compact_leaves(const NanoLeaf* src, con
   }
 }
 
+// In-place compaction with the lookup arrays updated alongside (no tree
+// lookups): leaf i of the old live range [0, *count_old) with keep[i] goes
+// to j = pos[i] (records from src to dst), its origin to origins[j], its
+// internal-node slot to j, and its neighbour list is remapped from the old
+// table (a neighbour that is kept becomes its new index, one that is
+// removed becomes -1); a removed leaf's slot becomes -1.  The new count
+// (*total) goes to count_out (and count_out2); count_old, nbr_old and the
+// outputs are distinct arrays.
+__global__ __launch_bounds__(256) void k_compact_topo(const NanoLeaf* src, const int* count_old, const int* keep,
+                                                      const int* pos, const int* total, NanoLeaf* dst,
+                                                      const int4* roots, int root_count, int* child, int4* origins,
+                                                      const int* nbr_old, int* nbr_new, int* count_out,
+                                                      int* count_out2) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int c = *total;
+    *count_out = c;
+    if (count_out2) *count_out2 = c;
+  }
+  const int n = *count_old;
+  const int sub = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = blockIdx.x * 4 + sub; i < n; i += gridDim.x * 4) {
+    const bool kp = keep[i] != 0;
+    const int j = kp ? pos[i] : 0;
+    if (kp) {
+      const uint4* s = reinterpret_cast<const uint4*>(&src[i]);
+      uint4* d = reinterpret_cast<uint4*>(&dst[j]);
+      for (int q = lane; q < VR_LEAF_BYTES / 16; q += 64) d[q] = s[q];
+      if (lane < VR_NBR) {
+        const int i2 = nbr_old[(size_t)i * VR_NBR + lane];
+        nbr_new[(size_t)j * VR_NBR + lane] = (i2 >= 0 && keep[i2]) ? pos[i2] : -1;
+      }
+    }
+    if (lane == 63) {
+      const int x = src[i].origin[0], y = src[i].origin[1], z = src[i].origin[2];
+      const int kx = x & ~127, ky = y & ~127, kz = z & ~127;
+      int lo = 0, hi = root_count;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int4 r = roots[mid];
+        if (coord_less(r.x, r.y, r.z, kx, ky, kz)) lo = mid + 1; else hi = mid;
+      }
+      const int slot = (((x - kx) >> 3) << 8) | (((y - ky) >> 3) << 4) | ((z - kz) >> 3);
+      child[(size_t)lo * 4096 + slot] = kp ? j : -1;
+      if (kp) origins[j] = make_int4(x, y, z, 0);
+    }
+  }
+}
+
 __global__ void k_copy_int(const int* src, int* dst, int* dst2) {
   const int v = *src;
   *dst = v;
@@ -579,6 +627,14 @@ hipError_t vr_launch_compact(const NanoLeaf* src, const int* count, int cap, con
                              NanoLeaf* dst, hipStream_t s, int* child_fill, size_t nchild) {
   hipLaunchKernelGGL(k_compact_leaves, dim3(grid_for(cap, 4)), dim3(256), 0, s, src, count, keep, pos, dst,
                      child_fill, nchild);
+  return hipGetLastError();
+}
+hipError_t vr_launch_compact_topo(const NanoLeaf* src, const int* count_old, int cap, const int* keep, const int* pos,
+                                  const int* total, NanoLeaf* dst, const int4* roots, int root_count, int* child,
+                                  int4* origins, const int* nbr_old, int* nbr_new, int* count_out, int* count_out2,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(k_compact_topo, dim3(grid_for(cap, 4)), dim3(256), 0, s, src, count_old, keep, pos, total, dst,
+                     roots, root_count, child, origins, nbr_old, nbr_new, count_out, count_out2);
   return hipGetLastError();
 }
 hipError_t vr_copy_int(const int* src, int* dst, hipStream_t s, int* dst2) {

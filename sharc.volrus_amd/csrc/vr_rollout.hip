@@ -128,7 +128,9 @@ static int clone_topology(vr_ctx* ctx, const Topo& s, std::shared_ptr<Topo>* out
       (e = hipMalloc(&t->d_origins, sizeof(int4) * cap)) != hipSuccess ||
       (e = hipMalloc(&t->d_roots, sizeof(int4) * R)) != hipSuccess ||
       (e = hipMalloc(&t->d_child, sizeof(int) * 4096 * R)) != hipSuccess ||
-      (e = hipMalloc(&t->d_nbr, sizeof(int) * VR_NBR * cap)) != hipSuccess)
+      (e = hipMalloc(&t->d_nbr, sizeof(int) * VR_NBR * cap)) != hipSuccess ||
+      (e = hipMalloc(&t->d_count_alt, sizeof(int))) != hipSuccess ||
+      (e = hipMalloc(&t->d_nbr_alt, sizeof(int) * VR_NBR * cap)) != hipSuccess)
     return vr_hip_fail(ctx, e, "hipMalloc");
   if ((e = hipMemcpyAsync(t->d_count, s.d_count, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream)) != hipSuccess ||
       (e = hipMemcpyAsync(t->d_origins, s.d_origins, sizeof(int4) * cap, hipMemcpyDeviceToDevice, ctx->stream)) !=

@@ -181,6 +181,32 @@ def test_rebuild_after_advect_sphere128(orc, ctx):
     assert_nano_equal(g.download(), ref.to_nano(), "rebuild sphere128")
 
 
+@pytest.mark.parametrize("op", ["trim", "rebuild"])
+def test_lookups_after_in_place_compaction(orc, ctx, op):
+    """The neighbour lists, internal-node slots and origins that an in-place
+    compaction updates alongside the leaves: an advect whose departures leave
+    the leaf's brick (a lookup through the tree per corner) and marching
+    cubes, right after the compaction, against the oracle."""
+    announce(f"lookups after {op}")
+    og = orc.sphere(6.0, (0.5, -1.0, 2.0), 0.5, 5.0)
+    g = ctx.upload(og.to_nano())
+    if op == "trim":
+        orc.trim(og, 1.5)
+        g.trim(1.5)
+    else:
+        orc.rebuild(og, 2.0)
+        g.rebuild(2.0)
+    assert_nano_equal(g.download(), og.to_nano(), op)
+    params = (0.0, 0.0, 2 * math.pi, 0.0, 0.0, 0.0)
+    dt = 0.04  # up to ~3 voxels of rotation: corners outside the brick
+    want = orc.advect_analytic(og, 3, params, dt, 1)
+    got = g.advect(ctx.velocity_analytic(3, params), dt)
+    assert_nano_equal(got.download(), want.to_nano(), f"advect after {op}")
+    vw, tw, _ = orc.mesh(og, 0.0)
+    vg, tg = g.mesh(0.0)
+    assert np.array_equal(tg, tw) and np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"mesh after {op}"
+
+
 def test_rebuild_noop_cases(orc, ctx):
     announce("rebuild no-op")
     e = orc.new_level_set(1.0, 3.0)
