@@ -197,12 +197,22 @@ __device__ __forceinline__ void mc_tile_base(const int* leaf_base, int G, const 
 }
 
 // tanchor[rank] = i << 3 | k: the tile is leaf i of its grid plus D[k].
-// Also writes tile_base.
-__global__ void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, int G, const int* keep, const int* pos,
-                                   const int* ntiles, int4* tiles, int* tanchor, int* tile_base) {
+// Also writes tile_base.  LDS (one grid of at most MC_SCATTER_LDS leaves):
+// each block first copies the origins into LDS and searches there.
+#define MC_SCATTER_LDS 4096
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, int G,
+                                                          const int* keep, const int* pos, const int* ntiles,
+                                                          int4* tiles, int* tanchor, int* tile_base) {
+  __shared__ int4 s_org[LDS ? MC_SCATTER_LDS : 1];
   const long long total = 7LL * leaf_base[G];
   const int nt = *ntiles;
   mc_tile_base(leaf_base, G, pos, nt, tile_base);
+  if (LDS) {
+    const int n0 = leaf_base[1] - leaf_base[0];
+    for (int q = threadIdx.x; q < n0; q += blockDim.x) s_org[q] = views[0].origins[q];
+    __syncthreads();
+  }
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     if (!keep[t]) continue;
@@ -212,7 +222,8 @@ __global__ void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, 
     const int n = leaf_base[g + 1] - leaf_base[g];
     const int k = (int)(loc / n), i = (int)(loc % n);
     const GridView& v = views[g];
-    const int4 o = v.origins[i];
+    const int4* org = LDS ? s_org : v.origins;
+    const int4 o = org[i];
     const int ex = o.x + c_dir7[k][0], ey = o.y + c_dir7[k][1], ez = o.z + c_dir7[k][2];
     // The 7 lower bounds in lockstep (the halving sequence depends on n
     // only), so each step issues 7 independent loads.
@@ -223,7 +234,7 @@ __global__ void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, 
       const int half = len >> 1;
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        const int4 r = v.origins[b[j] + half];
+        const int4 r = org[b[j] + half];
         if (coord_less(r.x, r.y, r.z, ex - c_dir7[j][0], ey - c_dir7[j][1], ez - c_dir7[j][2])) b[j] += half;
       }
       len -= half;
@@ -231,7 +242,7 @@ __global__ void k_mc_scatter_tiles(const GridView* views, const int* leaf_base, 
     long long rank = pos[seg];
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
-      const int4 r = v.origins[b[j]];
+      const int4 r = org[b[j]];
       const int lb = b[j] + (coord_less(r.x, r.y, r.z, ex - c_dir7[j][0], ey - c_dir7[j][1], ez - c_dir7[j][2]) ? 1 : 0);
       const long long lj = seg + (long long)j * n;
       rank += (lb < n ? pos[lj + lb] : pos[lj + n - 1] + keep[lj + n - 1]) - pos[lj];
@@ -941,9 +952,14 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
       (e = s.ranges.ensure(sizeof(int) * 4 * (size_t)(G + 1))) != hipSuccess)
     return vr_hip_fail(ctx, e, "scratch");
   int* totals = s.totals.as<int>();
-  hipLaunchKernelGGL(k_mc_scatter_tiles, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views, d_leaf_base, G,
-                     s.keep.as<int>(), s.pos.as<int>(), totals, s.tiles.as<int4>(), s.tanchor.as<int>(),
-                     s.tile_base.as<int>());
+  if (G == 1 && ltot <= MC_SCATTER_LDS)
+    hipLaunchKernelGGL(k_mc_scatter_tiles<true>, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views,
+                       d_leaf_base, G, s.keep.as<int>(), s.pos.as<int>(), totals, s.tiles.as<int4>(),
+                       s.tanchor.as<int>(), s.tile_base.as<int>());
+  else
+    hipLaunchKernelGGL(k_mc_scatter_tiles<false>, dim3(blocks_for(n7, 256)), dim3(256), 0, ctx->stream, d_views,
+                       d_leaf_base, G, s.keep.as<int>(), s.pos.as<int>(), totals, s.tiles.as<int4>(),
+                       s.tanchor.as<int>(), s.tile_base.as<int>());
   if (tcap >= MC_TN_BIG)
     hipLaunchKernelGGL(k_mc_tile_nbr, dim3(blocks_for((size_t)tcap, MC_TN_TILES)), dim3(256), 0, ctx->stream, d_views,
                        s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(), s.tleaf.as<int>(),
