@@ -40,6 +40,7 @@ struct vr_batch {
   std::vector<double> dts;
   MCMulti mc;
   int tcap = 0;
+  bool tiles_ready = false;  // mc holds the tile lists (fixed: advection keeps every leaf set)
   long long vcap = 0, tricap = 0;
   double* d_verts = nullptr;
   uint32_t* d_tris = nullptr;
@@ -329,20 +330,26 @@ static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int schem
 }
 
 // Batched MC over the advected grids into the batch's mesh buffers (sizes
-// fixed by the first, host-synchronised run).
+// fixed by the first, host-synchronised run).  The tile lists depend on the
+// leaf sets only, which advection never changes: they are built once, by
+// that first run, and every later step (and graph) starts at the count.
 static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
-  int rc = mc_tiles_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->d_leaf_base.as<int>(), b->G, b->ltot);
-  if (rc != VR_OK) return rc;
+  int rc;
   hipError_t e;
-  if (sizing) {
+  if (!b->tiles_ready) {
+    if (!sizing) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "batch tile lists not built");
+    rc = mc_tiles_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->d_leaf_base.as<int>(), b->G, b->ltot);
+    if (rc != VR_OK) return rc;
     int nt = 0;
     if ((e = hipMemcpyAsync(&nt, b->mc.totals.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
       return vr_hip_fail(ctx, e, "readback");
     b->tcap = nt;
+    rc = mc_tiles_finish(ctx, b->mc, b->d_out_views.as<GridView>(), b->d_leaf_base.as<int>(), b->G, b->ltot, b->tcap);
+    if (rc != VR_OK) return rc;
+    b->tiles_ready = true;
   }
-  rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->d_leaf_base.as<int>(), b->G, b->ltot, b->tcap,
-                      iso);
+  rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->ltot, b->tcap, iso);
   if (rc != VR_OK) return rc;
   if (sizing) {
     int th[3];

@@ -935,10 +935,10 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
   return VR_OK;
 }
 
-// Phase B (tile capacity tcap known): scatter (and tile_base), neighbours,
-// count, scans.
-int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
-                   int tcap, float iso) {
+// Phase B (tile capacity tcap known): scatter (and tile_base), neighbours.
+// Depends on the leaf sets only.
+int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
+                    int tcap) {
   hipError_t e;
   const size_t n7 = (size_t)(7 * ltot);
   const size_t tc = (size_t)(tcap + 1);
@@ -968,7 +968,13 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
     hipLaunchKernelGGL(k_mc_tile_nbr_slots, dim3(blocks_for((size_t)tcap * VR_NBR, 256)), dim3(256), 0, ctx->stream,
                        d_views, s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(),
                        s.tleaf.as<int>(), s.ttile.as<int>());
-  if ((e = hipGetLastError()) != hipSuccess) return vr_hip_fail(ctx, e, "mc tiles");
+  return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc tiles");
+}
+
+// Phase C: count, scans (the tile lists of phases A and B in s).
+int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso) {
+  hipError_t e;
+  int* totals = s.totals.as<int>();
   {
     TimedLaunch tl(ctx, TC_MC, (double)VR_LEAF_BYTES, 0.0, (double)ltot);
     hipLaunchKernelGGL(k_mc_count, dim3(mc_blocks(ctx, (const void*)k_mc_count, MC_WAVES, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
@@ -982,7 +988,7 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
   return VR_OK;
 }
 
-// Phase C: emit into verts/tris (capacities from the counts), and the
+// Phase D: emit into verts/tris (capacities from the counts), and the
 // per-grid ranges (s.ranges; counts_out, optional: the totals).
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int tcap, float iso, double* verts,
                   uint32_t* tris, double algorithmic_bytes, long long vcap, long long tricap, int* counts_out) {
@@ -1016,7 +1022,8 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
     // the capacity it has (a first call takes a guess); the counts go to
     // d_counts and the caller checks them against the capacity.
     const int tcap = 7 * (n < 1 ? 1 : n);
-    rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, tcap, iso);
+    rc = mc_tiles_finish(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, tcap);
+    if (rc == VR_OK) rc = mc_count_multi(ctx, s, s.views.as<GridView>(), n, tcap, iso);
     if (rc != VR_OK) return rc;
     if (m->cap_v < 1 || m->cap_t < 1) {
       const uint64_t cv = (uint64_t)16 * (uint64_t)(n < 256 ? 256 : n);
@@ -1038,7 +1045,8 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
   if ((e = hipMemcpyAsync(&nt, s.totals.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
     return vr_hip_fail(ctx, e, "readback");
-  rc = mc_count_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, nt, iso);
+  rc = mc_tiles_finish(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, nt);
+  if (rc == VR_OK) rc = mc_count_multi(ctx, s, s.views.as<GridView>(), n, nt, iso);
   if (rc != VR_OK) return rc;
   int th[3] = {0, 0, 0};
   if ((e = hipMemcpyAsync(th, s.totals.p, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
