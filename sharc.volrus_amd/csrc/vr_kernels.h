@@ -7,10 +7,8 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
                                const int8_t* corner, const int8_t* edge, const int8_t* cand);
 
 hipError_t vr_fill_int(int* p, int v, size_t n, hipStream_t s);
-// count_out / count_out2 (optional) receive *count as well.
 hipError_t vr_build_child(const NanoLeaf* leaves, const int* count, int cap, const int4* roots, int root_count,
-                          int* child, int4* origins, hipStream_t s, int* count_out = nullptr,
-                          int* count_out2 = nullptr);
+                          int* child, int4* origins, hipStream_t s);
 hipError_t vr_build_nbr(const GridView& g, int cap, int* nbr, hipStream_t s);
 
 size_t vr_scan_scratch_ints(size_t n);

@@ -16,15 +16,9 @@ __global__ void k_fill_int(int* p, int v, size_t n) {
 }
 
 // child[r * 4096 + slot] = leaf index, origins[i] = leaf origin.
-// count_out / count_out2 (optional): the count is also stored there (by one
-// thread; nothing in this kernel reads them).
 __global__ void k_build_child(const NanoLeaf* leaves, const int* count, const int4* roots, int root_count, int* child,
-                              int4* origins, int* count_out, int* count_out2) {
+                              int4* origins) {
   const int n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (count_out) *count_out = n;
-    if (count_out2) *count_out2 = n;
-  }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int x = leaves[i].origin[0], y = leaves[i].origin[1], z = leaves[i].origin[2];
     origins[i] = make_int4(x, y, z, 0);
@@ -570,9 +564,9 @@ hipError_t vr_fill_int(int* p, int v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t vr_build_child(const NanoLeaf* leaves, const int* count, int cap, const int4* roots, int root_count,
-                          int* child, int4* origins, hipStream_t s, int* count_out, int* count_out2) {
+                          int* child, int4* origins, hipStream_t s) {
   hipLaunchKernelGGL(k_build_child, dim3(grid_for(cap, 256)), dim3(256), 0, s, leaves, count, roots, root_count, child,
-                     origins, count_out, count_out2);
+                     origins);
   return hipGetLastError();
 }
 hipError_t vr_build_nbr(const GridView& g, int cap, int* nbr, hipStream_t s) {
