@@ -72,6 +72,7 @@ struct TimingRec {
   double bytes;
   double per_leaf, fixed;
   int seg, mesh;
+  int launches;  // launches between a and b (a run of back-to-back launches of one class)
 };
 
 struct vr_ctx {
@@ -88,6 +89,12 @@ struct vr_ctx {
   const void* cross_any = nullptr;  // the mark's buffer that generation counts on
   int roll_seg = -1;   // rollout segment of the launches being recorded (-1: none)
   int roll_mesh = -1;  // rollout mesh index of the launches being recorded
+  // An open run of back-to-back launches of class group_cls timed by one
+  // event pair (timing_group_begin/end): its launches record no events of
+  // their own, only their count and bytes.
+  int group_cls = -1, group_n = 0;
+  hipEvent_t group_a = nullptr;
+  double group_per_leaf = 0.0, group_fixed = 0.0, group_leaves = 0.0;
   double t_ms[TC_COUNT] = {0, 0, 0, 0, 0, 0};
   double t_bytes[TC_COUNT] = {0, 0, 0, 0, 0, 0};
   uint64_t t_n[TC_COUNT] = {0, 0, 0, 0, 0, 0};
@@ -104,6 +111,11 @@ struct TimedLaunch {
   TimedLaunch(vr_ctx* c, int k, double per_leaf_bytes, double fixed_bytes, double leaf_count, bool mesh_bytes = false);
   ~TimedLaunch();
 };
+// Opens / closes a run of back-to-back launches of one class timed as a
+// whole (no event packets between them); no-ops when timing is off.
+void timing_group_begin(vr_ctx* ctx, int cls);
+void timing_group_end(vr_ctx* ctx);
+
 
 struct vr_grid {
   std::shared_ptr<Topo> topo;

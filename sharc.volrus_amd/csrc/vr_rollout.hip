@@ -13,6 +13,14 @@ TimedLaunch::TimedLaunch(vr_ctx* c, int k, double per_leaf_bytes, double fixed_b
                          bool mesh_bytes)
     : ctx(c), cls(k), per_leaf(per_leaf_bytes), fixed(fixed_bytes), leaves(leaf_count), mesh(mesh_bytes) {
   if (!ctx || !ctx->timing) return;
+  if (ctx->group_cls == k) {
+    // inside a timed run: counted, not bracketed
+    ctx->group_n += 1;
+    ctx->group_per_leaf = per_leaf_bytes;
+    ctx->group_fixed = fixed_bytes;
+    ctx->group_leaves = leaf_count;
+    return;
+  }
   if (ctx->event_pool.empty()) {
     if (hipEventCreate(&a) != hipSuccess) {
       a = nullptr;
@@ -36,7 +44,43 @@ TimedLaunch::~TimedLaunch() {
   }
   (void)hipEventRecord(b, ctx->stream);
   ctx->pending.push_back({a, b, cls, per_leaf * leaves + fixed, per_leaf, fixed, ctx->roll_seg,
-                          mesh ? ctx->roll_mesh : -1});
+                          mesh ? ctx->roll_mesh : -1, 1});
+}
+
+static hipEvent_t pool_event(vr_ctx* ctx) {
+  hipEvent_t ev = nullptr;
+  if (ctx->event_pool.empty()) {
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+  } else {
+    ev = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+  }
+  return ev;
+}
+
+void timing_group_begin(vr_ctx* ctx, int cls) {
+  if (!ctx->timing || ctx->group_cls >= 0) return;
+  ctx->group_a = pool_event(ctx);
+  if (!ctx->group_a) return;
+  (void)hipEventRecord(ctx->group_a, ctx->stream);
+  ctx->group_cls = cls;
+  ctx->group_n = 0;
+}
+
+void timing_group_end(vr_ctx* ctx) {
+  if (ctx->group_cls < 0) return;
+  const int cls = ctx->group_cls, n = ctx->group_n;
+  ctx->group_cls = -1;
+  if (n == 0) {
+    ctx->event_pool.push_back(ctx->group_a);
+    return;
+  }
+  hipEvent_t b = pool_event(ctx);
+  if (!b) return;
+  (void)hipEventRecord(b, ctx->stream);
+  // bytes per launch (per_leaf, fixed) times the launches of the run
+  ctx->pending.push_back({ctx->group_a, b, cls, n * (ctx->group_per_leaf * ctx->group_leaves + ctx->group_fixed),
+                          n * ctx->group_per_leaf, n * ctx->group_fixed, ctx->roll_seg, -1, n});
 }
 
 static void drain_timing(vr_ctx* ctx) {
@@ -45,7 +89,7 @@ static void drain_timing(vr_ctx* ctx) {
     if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
       ctx->t_ms[r.cls] += ms;
       ctx->t_bytes[r.cls] += r.bytes;
-      ctx->t_n[r.cls] += 1;
+      ctx->t_n[r.cls] += (uint64_t)r.launches;
     }
     ctx->event_pool.push_back(r.a);
     ctx->event_pool.push_back(r.b);
@@ -239,6 +283,9 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
   }
   for (int s = 0; s < steps && rc == VR_OK; ++s) {
     const double t = t0 + (double)s * dt;
+    // the advect launches between two rebuild or mesh passes are timed as
+    // one run (one segment, so one leaf count)
+    timing_group_begin(ctx, TC_ADVECT);
     {
       // The view shares g's topology only for the launch, so the
       // compactions below see it unshared and run in place.
@@ -253,6 +300,9 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
     std::swap(g->d_leaves, other);
     seg_steps[seg] += 1;
     st.advect_steps += 1;
+    const bool rebuild_now = rebuild_every > 0 && (s + 1) % rebuild_every == 0;
+    const bool mesh_now = mesh_every > 0 && (s + 1) % mesh_every == 0 && imesh + 1 < nmesh;
+    if (g->topo->has_empty || rebuild_now || mesh_now || s + 1 == steps) timing_group_end(ctx);
     if (g->topo->has_empty) {
       // Inputs with leaves that hold no active voxel: the first step drops them.
       DevBuf& keep = ctx->scratch[SCR_KEEP];
@@ -269,7 +319,7 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
       rc = compact_into(ctx, g, other, keep.as<int>(), false, false);
       if (rc != VR_OK) break;
     }
-    if (rebuild_every > 0 && (s + 1) % rebuild_every == 0) {
+    if (rebuild_now) {
       const bool next = seg + 1 < nseg;
       // the rebuild writes the next segment's leaf and active counts
       rc = rebuild_impl(ctx, g, half_width, other, false, next ? seg_leaves.as<int>() + seg + 1 : nullptr,
@@ -281,7 +331,7 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
         ctx->roll_seg = seg;
       }
     }
-    if (mesh_every > 0 && (s + 1) % mesh_every == 0 && imesh + 1 < nmesh) {
+    if (mesh_now) {
       ++imesh;
       if (g->topo->count > 0) {  // an upper bound: 0 means no leaves
         ctx->roll_mesh = imesh;
@@ -295,6 +345,7 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
       st.meshes += 1;
     }
   }
+  timing_group_end(ctx);  // a run left open by an error
   ctx->roll_seg = -1;
   ctx->roll_mesh = -1;
   // The grid keeps whichever buffer holds its data; the scratch slot takes
@@ -325,6 +376,7 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
     TimingRec& r = ctx->pending[i];
     if (r.seg < 0) continue;
     const double mesh_bytes = r.mesh >= 0 ? 24.0 * mc[2 * r.mesh] + 12.0 * mc[2 * r.mesh + 1] : r.fixed;
+    // (for a run of launches, per_leaf and fixed already hold the run's totals)
     r.bytes = r.per_leaf * leaves[r.seg] + mesh_bytes;
   }
   for (int k = 0; k <= seg; ++k) st.voxel_updates += act[k] * (uint64_t)seg_steps[k];
