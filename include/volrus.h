@@ -219,8 +219,10 @@ enum {
   VR_TIMING_MC = 4,
   VR_TIMING_GRAPH = 5
 };
-/* Turns per-launch HIP-event timing on the context stream on or off and
- * clears the totals. */
+/* Turns HIP-event timing on the context stream on or off and clears the
+ * totals.  Each launch is bracketed by its own pair of events, except the
+ * back-to-back advect launches of a rollout segment (between two rebuild or
+ * mesh passes), which one pair brackets as a run (launches counted singly). */
 int vr_ctx_set_timing(vr_ctx* ctx, int enable);
 /* Device time, launch count and algorithmic bytes (each input and output
  * byte counted once) accumulated for one kernel class since timing was on. */

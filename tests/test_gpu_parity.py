@@ -342,6 +342,28 @@ def test_rollout_matches_oracle_composition(orc, ctx, vr, shared):
         assert (st.last_vertex_count, st.last_triangle_count) == (len(wmesh[0]), len(wmesh[1]))
 
 
+def test_rollout_timing_records(orc, ctx, vr):
+    """Event timing of a rollout: every advect launch is counted (the runs of
+    back-to-back advects between rebuild and mesh passes share one event
+    pair), and their algorithmic bytes are 2 x 2128 per leaf of the segment."""
+    announce("rollout timing")
+    vs, hw, period = 1.0 / 64, 3.0, 8.0
+    g = ctx.upload(slotted_sphere(orc, vs).to_nano())
+    leaves0 = g.counts()[0]
+    vel = ctx.velocity_analytic(vr.VEL_VORTEX, (), period)
+    ctx.set_timing(True)
+    try:
+        st = g.rollout(vel, 0.0, vs, vr.RK3, 9, 3, hw, 4, 0.0, vr.Mesh(ctx))
+        ms, n, b = ctx.timing(vr.TIMING_ADVECT)
+        rms, rn, _ = ctx.timing(vr.TIMING_REBUILD)
+    finally:
+        ctx.set_timing(False)
+    assert n == st.advect_steps == 9 and rn == st.rebuilds == 3
+    assert ms > 0.0 and rms > 0.0
+    assert b % (2 * 2128) == 0
+    assert 9 * 2 * 2128 * st.final_leaf_count <= b <= 9 * 2 * 2128 * leaves0
+
+
 def test_rollout_mesh_capacity_regrowth(orc, ctx, vr):
     """A mesh larger than a fresh mesh's first capacity guess (16 vertices per
     leaf): the rollout grows the buffers and meshes the final state again, and
