@@ -141,7 +141,6 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
                                             AdvectLDS& s, int tid, double dt, const double tf[3]) {
   constexpr int NT = 64 * NW;
   const int lane = tid & 63, wave = tid >> 6;
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const NanoLeaf& L = in.leaves[li];
   NanoLeaf& O = out[li];
   const float bg = in.background;
@@ -163,7 +162,9 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
     const uint64_t m = mk[x];
-    if ((x % NW) == wave && ((m >> lane) & 1ull)) s.list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
+    // active voxels of this slab in lanes below this one (mbcnt: popcount of m under the lane mask)
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if ((x % NW) == wave && ((m >> lane) & 1ull)) s.list[total + rank] = (uint16_t)(x * 64 + lane);
     total += __popcll(m);
   }
   const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
