@@ -189,18 +189,34 @@ __device__ __forceinline__ double axis_weight(int node, int b, double f) {
 // A neighbour at offset (dx, dy, dz) covers the voxel only where its local
 // coordinate is 0 (for d = -1) or 7 (for d = +1) on every axis with d != 0,
 // so at most 8 bricks are visited, in ascending neighbour order.
+// The (up to 8) candidates are visited as a fixed 2 x 2 x 2 walk whose
+// entry (i, j, k) takes the i-th offset of the axis's ascending list ([0],
+// [-1, 0] at local 0, [0, 1] at local 7): all loads are issued before the
+// ordered sum, and an absent or unused candidate adds +0 (the sum starts at
+// +0 and is never -0, so adding +0 leaves it unchanged).
 __device__ __forceinline__ float merged_lambda(const float* ob, const float* far, const int* nb, int li, int x, int y,
                                                int z) {
+  const int ox0 = x == 0 ? -1 : 0, oy0 = y == 0 ? -1 : 0, oz0 = z == 0 ? -1 : 0;
+  const int nx = (x == 0 || x == 7) ? 2 : 1, ny = (y == 0 || y == 7) ? 2 : 1, nz = (z == 0 || z == 7) ? 2 : 1;
+  const float fv = far[(size_t)li * 512 + ((x << 6) | (y << 3) | z)];
+  float v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int i = q >> 2, j = (q >> 1) & 1, k = q & 1;
+    const int dx = ox0 + i, dy = oy0 + j, dz = oz0 + k;
+    const bool use = i < nx && j < ny && k < nz;
+    const int src = use ? nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] : -1;
+    const int lx = x - 8 * dx, ly = y - 8 * dy, lz = z - 8 * dz;
+    // an unused candidate reads the own brick's entry, which is discarded
+    const size_t at = src >= 0 ? (size_t)src * 1000 + (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1)
+                               : (size_t)li * 1000 + (x + 1) * 100 + (y + 1) * 10 + (z + 1);
+    const float t = ob[at];
+    v[q] = src >= 0 ? t : 0.0f;
+  }
   float s = 0.0f;
-  for (int dx = (x == 0 ? -1 : 0); dx <= (x == 7 ? 1 : 0); ++dx)
-    for (int dy = (y == 0 ? -1 : 0); dy <= (y == 7 ? 1 : 0); ++dy)
-      for (int dz = (z == 0 ? -1 : 0); dz <= (z == 7 ? 1 : 0); ++dz) {
-        const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
-        if (src < 0) continue;
-        const int lx = x - 8 * dx, ly = y - 8 * dy, lz = z - 8 * dz;
-        s += ob[(size_t)src * 1000 + (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1)];
-      }
-  return s + far[(size_t)li * 512 + ((x << 6) | (y << 3) | z)];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s += v[q];
+  return s + fv;
 }
 
 // One backward step, one leaf per block of ADJ_WAVES waves.
