@@ -316,11 +316,25 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
       for (int k = 0; k < VW; ++k) s_wt[a][l][k] = (float)axis_weight(k, bn, idx - fl);
     }
     {
+      // A thread's window nodes all load before any is stored (nodes
+      // outside the field read node 0 and store 0).
+      constexpr int UQ = (3 * VW3 + ADJ_THREADS - 1) / ADJ_THREADS;
       const int nn = P.vel.n;
       const size_t n3 = (size_t)nn * nn * nn;
-      for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) {
+      float uv[UQ];
+      bool inside[UQ];
+#pragma unroll
+      for (int q = 0; q < UQ; ++q) {
+        const int i0 = tid + q * ADJ_THREADS, i = i0 < 3 * VW3 ? i0 : 0;
         const int c = i / VW3, r = i - c * VW3;
-        s_u[i] = dense_at(P.vel.dense + c * n3, nn, wb[0] + r / (VW * VW), wb[1] + (r / VW) % VW, wb[2] + r % VW);
+        const int gi = wb[0] + r / (VW * VW), gj = wb[1] + (r / VW) % VW, gk = wb[2] + r % VW;
+        inside[q] = (unsigned)gi < (unsigned)nn && (unsigned)gj < (unsigned)nn && (unsigned)gk < (unsigned)nn;
+        uv[q] = P.vel.dense[inside[q] ? c * n3 + ((size_t)gi * nn + gj) * nn + gk : 0];
+      }
+#pragma unroll
+      for (int q = 0; q < UQ; ++q) {
+        const int i0 = tid + q * ADJ_THREADS;
+        if (i0 < 3 * VW3) s_u[i0] = inside[q] ? uv[q] : 0.0f;
       }
     }
     stage_interior<ADJ_THREADS>(g, li, brick, tid);
