@@ -241,23 +241,31 @@ __global__ __launch_bounds__(256) void k_crossings(GridView g, uint64_t* cross, 
       b[3] = (m & 8u) ? v.w : 0.0f;
     }
     __syncthreads();
-    // face layers: f = 0..5 for x = -1, x = 8, y = -1, y = 8, z = -1, z = 8
-    for (int h = tid; h < 384; h += 256) {
+    // face layers: f = 0..5 for x = -1, x = 8, y = -1, y = 8, z = -1, z = 8;
+    // a thread's two entries load together (an absent neighbour reads leaf
+    // 0, which is discarded)
+    float fv[2];
+    uint64_t fm[2];
+    int fb[2], fsrc[2], foff[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int h = tid + 256 * q < 384 ? tid + 256 * q : tid;
       const int f = h >> 6, u = (h >> 3) & 7, w = h & 7;
       const int side = (f & 1) ? 8 : -1;
       const int bx = f < 2 ? side : u, by = f < 2 ? u : (f < 4 ? side : w), bz = f < 4 ? w : side;
       const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
                 dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
-      const int src = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
-      const int off = voxel_offset(bx, by, bz);
-      float v = 0.0f;
-      if (src >= 0) {
-        const NanoLeaf& S = g.leaves[src];
-        const float sv = S.values[off];
-        v = ((S.mask[off >> 6] >> (off & 63)) & 1ull) ? sv : 0.0f;
-      }
-      brick[(bx + 1) * 100 + (by + 1) * 10 + (bz + 1)] = v;
+      fsrc[q] = nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+      foff[q] = voxel_offset(bx, by, bz);
+      fb[q] = (bx + 1) * 100 + (by + 1) * 10 + (bz + 1);
+      const NanoLeaf& S = g.leaves[fsrc[q] < 0 ? 0 : fsrc[q]];
+      fv[q] = S.values[foff[q]];
+      fm[q] = S.mask[foff[q] >> 6];
     }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (tid + 256 * q < 384)
+        brick[fb[q]] = (fsrc[q] >= 0 && ((fm[q] >> (foff[q] & 63)) & 1ull)) ? fv[q] : 0.0f;
     __syncthreads();
     bool found = false;
     for (int x = wave; x < 8; x += 4) {
@@ -327,16 +335,33 @@ __global__ __launch_bounds__(256) void k_rebuild(GridView g, const uint64_t* cro
     }
     if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
     __syncthreads();
-    for (int r = tid; r < 576; r += 256) {
-      const int X = r / 24, Y = r % 24;
-      const int nx = X >> 3, ny = Y >> 3;
-      uint32_t bits = 0;
+    {
+      // a thread's (up to) three rows of three words all load together (an
+      // absent neighbour reads leaf 0's word, which is discarded)
+      uint64_t cw[3][3];
+      int csrc[3][3];
 #pragma unroll
-      for (int nz = 0; nz < 3; ++nz) {
-        const int src = nb[nx * 9 + ny * 3 + nz];
-        if (src >= 0) bits |= (uint32_t)((cross[(size_t)src * 8 + (X & 7)] >> ((Y & 7) * 8)) & 0xFFull) << (8 * nz);
+      for (int q = 0; q < 3; ++q) {
+        const int r = tid + 256 * q < 576 ? tid + 256 * q : tid;
+        const int X = r / 24, Y = r % 24, nx = X >> 3, ny = Y >> 3;
+#pragma unroll
+        for (int nz = 0; nz < 3; ++nz) {
+          csrc[q][nz] = nb[nx * 9 + ny * 3 + nz];
+          cw[q][nz] = cross[(size_t)(csrc[q][nz] < 0 ? 0 : csrc[q][nz]) * 8 + (X & 7)];
+        }
       }
-      rows[r] = bits;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int r = tid + 256 * q;
+        if (r < 576) {
+          const int Y = r % 24;
+          uint32_t bits = 0;
+#pragma unroll
+          for (int nz = 0; nz < 3; ++nz)
+            if (csrc[q][nz] >= 0) bits |= (uint32_t)((cw[q][nz] >> ((Y & 7) * 8)) & 0xFFull) << (8 * nz);
+          rows[r] = bits;
+        }
+      }
     }
     __syncthreads();
     const uint32_t win = (2u << (2 * R)) - 1u;  // 2R+1 bits
