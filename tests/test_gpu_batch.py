@@ -132,17 +132,20 @@ def test_batch_velocity_kinds_and_tunings(orc, ctx, vr, kinds, tuning):
             assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"mode {mode} grid {i} vertices"
 
 
-def test_batch_mesh_large_tile_list(orc, ctx, vr):
+@pytest.mark.parametrize("lo,hi", [(0.3, 0.7), (-0.2, 0.2)])
+def test_batch_mesh_large_tile_list(orc, ctx, vr, lo, hi):
     """A batch whose tile list (at least its leaf count) takes the
     tile-per-lane neighbour kernel (32768 tiles and up), while each grid
     alone (at most 7 x its leaves) takes the per-slot one: every grid's mesh
-    from the batch equals its own mesh, and grid 0's equals the oracle's."""
-    announce("batch mesh large tile list")
+    from the batch equals its own mesh, and grid 0's equals the oracle's.
+    Centres in [lo, hi]^3: the second set puts the spheres across the origin
+    (negative coordinates, roots on both sides of 0)."""
+    announce(f"batch mesh large tile list {lo} {hi}")
     rng = np.random.default_rng(31)
     vs, hw = 1.0 / 256, 3.0
     spec = []
     for _ in range(40):
-        c = rng.uniform(0.3, 0.7, 3)
+        c = rng.uniform(lo, hi, 3)
         r = float(rng.uniform(0.15, 0.3))
         spec.append([(vr.PRIM_SPHERE, vr.CSG_UNION, (float(c[0]), float(c[1]), float(c[2]), r))])
     grids = [ctx.from_primitives(p, vs, hw) for p in spec]
