@@ -307,7 +307,8 @@ static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int schem
   vr_stage_factors(b->vel[0]->period, t, dt, scheme, tf);
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, ctx->stream, b->d_counters.as<int>());
   // wpc > 0: persistent, wpc waves per CU; wpc == 0: one block per chunk.
-  const int nw = vr_advect_waves(b->ltot, b->cus);
+  // the batched kernels come in one- and four-wave forms
+  const int nw = vr_advect_waves(b->ltot, b->cus) == 1 ? 1 : 4;
   int blocks = b->wpc > 0 ? b->cus * b->wpc / nw : 8 * b->max_range;
   if (blocks < 8) blocks = 8;
   const int persistent = b->wpc > 0 ? 1 : 0;

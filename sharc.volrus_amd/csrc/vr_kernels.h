@@ -25,8 +25,9 @@ hipError_t vr_launch_advect(const GridView& in, const VelView& vel, NanoLeaf* ou
                             const double tf[3], int scheme, int nw, hipStream_t s);
 // Waves per leaf for an advect over `leaves` leaves on `cus` compute units:
 // one (fewest instructions per leaf) once there are enough leaves to give
-// every CU 16 waves, else four (a leaf finishes sooner).
-inline int vr_advect_waves(long long leaves, int cus) { return leaves >= 16LL * (cus > 0 ? cus : 256) ? 1 : 4; }
+// every CU 16 waves, else several (a leaf finishes sooner; the launch picks
+// how many per velocity kind).
+inline int vr_advect_waves(long long leaves, int cus) { return leaves >= 16LL * (cus > 0 ? cus : 256) ? 1 : 8; }
 hipError_t vr_launch_crossings(const GridView& g, int cap, uint64_t* cross, int* any, int gen, hipStream_t s);
 hipError_t vr_launch_rebuild(const GridView& g, int cap, const uint64_t* cross, const int* any, int gen, NanoLeaf* out,
                              int* keep, int keep_len, int R, float vs, float band, unsigned long long* active_out,

@@ -168,8 +168,13 @@ __global__ void k_scan_add(int* out, const int* sums, size_t n) {
 // ballot/popcount prefix over the 8 mask words and spread over the block's
 // lanes; results go straight to the output leaf, the background to its
 // inactive positions.  NW = 1 issues the fewest
-// instructions per leaf (dense passes of 64 active voxels); NW = 4 finishes
-// a leaf sooner, for grids with too few leaves to fill the device.
+// instructions per leaf (dense passes of 64 active voxels); more waves
+// finish a leaf sooner, for grids with too few leaves to fill the device:
+// eight for the analytic fields (every thread stages at most one halo value
+// and advects at most one voxel of a band leaf; on the 1/512 single-vortex
+// rollout 8 waves beat 2, 4 and 16), four for the sampled fields (dense or
+// grids, whose per-voxel work is loads rather than arithmetic; 8 was slower
+// on the 1/256 dense-field rollout).
 template <int VK, int SCHEME, int NW>
 __global__ __launch_bounds__(64 * NW) void k_advect(GridView in, VelView vel, NanoLeaf* out, double dt, double tf0,
                                                     double tf1, double tf2) {
@@ -186,10 +191,12 @@ template <int VK, int SCHEME>
 static hipError_t launch_advect_t(const GridView& in, const VelView& vel, NanoLeaf* out, int cap, double dt,
                                   const double tf[3], int nw, hipStream_t s) {
   const int blocks = 8 * ((cap + 7) / 8 + 1);  // >= ceil(count / 8) blocks on each XCD
+  constexpr int NWS = (VK == VK_DENSE || VK == VK_GRIDS3) ? 4 : 8;
   if (nw == 1)
     hipLaunchKernelGGL((k_advect<VK, SCHEME, 1>), dim3(blocks), dim3(64), 0, s, in, vel, out, dt, tf[0], tf[1], tf[2]);
   else
-    hipLaunchKernelGGL((k_advect<VK, SCHEME, 4>), dim3(blocks), dim3(256), 0, s, in, vel, out, dt, tf[0], tf[1], tf[2]);
+    hipLaunchKernelGGL((k_advect<VK, SCHEME, NWS>), dim3(blocks), dim3(64 * NWS), 0, s, in, vel, out, dt, tf[0], tf[1],
+                       tf[2]);
   return hipGetLastError();
 }
 
