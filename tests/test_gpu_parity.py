@@ -342,6 +342,43 @@ def test_rollout_matches_oracle_composition(orc, ctx, vr, shared):
         assert (st.last_vertex_count, st.last_triangle_count) == (len(wmesh[0]), len(wmesh[1]))
 
 
+def test_empty_grid_through_every_pass(orc, ctx, vr):
+    """A level set without leaves (level_set_track.rs:265, volume_to_mesh.rs:582
+    test it per tool): advect (RK1 and RK3, analytic and dense fields),
+    rebuild, trim and a rollout with meshes all give the oracle's empty grid."""
+    announce("empty grid every pass")
+    og = orc.new_level_set(1.0 / 64, 3.0)
+    nano = og.to_nano()
+    assert (len(nano) - 320) // 2128 == 0
+    n = 8
+    u = np.random.default_rng(3).normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32)
+    dense = ctx.velocity_dense(u, n, 1.0 / n)
+    vortex = ctx.velocity_analytic(vr.VEL_VORTEX, (), 8.0)
+    for scheme in (1, 3):
+        want = orc.advect_dense(og, u, n, 1.0 / n, 0.01, scheme).to_nano()
+        assert_nano_equal(ctx.upload(nano).advect(dense, 0.01, 0.0, scheme).download(), want, f"dense {scheme}")
+        tf = orc.stage_factors(8.0, 0.0, 0.01, scheme)
+        want = orc.advect_analytic(og, 4, (), 0.01, scheme, tf).to_nano()
+        assert_nano_equal(ctx.upload(nano).advect(vortex, 0.01, 0.0, scheme).download(), want, f"vortex {scheme}")
+    for hw in (3.0, 1.5):
+        ref = orc.from_nano(nano)
+        orc.rebuild(ref, hw)
+        g = ctx.upload(nano)
+        g.rebuild(hw)
+        assert_nano_equal(g.download(), ref.to_nano(), f"rebuild {hw}")
+        ref = orc.from_nano(nano)
+        orc.trim(ref, hw)
+        g = ctx.upload(nano)
+        g.trim(hw)
+        assert_nano_equal(g.download(), ref.to_nano(), f"trim {hw}")
+    g = ctx.upload(nano)
+    st = g.rollout(vortex, 0.0, 0.01, vr.RK3, 6, 2, 3.0, 3, 0.0, vr.Mesh(ctx))
+    assert_nano_equal(g.download(), nano, "rollout")
+    assert st.voxel_updates == 0 and st.final_leaf_count == 0
+    assert (st.last_vertex_count, st.last_triangle_count) == (0, 0)
+    assert g.counts() == (0, 0)
+
+
 def test_rollout_timing_records(orc, ctx, vr):
     """Event timing of a rollout: every advect launch is counted (the runs of
     back-to-back advects between rebuild and mesh passes share one event
