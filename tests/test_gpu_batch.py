@@ -53,6 +53,39 @@ def test_batch_matches_oracle(orc, ctx, vr, mode):
         assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"grid {i} vertices"
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_batch_with_empty_member(orc, ctx, vr, mode):
+    """A grid without leaves first, between and after non-empty ones: its
+    output and mesh are empty, and the others still equal the oracle."""
+    announce(f"batch empty member mode={mode}")
+    rng = np.random.default_rng(8)
+    vs, hw = 1.0 / 128, 3.0
+    spec = scenes(vr, rng, 2, vs)
+    empty = orc.new_level_set(vs, hw).to_nano()
+    order = [None, spec[0], None, spec[1], None]
+    grids, vels = [], []
+    for item in order:
+        grids.append(ctx.upload(empty) if item is None else ctx.from_primitives(item[0], vs, hw))
+        vels.append(ctx.velocity_analytic(vr.VEL_UNIFORM, (0.5, -0.25, 0.75) if item is None else item[1]))
+    dt = 0.3 * vs
+    b = vr.Batch(ctx, grids, vels)
+    if mode == vr.BATCH_GRAPH:
+        b.step(dt, 0.0, vr.RK3, 0.0, vr.BATCH_GRAPH)  # first call sizes and captures
+    b.step(dt, 0.0, vr.RK3, 0.0, mode)
+    for i, item in enumerate(order):
+        vg, tg = b.mesh(i, mode)
+        if item is None:
+            assert_nano_equal(b.output_nano(i), empty, f"empty grid {i}")
+            assert len(vg) == 0 and len(tg) == 0, f"empty grid {i} mesh"
+            continue
+        og = orc.from_nano(grids[i].download())
+        want = orc.advect_analytic(og, vr.VEL_UNIFORM, item[1], dt, 3)
+        assert_nano_equal(b.output_nano(i), want.to_nano(), f"grid {i}")
+        vw, tw, _ = orc.mesh(want, 0.0)
+        assert tg.shape == tw.shape and np.array_equal(tg, tw), f"grid {i} triangles"
+        assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"grid {i} vertices"
+
+
 def test_batch_graph_replays_are_stable(orc, ctx, vr):
     announce("batch graph replay")
     rng = np.random.default_rng(5)
