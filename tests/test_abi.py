@@ -29,6 +29,20 @@ def test_python_binding_covers_header(vr):
     assert set(declared_symbols()) <= set(vr._SIGS), sorted(set(declared_symbols()) - set(vr._SIGS))
 
 
+def test_rust_binding_declares_every_entry_point():
+    """The reference-side (Rust) extern block in the integration notes names
+    every function of the header, with its own parameter count."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    header = open(os.path.join(ROOT, "include", "volrus.h")).read()
+    for name in declared_symbols():
+        m = re.search(r"pub fn " + name + r"\(([^)]*)\)", text)
+        assert m, f"{name} has no Rust declaration"
+        c = re.search(r"^[A-Za-z_][\w *]*\b" + name + r"\(([^)]*)\)", header, re.M)  # the declaration
+        n_rust = len([a for a in m.group(1).split(",") if a.strip()])
+        n_c = len([a for a in c.group(1).split(",") if a.strip() and a.strip() != "void"])
+        assert n_rust == n_c, (name, n_rust, n_c)
+
+
 def test_no_cpu_fallback_in_product():
     """The product package never reaches the oracle."""
     pkg = os.path.join(ROOT, "sharc.volrus_amd")
