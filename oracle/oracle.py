@@ -46,6 +46,8 @@ _SIGS = {
     "or_primitive_sdf": (_D, [_I, _P, _D, _D, _D]),
     "or_sincospi": (None, [_D, _P, _P]),
     "or_splitmix_next": (_U64, [_P]),
+    "or_value_noise4": (_D, [_D, _D, _D, _D, _D, _U64]),
+    "or_value_noise_lattice": (_D, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _U64]),
     "or_advect_grids3": (_P, [_P, _P, _P, _P, _D]),
     "or_advect_grids3_scheme": (_P, [_P, _P, _P, _P, _D, _I, _P]),
     "or_advect_analytic": (_P, [_P, _I, _P, _D, _I, _P]),
@@ -216,6 +218,24 @@ def sincospi(x):
     s, c = ctypes.c_double(), ctypes.c_double()
     lib().or_sincospi(x, ctypes.byref(s), ctypes.byref(c))
     return s.value, c.value
+
+
+def splitmix64(seed, n):
+    """The first n outputs of SplitMix64 from state `seed`."""
+    s = ctypes.c_uint64(seed)
+    return [lib().or_splitmix_next(ctypes.byref(s)) for _ in range(n)]
+
+
+def primitive_sdf(kind, params, x, y, z):
+    return lib().or_primitive_sdf(kind, _ptr(_dvec(params, 8)), x, y, z)
+
+
+def value_noise4(x, y, z, amp, f0, seed):
+    return lib().or_value_noise4(x, y, z, amp, f0, seed)
+
+
+def value_noise_lattice(i, j, k, seed):
+    return lib().or_value_noise_lattice(i, j, k, seed)
 
 
 def stage_factors(period, t, dt, scheme):

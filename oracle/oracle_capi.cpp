@@ -105,6 +105,13 @@ uint64_t or_splitmix_next(uint64_t* state) {
   *state = r.s;
   return v;
 }
+double or_value_noise4(double x, double y, double z, double amp, double f0, uint64_t seed) {
+  const double w[3] = {x, y, z};
+  return value_noise4(w, amp, f0, seed);
+}
+double or_value_noise_lattice(int64_t i, int64_t j, int64_t k, uint64_t seed) {
+  return value_noise_lattice(i, j, k, seed);
+}
 
 // ---------------------------------------------------------------- advection
 // Reference RK1 step with a VelocityField of three grids (advection.rs:32-56).
