@@ -143,8 +143,11 @@ int vr_mesh_create(vr_ctx* ctx, vr_mesh** out);
  * in place on g.  The last mesh is left in `mesh` (may be NULL).  The call
  * synchronises with the host once, at the end; meshes on the way are written
  * into the mesh's buffers up to their capacity, and when a mesh is larger
- * the buffers grow and the final state is meshed again, so the returned mesh
- * is always whole (pass the same mesh to later calls to keep the capacity). */
+ * the buffers grow and the state of the last mesh step is meshed again (at
+ * that step, with one extra synchronisation, when it is not the last step),
+ * so the returned mesh is always whole and always shows the state after step
+ * (steps / mesh_every) * mesh_every (pass the same mesh to later calls to
+ * keep the capacity). */
 typedef struct vr_rollout_stats {
   uint64_t voxel_updates; /* sum over steps of the active voxels advected */
   uint64_t advect_steps;
@@ -186,8 +189,12 @@ int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_velocity* vel,
  * taking chunks from per-XCD atomic counters, or, with waves_per_cu = 0, one
  * block per chunk -- followed by marching cubes over the concatenated tiles
  * of all grids.  The advect code is specialised for the velocity kind when
- * all grids share one.  Inputs are borrowed and left
- * unchanged; outputs live in the batch (vr_batch_output).  Modes:
+ * all grids share one.  The batch copies the input leaves at creation, so
+ * the input grids may be changed or destroyed afterwards; the velocities are
+ * borrowed and must outlive the batch.  Outputs live in the batch
+ * (vr_batch_output).  A GRAPH step replays the graph captured for the same
+ * dt, t, scheme, iso and tuning; anything that changes a buffer it uses
+ * (vr_batch_set_dt, a new tuning, a larger mesh) captures it again.  Modes:
  *   VR_BATCH_LOOP  the per-grid path (one advect launch + one MC pipeline per grid),
  *   VR_BATCH_EAGER the batched kernels launched directly,
  *   VR_BATCH_GRAPH the batched kernels captured once into a hipGraph and replayed. */
@@ -202,7 +209,8 @@ int vr_batch_set_dt(vr_ctx* ctx, vr_batch* b, const double* dts);
  * pair. */
 int vr_batch_autotune(vr_ctx* ctx, vr_batch* b, double dt, int scheme, int* leaves_per_grab, int* waves_per_cu);
 int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int scheme, float iso, int mode);
-/* Mesh of grid i from the last step in `mode` (LOOP keeps its own meshes). */
+/* Mesh of grid i from the last step in `mode` (LOOP keeps its own meshes).
+ * VR_ERR_CAPACITY if the mesh does not lie inside the batch's buffers. */
 int vr_batch_mesh_counts(vr_ctx* ctx, vr_batch* b, int i, int mode, uint64_t* vertex_count, uint64_t* tri_count);
 int vr_batch_mesh_copy(vr_ctx* ctx, vr_batch* b, int i, int mode, double* xyz, size_t vcap, uint32_t* tri,
                        size_t tcap);

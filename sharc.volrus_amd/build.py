@@ -3,6 +3,7 @@
 Run as `python sharc.volrus_amd/build.py`.  The library is written next to
 this file; nothing is installed.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -24,16 +25,24 @@ FLAGS = [
 
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
+    objs, cmds = [], []
     for s in SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(HERE, "_build", s + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
-        cmd = [hipcc] + FLAGS + ["-c", src, "-o", obj]
+        cmds.append([hipcc] + FLAGS + ["-c", src, "-o", obj])
+        objs.append(obj)
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    # one hipcc per source, at most one per host core (at most 16)
+    workers = max(1, min(len(cmds), os.cpu_count() or 1, 16))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(compile_one, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

@@ -25,7 +25,10 @@
 
 struct vr_batch {
   int G = 0;
-  std::vector<vr_grid*> in;
+  // The batch's own copies of the input leaves (sharing the inputs' lookup
+  // arrays, which no later call on an input changes in place while they are
+  // shared), so the inputs may be modified or destroyed after creation.
+  std::vector<std::unique_ptr<vr_grid>> in;
   std::vector<const vr_velocity*> vel;
   std::vector<std::unique_ptr<vr_grid>> out;  // advected grids (own leaves, share topology)
   std::vector<int> leaf_base;                 // G + 1
@@ -45,7 +48,15 @@ struct vr_batch {
   double* d_verts = nullptr;
   uint32_t* d_tris = nullptr;
   std::vector<int> ranges_host;  // 4 per grid after a step
+  // Mesh capacities come from a host-synchronised sizing run; they hold for
+  // the step parameters of that run (s_*) and the per-grid dts of that time.
   bool sized = false;
+  double s_dt = 0, s_t = 0;
+  int s_scheme = 0;
+  float s_iso = 0;
+  // The captured step and the parameters it was captured with.  Any change
+  // to a buffer the graph reads or writes (chunk table, mesh buffers, dts)
+  // drops it (batch_drop_graph), so a replay never sees a freed buffer.
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double g_dt = 0, g_t = 0;
@@ -59,6 +70,13 @@ struct vr_batch {
     if (d_tris) (void)hipFree(d_tris);
   }
 };
+
+static void batch_drop_graph(vr_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
 
 // ------------------------------------------------------------------ persistent advect
 // VK = the velocity kind shared by every grid of the batch (the leaf code is
@@ -185,6 +203,7 @@ __global__ void k_batch_reset(int* counters) {
 // (never inside a graph capture: the step builds it before capturing).
 static int batch_chunks(vr_ctx* ctx, vr_batch* b) {
   if (b->chunk_lpg == b->lpg && b->d_chunks.p) return VR_OK;
+  batch_drop_graph(b);  // the table (and maybe its buffer) changes
   std::vector<int4> ch;
   for (int g = 0; g < b->G; ++g) {
     const int n = b->leaf_base[g + 1] - b->leaf_base[g];
@@ -228,15 +247,25 @@ extern "C" int vr_batch_create(vr_ctx* ctx, vr_grid* const* grids, const vr_velo
     const vr_velocity* v = vels[i];
     if (!g || !v) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "null grid or velocity %d", i);
     if (g->topo->has_empty) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "grid %d has leaves without active voxels", i);
-    b->in.push_back(g);
+    const int n = g->topo->count;
+    std::unique_ptr<vr_grid> c(new vr_grid());
+    copy_header_into(c.get(), g);
+    c->topo = g->topo;
+    c->cap = n;
+    if ((e = hipMalloc(&c->d_leaves, sizeof(NanoLeaf) * (size_t)(n ? n : 1))) != hipSuccess)
+      return vr_hip_fail(ctx, e, "hipMalloc");
+    if (n && (e = hipMemcpyAsync(c->d_leaves, g->d_leaves, sizeof(NanoLeaf) * (size_t)n, hipMemcpyDeviceToDevice,
+                                 ctx->stream)) != hipSuccess)
+      return vr_hip_fail(ctx, e, "hipMemcpyAsync");
     b->vel.push_back(v);
     std::unique_ptr<vr_grid> o(new vr_grid());
     copy_header_into(o.get(), g);
     o->topo = g->topo;
-    o->cap = g->topo->count;
+    o->cap = n;
     if ((e = hipMalloc(&o->d_leaves, sizeof(NanoLeaf) * (size_t)(o->cap ? o->cap : 1))) != hipSuccess)
       return vr_hip_fail(ctx, e, "hipMalloc");
-    vin.push_back(make_view(g));
+    vin.push_back(make_view(c.get()));
+    b->in.push_back(std::move(c));
     vout.push_back(make_view(o.get()));
     vv.push_back(make_vel_view(v));
     outs.push_back(o->d_leaves);
@@ -275,6 +304,9 @@ extern "C" int vr_batch_create(vr_ctx* ctx, vr_grid* const* grids, const vr_velo
 
 extern "C" int vr_batch_set_dt(vr_ctx* ctx, vr_batch* b, const double* dts) {
   if (!ctx || !b) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "null argument");
+  // Captured graphs and mesh capacities hold for the old choice.
+  batch_drop_graph(b);
+  b->sized = false;
   if (!dts) {
     b->has_dts = false;
     return VR_OK;
@@ -286,10 +318,6 @@ extern "C" int vr_batch_set_dt(vr_ctx* ctx, vr_batch* b, const double* dts) {
       (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
     return vr_hip_fail(ctx, e, "hipMemcpy");
   b->has_dts = true;
-  if (b->exec) {  // captured graphs hold the old choice
-    (void)hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
-  }
   return VR_OK;
 }
 
@@ -357,6 +385,7 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
     if ((e = hipMemcpyAsync(th, b->mc.totals.p, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
       return vr_hip_fail(ctx, e, "readback");
+    if (th[1] + 1 > b->vcap || th[2] + 1 > b->tricap) batch_drop_graph(b);  // it writes the old buffers
     if (th[1] + 1 > b->vcap) {
       if (b->d_verts) (void)hipFree(b->d_verts);
       b->d_verts = nullptr;
@@ -403,7 +432,7 @@ extern "C" int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int 
       for (int i = 0; i < b->G; ++i) b->loop_mesh.emplace_back(new vr_mesh());
     }
     for (int i = 0; i < b->G; ++i) {
-      int rc = advect_into(ctx, b->in[i], b->vel[i], b->has_dts ? b->dts[i] : dt, t, scheme, b->out[i].get());
+      int rc = advect_into(ctx, b->in[i].get(), b->vel[i], b->has_dts ? b->dts[i] : dt, t, scheme, b->out[i].get());
       if (rc != VR_OK) return rc;
       if (b->out[i]->topo->count > 0) {
         MCScratch sc;
@@ -415,25 +444,25 @@ extern "C" int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int 
   }
   // Mesh capacities come from a host-synchronised run with the same
   // parameters (the tile count depends on the topology only, but vertex and
-  // triangle counts depend on dt, t and iso).
-  if (b->sized && b->exec &&
-      (b->g_dt != dt || b->g_t != t || b->g_scheme != scheme || b->g_iso != iso))
-    b->sized = false;
+  // triangle counts depend on dt, t, the scheme, iso and the per-grid dts).
+  if (b->sized && (b->s_dt != dt || b->s_t != t || b->s_scheme != scheme || b->s_iso != iso)) b->sized = false;
   if (!b->sized || mode == VR_BATCH_EAGER) {
+    b->sized = false;
     int rc = batch_advect(ctx, b, dt, t, scheme);
     if (rc == VR_OK) rc = batch_mc(ctx, b, iso, true);
     if (rc == VR_OK) rc = batch_read_ranges(ctx, b);
     if (rc != VR_OK) return rc;
     b->sized = true;
+    b->s_dt = dt;
+    b->s_t = t;
+    b->s_scheme = scheme;
+    b->s_iso = iso;
     if (mode == VR_BATCH_EAGER) return VR_OK;
   }
   // VR_BATCH_GRAPH: capture once per parameter set, then replay.
   if (!b->exec || b->g_dt != dt || b->g_t != t || b->g_scheme != scheme || b->g_iso != iso || b->g_lpg != b->lpg ||
       b->g_wpc != b->wpc) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    batch_drop_graph(b);
     const bool timing = ctx->timing;
     ctx->timing = false;  // no event records inside the captured graph
     if ((e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
@@ -523,8 +552,12 @@ extern "C" int vr_batch_mesh_counts(vr_ctx* ctx, vr_batch* b, int i, int mode, u
     int rc = batch_read_ranges(ctx, b);
     if (rc != VR_OK) return rc;
   }
-  if (nv) *nv = (uint64_t)b->ranges_host[4 * i + 1];
-  if (nt) *nt = (uint64_t)b->ranges_host[4 * i + 3];
+  const long long v0 = b->ranges_host[4 * i], t0 = b->ranges_host[4 * i + 2];
+  const long long vn = b->ranges_host[4 * i + 1], tn = b->ranges_host[4 * i + 3];
+  if (v0 < 0 || vn < 0 || t0 < 0 || tn < 0 || v0 + vn > b->vcap || t0 + tn > b->tricap)
+    return vr_fail(ctx, VR_ERR_CAPACITY, "mesh of grid %d exceeds the batch's mesh buffers", i);
+  if (nv) *nv = (uint64_t)vn;
+  if (nt) *nt = (uint64_t)tn;
   return VR_OK;
 }
 

@@ -221,6 +221,55 @@ extern "C" int vr_mesh_create(vr_ctx* ctx, vr_mesh** out) {
 }
 
 // ------------------------------------------------------------------ rollout
+// Grows mesh m to the largest vertex and triangle counts of meshes 0..last
+// (device counts d_mc, two per mesh; read back here, host-synchronised) and,
+// when mesh `last` was cut at the old capacity, meshes g again into the new
+// buffers.  Also makes g's host leaf count exact.
+static int rollout_fix_last_mesh(vr_ctx* ctx, vr_grid* g, float iso, vr_mesh* m, const int* d_mc, int last) {
+  hipError_t e;
+  std::vector<int> mc((size_t)2 * (last + 1), 0);
+  int cnt = 0;
+  if ((e = hipMemcpyAsync(mc.data(), d_mc, sizeof(int) * mc.size(), hipMemcpyDeviceToHost, ctx->stream)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(&cnt, g->topo->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return vr_hip_fail(ctx, e, "mesh counts");
+  g->topo->count = cnt;
+  uint64_t mv = 0, mt = 0;
+  for (int k = 0; k <= last; ++k) {
+    mv = std::max<uint64_t>(mv, (uint64_t)mc[2 * k]);
+    mt = std::max<uint64_t>(mt, (uint64_t)mc[2 * k + 1]);
+  }
+  const bool cut = (uint64_t)mc[2 * last] > m->cap_v || (uint64_t)mc[2 * last + 1] > m->cap_t;
+  if (mv > m->cap_v) {
+    if (m->d_v) (void)hipFree(m->d_v);
+    m->d_v = nullptr;
+    m->cap_v = 0;
+    if ((e = hipMalloc(&m->d_v, sizeof(double) * 3 * (size_t)(mv + 1))) != hipSuccess)
+      return vr_hip_fail(ctx, e, "hipMalloc mesh");
+    m->cap_v = mv + 1;
+  }
+  if (mt > m->cap_t) {
+    if (m->d_t) (void)hipFree(m->d_t);
+    m->d_t = nullptr;
+    m->cap_t = 0;
+    if ((e = hipMalloc(&m->d_t, sizeof(uint32_t) * 3 * (size_t)(mt + 1))) != hipSuccess)
+      return vr_hip_fail(ctx, e, "hipMalloc mesh");
+    m->cap_t = mt + 1;
+  }
+  if (!cut) return VR_OK;
+  MCScratch sc;
+  const int seg = ctx->roll_seg, msh = ctx->roll_mesh;
+  ctx->roll_seg = ctx->roll_mesh = -1;  // not part of the rollout's timed schedule
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  const int rc = mc_run(ctx, g, iso, &sc, m);
+  ctx->timing = timing;
+  ctx->roll_seg = seg;
+  ctx->roll_mesh = msh;
+  return rc;
+}
+
 extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, double t0, double dt, int scheme,
                           int steps, int rebuild_every, float half_width, int mesh_every, float iso, vr_mesh* mesh,
                           vr_rollout_stats* stats) {
@@ -277,6 +326,7 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
   const size_t pbuf_bytes = ping.bytes;
   NanoLeaf* other = pbuf;
   std::unique_ptr<vr_mesh> own_mesh;
+  bool last_mesh_fixed = false;  // the last mesh was checked (and regrown) where it was taken
   if (!mesh && mesh_every > 0) {
     own_mesh.reset(new vr_mesh());
     mesh = own_mesh.get();
@@ -343,6 +393,14 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
       }
       if (rc != VR_OK) break;
       st.meshes += 1;
+      if (imesh + 1 == nmesh && s + 1 < steps && mesh) {
+        // The last mesh is taken before the last step, so the final state is
+        // not the one it shows: check it now (one synchronisation) and, if it
+        // was cut at the capacity, grow the buffers and mesh this state again.
+        rc = rollout_fix_last_mesh(ctx, g, iso, mesh, mesh_counts.as<int>(), imesh);
+        if (rc != VR_OK) break;
+        last_mesh_fixed = true;
+      }
     }
   }
   timing_group_end(ctx);  // a run left open by an error
@@ -380,9 +438,13 @@ extern "C" int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, doubl
     r.bytes = r.per_leaf * leaves[r.seg] + mesh_bytes;
   }
   for (int k = 0; k <= seg; ++k) st.voxel_updates += act[k] * (uint64_t)seg_steps[k];
-  if (mesh && imesh >= 0) {
+  if (mesh && imesh >= 0 && last_mesh_fixed) {
+    mesh->nv = (uint64_t)mc[2 * imesh];
+    mesh->nt = (uint64_t)mc[2 * imesh + 1];
+  } else if (mesh && imesh >= 0) {
     // Meshes past the capacity were written up to it: grow the capacity to
-    // the largest count, and mesh the final state again if it was cut.
+    // the largest count, and mesh the final state (the last mesh's state:
+    // the last mesh is taken at the last step) again if it was cut.
     uint64_t mv = 0, mt = 0;
     for (int k = 0; k <= imesh; ++k) {
       mv = std::max<uint64_t>(mv, (uint64_t)mc[2 * k]);

@@ -195,3 +195,89 @@ def test_batch_mesh_large_tile_list(orc, ctx, vr, lo, hi):
         if i == 0:
             vw, tw, _ = orc.mesh(orc.from_nano(b.output_nano(0)), 0.0)
             assert np.array_equal(tg, tw) and np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), "oracle"
+
+
+def check_against_oracle(orc, b, inputs, spec, dts, iso, mode, what):
+    """Each output grid and mesh of the batch's last `mode` step equals the
+    oracle's RK3 advect of the input bytes with that grid's dt, then MC at iso."""
+    for i, (_, vel) in enumerate(spec):
+        want = orc.advect_analytic(orc.from_nano(inputs[i]), 2, vel, dts[i], 3)
+        assert_nano_equal(b.output_nano(i), want.to_nano(), f"{what}: grid {i}")
+        vw, tw, _ = orc.mesh(want, iso)
+        vg, tg = b.mesh(i, mode)
+        assert tg.shape == tw.shape and np.array_equal(tg, tw), f"{what}: grid {i} triangles"
+        assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"{what}: grid {i} vertices"
+
+
+def small_batch(ctx, vr, seed, count=4, vs=1.0 / 128):
+    spec = scenes(vr, np.random.default_rng(seed), count, vs)
+    grids = [ctx.from_primitives(p, vs, 3.0) for p, _ in spec]
+    vels = [ctx.velocity_analytic(vr.VEL_UNIFORM, v) for _, v in spec]
+    return spec, grids, vels, [g.download() for g in grids]
+
+
+def test_batch_graph_after_autotune(orc, ctx, vr):
+    """GRAPH at the default tuning, then the autotuner (whose one-leaf pass
+    grows the chunk table), then GRAPH again: the step never replays a graph
+    that reads the old chunk table."""
+    announce("batch graph after autotune")
+    vs = 1.0 / 128
+    spec, grids, vels, inputs = small_batch(ctx, vr, 31)
+    b = vr.Batch(ctx, grids, vels)
+    dt = 0.3 * vs
+    b.step(dt, 0.0, vr.RK3, 0.0, vr.BATCH_GRAPH)
+    b.autotune(dt, vr.RK3)
+    for tuning in [None, (4, 8), (1, 0), (4, 8)]:
+        if tuning:
+            b.set_tuning(*tuning)
+        b.step(dt, 0.0, vr.RK3, 0.0, vr.BATCH_GRAPH)
+        check_against_oracle(orc, b, inputs, spec, [dt] * len(spec), 0.0, vr.BATCH_GRAPH, f"tuning {tuning}")
+
+
+def test_batch_graph_eager_graph_with_larger_mesh(orc, ctx, vr):
+    """GRAPH(a), EAGER(b) with a larger mesh (reallocated buffers), GRAPH(a):
+    the last step captures again instead of writing the freed buffers, and
+    its mesh is whole."""
+    announce("batch graph/eager/graph")
+    vs = 1.0 / 128
+    spec, grids, vels, inputs = small_batch(ctx, vr, 37)
+    b = vr.Batch(ctx, grids, vels)
+    a = (0.3 * vs, 0.0)
+    for dt, iso, mode in [(*a, vr.BATCH_GRAPH), (0.6 * vs, 2.0 * vs, vr.BATCH_EAGER), (*a, vr.BATCH_GRAPH),
+                          (0.6 * vs, 2.0 * vs, vr.BATCH_GRAPH), (*a, vr.BATCH_EAGER), (*a, vr.BATCH_GRAPH)]:
+        b.step(dt, 0.0, vr.RK3, iso, mode)
+        check_against_oracle(orc, b, inputs, spec, [dt] * len(spec), iso, mode, f"dt={dt} iso={iso} mode={mode}")
+
+
+def test_batch_per_grid_dts_then_uniform(orc, ctx, vr):
+    """Per-grid dts through a graph, then set_dt(None): the next graph step
+    uses the step's dt for every grid, with a mesh sized for it."""
+    announce("batch per-grid dts then uniform")
+    vs = 1.0 / 128
+    spec, grids, vels, inputs = small_batch(ctx, vr, 41)
+    b = vr.Batch(ctx, grids, vels)
+    dts = [0.5 * vs / max(abs(c) for c in v) for _, v in spec]
+    b.set_dt(dts)
+    b.step(0.0, 0.0, vr.RK3, 0.0, vr.BATCH_GRAPH)
+    check_against_oracle(orc, b, inputs, spec, dts, 0.0, vr.BATCH_GRAPH, "per-grid dts")
+    b.set_dt(None)
+    b.step(0.2 * vs, 0.0, vr.RK3, 0.0, vr.BATCH_GRAPH)
+    check_against_oracle(orc, b, inputs, spec, [0.2 * vs] * len(spec), 0.0, vr.BATCH_GRAPH, "uniform dt")
+
+
+def test_batch_inputs_changed_after_create(orc, ctx, vr):
+    """The batch keeps its own copy of the inputs: rebuilding, trimming,
+    rolling out or destroying an input grid afterwards does not change the
+    batch's results."""
+    announce("batch inputs changed after create")
+    vs = 1.0 / 128
+    spec, grids, vels, inputs = small_batch(ctx, vr, 43)
+    b = vr.Batch(ctx, grids, vels)
+    dt = 0.3 * vs
+    grids[0].rebuild(1.5)
+    grids[1].trim(1.0)
+    grids[2].rollout(vels[2], 0.0, dt, vr.RK1, 3, 2, 3.0, 0, 0.0, None)
+    grids[3].close()
+    for mode in [vr.BATCH_EAGER, vr.BATCH_GRAPH, vr.BATCH_LOOP]:
+        b.step(dt, 0.0, vr.RK3, 0.0, mode)
+        check_against_oracle(orc, b, inputs, spec, [dt] * len(spec), 0.0, mode, f"mode {mode}")

@@ -427,6 +427,32 @@ def test_rollout_mesh_capacity_regrowth(orc, ctx, vr):
         assert (st.last_vertex_count, st.last_triangle_count) == (len(wmesh[0]), len(wmesh[1]))
 
 
+def test_rollout_last_mesh_before_last_step(orc, ctx, vr):
+    """Meshes at steps 2 and 4 of a 5-step rollout into a fresh mesh, whose
+    first capacity guess the mesh exceeds: the returned mesh is the state
+    after step 4 (meshed again at that step), not the final state."""
+    announce("rollout last mesh before last step")
+    og = sphere128(orc)
+    nano = og.to_nano()
+    leaves = (len(nano) - 320) // 2128
+    params = (0.0, 0.0, 2 * math.pi, 0.5, 0.5, 0.5)
+    dt = sphere128_rigid_dt()
+    steps, rb, me = 5, 2, 2
+    want, updates, wmesh = oracle_rollout(orc, og, 3, params, 0.0, 0.0, dt, 3, steps, rb, 3.0, me)
+    assert len(wmesh[0]) > 16 * max(leaves, 256)
+    vel = ctx.velocity_analytic(vr.VEL_RIGID, params, 0.0)
+    base = ctx.upload(nano)
+    for rep in range(2):
+        mesh = vr.Mesh(ctx)
+        g = base.clone()
+        st = g.rollout(vel, 0.0, dt, vr.RK3, steps, rb, 3.0, me, 0.0, mesh)
+        assert_nano_equal(g.download(), want.to_nano(), f"rollout rep {rep}")
+        assert st.voxel_updates == updates and st.meshes == 2
+        vg, tg = mesh.arrays()
+        mesh_equal(vg, tg, wmesh[0], wmesh[1], 1.0 / 128, f"last mesh rep {rep}")
+        assert (st.last_vertex_count, st.last_triangle_count) == (len(wmesh[0]), len(wmesh[1]))
+
+
 # ---------------------------------------------------------------- FP32 error against the FP64 oracle
 @pytest.mark.parametrize("kind,params,scheme,steps", [
     (3, (0, 0, 2 * math.pi, 0.5, 0.5, 0.5), 1, 1),   # rigid rotation, one RK1 step
