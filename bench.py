@@ -1,18 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the narrow-band level-set path on one MI355X.
 
-Workload (one bench step = one full rollout, inputs resident in HBM):
-Zalesak slotted sphere (r = 0.15 at (0.5, 0.75, 0.5), slot 0.05 wide, 0.125
-deep, cut by csg_difference), voxel size 1/512, band +-3 voxels, f32; single
-vortex velocity u = (sin^2(pi x) sin(2 pi y), -sin^2(pi y) sin(2 pi x), 0)
-* cos(pi t / 8); 100 RK3 steps at CFL 0.5 (dt = 0.5 vs, max |u| = 1), band
-rebuild every 5 steps, marching cubes every 10 steps.
+Headline workload (one bench step = one batched step, inputs resident in HBM):
+64 level sets, each the csg_union of 1..32 spheres with centres uniform in
+[0,1]^3 and radii log-uniform in [0.01, 0.2] (SplitMix64, seed 0), voxel size
+1/1024, band +-3 voxels, f32; each grid has its own uniform velocity with
+components uniform in [-1, 1] and its own dt = 0.5 vs / max|v_i|.  One step is
+RK3 advection of every active voxel of all 64 grids followed by marching cubes
+(iso 0) of all 64 results, replayed as one captured hipGraph (leaves per grab
+and waves per CU picked by the autotuner).
 
-Metric: active-voxel updates per second (sum over steps of the active voxels
-advected, divided by wall time of the rollouts), plus the advect kernel's
-achieved HBM bandwidth against the 8 TB/s roofline, measured with HIP events
-on the library's stream.  The CPU baseline is the C++ restatement of the
-reference's CPU path (oracle/), timed single-threaded on a bounded sample.
+Metric: active-voxel updates per second (active voxels of all grids times
+steps, divided by the wall time of the timed steps).  `roofline` is the
+dominant kernel's algorithmic bytes per launch over its average launch time,
+measured with HIP events on the library's stream (each launch of an eager
+step bracketed by its own events), against the 8 TB/s HBM peak; `roofline_step`
+is the whole step's algorithmic bytes (2128 B per leaf read by advection,
+2128 written, 2128 read by marching cubes, 24 B per vertex and 12 per
+triangle written) over the graph step time.  The CPU baseline is the C++
+restatement of the reference's CPU path (oracle/), timed single-threaded on a
+bounded sample of the same grids.  Side workloads (the Zalesak rollout, the
+differentiable rollouts, the per-grid loop) are reported under `extras`.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -26,15 +34,19 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK_GBPS = 8000.0
-
-VS = 1.0 / 512
+LEAF = 2128
 HW = 3.0
+
+# Zalesak rollout (side workload)
+VS_Z = 1.0 / 512
 PERIOD = 8.0
-STEPS = 100
+STEPS_Z = 100
 REBUILD_EVERY = 5
 MESH_EVERY = 10
 SPHERE = (0.5, 0.75, 0.5, 0.15)
 SLOT = (0.475, 0.55, 0.3, 0.525, 0.725, 0.7)
+
+VS_B = 1.0 / 1024  # batched headline
 
 
 def load_package():
@@ -52,13 +64,30 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per advect launch from the committed rocprofv3 counter summary, if any."""
-    p = os.path.join(ROOT, "profiles", "advect_vortex_slotted512_traffic.json")
+def profile_traffic(name):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 counter summary, if any."""
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("bytes_per_launch")
     return None
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"nproc": nproc, "model": model}
 
 
 class SplitMix64:
@@ -98,39 +127,130 @@ def batched_sphere_scenes(vr, count=64, seed=0):
     return out
 
 
-def batched_extra(vr, ctx, reps=3):
-    """64 sphere unions at 1/1024, one RK3 step + MC each: batched persistent graph step vs the
-    per-grid loop, one GPU."""
-    vs = 1.0 / 1024
+def kernel_times(vr, ctx):
+    """Per-class device time, launches and algorithmic bytes since timing was turned on."""
+    out = {}
+    for name, cls in [("advect", vr.TIMING_ADVECT), ("crossings", vr.TIMING_CROSSINGS),
+                      ("rebuild", vr.TIMING_REBUILD), ("compact", vr.TIMING_COMPACT),
+                      ("mc_count", vr.TIMING_MC), ("mc_emit", vr.TIMING_MC_EMIT)]:
+        ms, n, b = ctx.timing(cls)
+        if n:
+            out[name] = {"launches": n, "avg_us": round(1e3 * ms / n, 3),
+                         "algorithmic_bytes_per_launch": round(b / n, 1),
+                         "achieved_GBps": round((b / n) / (ms / n * 1e-3) / 1e9, 1),
+                         "frac": round((b / n) / (ms / n * 1e-3) / 1e9 / PEAK_GBPS, 4)}
+    return out
+
+
+KERNEL_NAMES = {"advect": "k_batch_advect", "mc_count": "k_mc_count", "mc_emit": "k_mc_emit"}
+
+
+class Batched:
+    """The headline workload, built once and reused by the timed steps."""
+
+    def __init__(self, vr, ctx):
+        self.vr, self.ctx = vr, ctx
+        t0 = time.perf_counter()
+        self.spec = batched_sphere_scenes(vr)
+        self.grids = [ctx.from_primitives(p, VS_B, HW) for p, _ in self.spec]
+        self.vels = [ctx.velocity_analytic(vr.VEL_UNIFORM, v) for _, v in self.spec]
+        self.dts = [0.5 * VS_B / max(abs(c) for c in v) for _, v in self.spec]
+        counts = [g.counts() for g in self.grids]
+        self.leaves = sum(l for l, _ in counts)
+        self.active = sum(a for _, a in counts)
+        self.per_grid_active = [a for _, a in counts]
+        log(f"batched sphere unions: 64 grids, {self.leaves} leaves, {self.active} active voxels "
+            f"(min {min(self.per_grid_active)}, max {max(self.per_grid_active)}), built in "
+            f"{time.perf_counter() - t0:.1f} s")
+        self.b = vr.Batch(ctx, self.grids, self.vels)
+        self.b.set_dt(self.dts)
+        self.lpg, self.wpc = self.b.autotune(self.dts[0], vr.RK3)
+        log(f"autotune: leaves_per_grab {self.lpg}, waves_per_cu {self.wpc}")
+
+    def step(self, mode):
+        self.b.step(0.0, 0.0, self.vr.RK3, 0.0, mode)
+
+    def timed(self, mode, steps, warmup):
+        for _ in range(max(1, warmup)):
+            self.step(mode)
+        self.ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step(mode)
+        self.ctx.synchronize()
+        return time.perf_counter() - t0
+
+    def mesh_totals(self, mode):
+        counts = [self.b.mesh_counts(i, mode) for i in range(len(self.grids))]
+        return sum(v for v, _ in counts), sum(t for _, t in counts)
+
+
+def cpu_baseline_batched(bt, seconds_budget):
+    """Oracle (literal C++ restatement of the reference CPU path) on a bounded
+    sample of the headline workload: grids 0, 1, 2, ... of the 64, one RK3
+    advect + marching cubes each, single thread, until the budget is spent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    total_s, updates, used = 0.0, 0, []
+    for i, (g, (_, vel)) in enumerate(zip(bt.grids, bt.spec)):
+        if total_s >= seconds_budget:
+            break
+        og = orc.from_nano(g.download())
+        a = og.active_count()
+        total_s += orc.time_step(og, 2, vel, bt.dts[i], 3, (1.0, 1.0, 1.0), False, HW, True, 0.0)
+        updates += a
+        used.append(i)
+        del og
+    return {
+        "value": round(updates / total_s, 1),
+        "unit": "active-voxel updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"grids {used[0]}..{used[-1]} of the 64 ({updates} active voxels of {bt.active}), one RK3 "
+                  f"advect + marching cubes each, {total_s:.1f} s of single-thread CPU time",
+        "host": host_cpu(),
+    }
+
+
+def rollout_extra(vr, ctx, steps=5, warmup=2):
+    """Zalesak slotted sphere (r = 0.15 at (0.5, 0.75, 0.5), slot 0.05 wide,
+    0.125 deep), 1/512, band 3, single vortex with cos(pi t / 8), 100 RK3
+    steps at CFL 0.5 (dt = 0.5 vs), rebuild every 5, marching cubes every 10;
+    one step = one rollout."""
+    prims = [(vr.PRIM_SPHERE, 0, SPHERE), (vr.PRIM_BOX, vr.CSG_DIFFERENCE, SLOT)]
+    base = ctx.from_primitives(prims, VS_Z, HW)
+    leaves0, active0 = base.counts()
+    vel = ctx.velocity_analytic(vr.VEL_VORTEX, (), PERIOD)
+    dt = 0.5 * VS_Z
+    mesh = vr.Mesh(ctx)
+    grids = [base.clone() for _ in range(warmup + steps + 1)]
+    ctx.synchronize()
+
+    def rollout(g):
+        return g.rollout(vel, 0.0, dt, vr.RK3, STEPS_Z, REBUILD_EVERY, HW, MESH_EVERY, 0.0, mesh)
+
+    for i in range(warmup):
+        rollout(grids[i])
+    ctx.synchronize()
+    updates, st = 0, None
     t0 = time.perf_counter()
-    spec = batched_sphere_scenes(vr)
-    grids = [ctx.from_primitives(p, vs, HW) for p, _ in spec]
-    vels = [ctx.velocity_analytic(vr.VEL_UNIFORM, v) for _, v in spec]
-    dts = [0.5 * vs / max(abs(c) for c in v) for _, v in spec]
-    counts = [g.counts() for g in grids]
-    active = sum(a for _, a in counts)
-    log(f"batched sphere unions: 64 grids, {sum(l for l, _ in counts)} leaves, {active} active voxels "
-        f"(min {min(a for _, a in counts)}, max {max(a for _, a in counts)}), built in {time.perf_counter() - t0:.1f} s")
-    b = vr.Batch(ctx, grids, vels)
-    b.set_dt(dts)
-    lpg, wpc = b.autotune(dts[0], vr.RK3)
-    res = {"grids": len(grids), "active_voxels": active, "leaves_per_grab": lpg, "waves_per_cu": wpc}
-    for name, mode in [("loop", vr.BATCH_LOOP), ("graph", vr.BATCH_GRAPH)]:
-        b.step(0.0, 0.0, vr.RK3, 0.0, mode)  # warm (sizes, captures)
-        ctx.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(reps):
-            b.step(0.0, 0.0, vr.RK3, 0.0, mode)
-        ctx.synchronize()
-        dtw = (time.perf_counter() - t1) / reps
-        res[f"{name}_ms"] = round(1e3 * dtw, 3)
-        res[f"{name}_updates_per_s"] = round(active / dtw, 1)
-    res["speedup_graph_vs_loop"] = round(res["loop_ms"] / res["graph_ms"], 3)
-    counts = [b.mesh_counts(i) for i in range(len(grids))]
-    res["vertices"] = sum(v for v, _ in counts)
-    res["triangles"] = sum(t for _, t in counts)
-    b.close()
-    return res
+    for i in range(steps):
+        st = rollout(grids[warmup + i])
+        updates += st.voxel_updates
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.set_timing(True)
+    rollout(grids[-1])
+    kern = kernel_times(vr, ctx)
+    ctx.set_timing(False)
+    adv = kern["advect"]
+    return {"leaves": leaves0, "active_voxels": active0, "voxel_updates_per_rollout": st.voxel_updates,
+            "ms_per_rollout": round(1e3 * wall / steps, 4), "updates_per_s": round(updates / wall, 1),
+            "final_leaf_count": st.final_leaf_count, "final_triangles": st.last_triangle_count,
+            "advect_roofline": {"achieved": adv["achieved_GBps"], "frac": adv["frac"],
+                                "traffic": profile_traffic("advect_vortex_slotted512_traffic.json")},
+            "kernels": kern}
 
 
 def adjoint_extra(vr, ctx):
@@ -184,44 +304,16 @@ def adjoint_noisy_extra(vr, ctx, count=8):
             "forward_ms": round(fwd, 3), "adjoint_ms": round(adj, 3), "adjoint_over_forward": round(adj / fwd, 3)}
 
 
-def cpu_baseline(seconds_budget):
-    """Oracle (literal C++ restatement of the reference CPU path) on a bounded
-    sample of the same workload: the first steps of the rollout, single thread."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
-
-    g = orc.scene([(0, SPHERE), (1, SLOT)], [0, 2], VS, HW)
-    dt = 0.5 * VS
-    total_s, updates, steps = 0.0, 0, 0
-    while total_s < seconds_budget and steps < STEPS:
-        a = g.active_count()
-        tf = orc.stage_factors(PERIOD, steps * dt, dt, 3)
-        do_rb = (steps + 1) % REBUILD_EVERY == 0
-        do_mc = (steps + 1) % MESH_EVERY == 0
-        total_s += orc.time_step(g, 4, (), dt, 3, tf, do_rb, HW, do_mc, 0.0)
-        updates += a
-        steps += 1
-    return {
-        "value": updates / total_s,
-        "unit": "active-voxel updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"first {steps} of {STEPS} rollout steps (RK3 advect, rebuild every {REBUILD_EVERY} in the "
-                  f"bounded-radius form proven equal to the brute force, MC every {MESH_EVERY}) on the same "
-                  f"slotted sphere, {total_s:.1f} s of single-thread CPU time",
-    }
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint lines")
-    ap.add_argument("--only", choices=["batched", "adjoint", "adjoint_noisy"],
-                    help="run one side workload alone (for profiling)")
+    ap.add_argument("--no-extras", action="store_true", help="skip the side workloads")
+    ap.add_argument("--only", choices=["batched", "rollout", "adjoint", "adjoint_noisy"],
+                    help="run one workload alone (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -229,81 +321,67 @@ def main():
 
     vr = load_package()
     ctx = vr.Context(0)
-    if args.only:
-        res = {"batched": batched_extra, "adjoint": adjoint_extra, "adjoint_noisy": adjoint_noisy_extra}[args.only](vr, ctx)
+    if args.only and args.only != "batched":
+        res = {"rollout": rollout_extra, "adjoint": adjoint_extra, "adjoint_noisy": adjoint_noisy_extra}[args.only](
+            vr, ctx)
         print(json.dumps({args.only: res}), flush=True)
         ctx.close()
         return
-    prims = [(vr.PRIM_SPHERE, 0, SPHERE), (vr.PRIM_BOX, vr.CSG_DIFFERENCE, SLOT)]
-    base = ctx.from_primitives(prims, VS, HW)
-    leaves0, active0 = base.counts()
-    log(f"slotted sphere at 1/512: {leaves0} leaves, {active0} active voxels")
-    vel = ctx.velocity_analytic(vr.VEL_VORTEX, (), PERIOD)
-    dt = 0.5 * VS
-    mesh = vr.Mesh(ctx)
-    grids = [base.clone() for _ in range(args.warmup + args.steps + 1)]
-    ctx.synchronize()
 
-    def rollout(g):
-        return g.rollout(vel, 0.0, dt, vr.RK3, STEPS, REBUILD_EVERY, HW, MESH_EVERY, 0.0, mesh)
+    bt = Batched(vr, ctx)
+    wall = bt.timed(vr.BATCH_GRAPH, args.steps, args.warmup)
+    ms_step = 1e3 * wall / args.steps
+    nv, ntri = bt.mesh_totals(vr.BATCH_GRAPH)
+    log(f"timed {args.steps} graph steps in {wall:.4f} s ({ms_step:.3f} ms/step); mesh {nv} vertices, {ntri} triangles")
+    if args.only == "batched":
+        # A plain copy of known size (a clone of the largest grid: k_copy_u4 over its leaf records) for
+        # scaling the FETCH_SIZE / WRITE_SIZE counters of counter runs to bytes.
+        big = max(range(len(bt.grids)), key=lambda i: bt.per_grid_active[i])
+        copy_bytes = LEAF * bt.grids[big].counts()[0]
+        bt.grids[big].clone()
+        ctx.synchronize()
+        print(json.dumps({"batched_graph_ms": round(ms_step, 4), "calibration_copy_bytes": copy_bytes}), flush=True)
+        ctx.close()
+        return
 
-    for i in range(args.warmup):
-        rollout(grids[i])
-    ctx.synchronize()
-    log(f"warmup done ({args.warmup})")
-    updates = 0
-    stats = None
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        stats = rollout(grids[args.warmup + i])
-        updates += stats.voxel_updates
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    log(f"timed {args.steps} rollouts in {wall:.4f} s; last: {stats.as_dict()}")
-
-    # Per-kernel device time with HIP events on the library stream (one more rollout).
+    # Per-kernel device time: the same kernels launched eagerly, each launch
+    # between its own pair of HIP events on the library stream.
     ctx.set_timing(True)
-    rollout(grids[-1])
-    kern = {}
-    for name, cls in [("advect", vr.TIMING_ADVECT), ("crossings", vr.TIMING_CROSSINGS),
-                      ("rebuild", vr.TIMING_REBUILD), ("marching_cubes", vr.TIMING_MC)]:
-        ms, n, b = ctx.timing(cls)
-        if n:
-            kern[name] = {"launches": n, "avg_us": 1e3 * ms / n, "algorithmic_bytes_per_launch": b / n,
-                          "achieved_GBps": (b / n) / (ms / n * 1e-3) / 1e9}
+    for _ in range(3):
+        bt.step(vr.BATCH_EAGER)
+    kern = kernel_times(vr, ctx)
     ctx.set_timing(False)
-    adv = kern["advect"]
-    traffic = traffic_from_profiles()
-    roofline = {"bound": "hbm", "achieved": round(adv["achieved_GBps"], 1), "peak": PEAK_GBPS, "unit": "GB/s",
-                "frac": round(adv["achieved_GBps"] / PEAK_GBPS, 4), "traffic": traffic}
     log(f"kernels: {json.dumps(kern)}")
+    dom = max(("advect", "mc_count", "mc_emit"), key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
+    dk = kern[dom]
+    roofline = {"bound": "hbm", "achieved": dk["achieved_GBps"], "peak": PEAK_GBPS, "unit": "GB/s",
+                "frac": dk["frac"], "traffic": profile_traffic(f"{KERNEL_NAMES[dom]}_batched_sphere_unions_64x1024_traffic.json"),
+                "kernel": KERNEL_NAMES[dom]}
+    step_bytes = 3.0 * LEAF * bt.leaves + 24.0 * nv + 12.0 * ntri
+    roofline_step = {"algorithmic_bytes": step_bytes, "ms": round(ms_step, 4),
+                     "achieved": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
+                     "frac": round(step_bytes / (ms_step * 1e-3) / 1e9 / PEAK_GBPS, 4)}
 
     extras = {}
     if not args.no_extras:
-        try:
-            extras["adjoint_sphere256_dense64"] = adjoint_extra(vr, ctx)
-            log(f"adjoint: {extras['adjoint_sphere256_dense64']}")
-        except Exception as e:  # reported, not fatal for the headline line
-            extras["adjoint_sphere256_dense64"] = {"error": str(e)}
-        try:
-            extras["adjoint_8_noisy_spheres_2048"] = adjoint_noisy_extra(vr, ctx)
-            log(f"adjoint noisy: {extras['adjoint_8_noisy_spheres_2048']}")
-        except Exception as e:
-            extras["adjoint_8_noisy_spheres_2048"] = {"error": str(e)}
-        try:
-            extras["batched_64_sphere_unions_1024"] = batched_extra(vr, ctx)
-            log(f"batched: {extras['batched_64_sphere_unions_1024']}")
-        except Exception as e:
-            extras["batched_64_sphere_unions_1024"] = {"error": str(e)}
+        loop_wall = bt.timed(vr.BATCH_LOOP, 3, 1)
+        extras["per_grid_loop"] = {"ms_per_step": round(1e3 * loop_wall / 3, 3),
+                                   "graph_speedup": round(loop_wall / 3 / (wall / args.steps), 3)}
+        log(f"loop: {extras['per_grid_loop']}")
+        for name, fn in [("rollout_zalesak512", rollout_extra), ("adjoint_sphere256_dense64", adjoint_extra),
+                         ("adjoint_8_noisy_spheres_2048", adjoint_noisy_extra)]:
+            try:
+                extras[name] = fn(vr, ctx)
+            except Exception as e:  # reported, not fatal for the headline line
+                extras[name] = {"error": str(e)}
+            log(f"{name}: {extras[name]}")
 
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(args.cpu_seconds)
-        cpu["value"] = round(cpu["value"], 1)
+        cpu = cpu_baseline_batched(bt, args.cpu_seconds)
 
-    value = updates / wall
+    value = bt.active * args.steps / wall
     out = {
         "metric": "active-voxel updates/s (advect+MC)",
         "value": round(value, 1),
@@ -311,19 +389,21 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (slotted sphere generated on the device)",
+        "data": "synthetic (sphere unions generated on the device from SplitMix64 seed 0)",
         "config": {
-            "workload": "Zalesak slotted sphere r=0.15, vs=1/512, band 3, single vortex T=8, "
-                        "100 RK3 steps at CFL 0.5, rebuild every 5, marching cubes every 10; one step = one rollout",
-            "leaves": leaves0, "active_voxels": active0, "voxel_updates_per_rollout": stats.voxel_updates,
-            "final_triangles": stats.last_triangle_count, "parallelism": "single GPU",
+            "workload": "64 sphere unions (1-32 spheres each, radii log-uniform in [0.01, 0.2]), vs=1/1024, band 3, "
+                        "uniform velocity per grid, dt = 0.5 vs / max|v|; one step = RK3 advect + marching cubes of "
+                        "all 64 grids as one hipGraph replay",
+            "grids": len(bt.grids), "leaves": bt.leaves, "active_voxels": bt.active, "vertices": nv,
+            "triangles": ntri, "leaves_per_grab": bt.lpg, "waves_per_cu": bt.wpc, "parallelism": "single GPU",
         },
         "roofline": roofline,
+        "roofline_step": roofline_step,
         "cpu_baseline": cpu,
         "kernels": kern,
         "extras": extras,

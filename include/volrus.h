@@ -218,14 +218,16 @@ int vr_batch_output(vr_batch* b, int i, const vr_grid** out);
 int vr_batch_destroy(vr_ctx* ctx, vr_batch* b);
 
 /* ---- measurement -------------------------------------------------------- */
-/* Kernel classes for vr_ctx_timing (GRAPH: whole batch graph launches). */
+/* Kernel classes for vr_ctx_timing (GRAPH: whole batch graph launches; MC:
+ * the marching-cubes count pass, MC_EMIT: its vertex and triangle pass). */
 enum {
   VR_TIMING_ADVECT = 0,
   VR_TIMING_CROSSINGS = 1,
   VR_TIMING_REBUILD = 2,
   VR_TIMING_COMPACT = 3,
   VR_TIMING_MC = 4,
-  VR_TIMING_GRAPH = 5
+  VR_TIMING_GRAPH = 5,
+  VR_TIMING_MC_EMIT = 6
 };
 /* Turns HIP-event timing on the context stream on or off and clears the
  * totals.  Each launch is bracketed by its own pair of events, except the

@@ -103,6 +103,7 @@ TIMING_CROSSINGS = 1
 TIMING_REBUILD = 2
 TIMING_COMPACT = 3
 TIMING_MC = 4
+TIMING_MC_EMIT = 6
 
 
 class AdjointStats(ctypes.Structure):

@@ -380,11 +380,13 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
   }
   rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->ltot, b->tcap, iso);
   if (rc != VR_OK) return rc;
+  double mesh_bytes = 0.0;  // known to the host after a sizing run only (timing is off in graphs)
   if (sizing) {
     int th[3];
     if ((e = hipMemcpyAsync(th, b->mc.totals.p, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
       return vr_hip_fail(ctx, e, "readback");
+    mesh_bytes = 24.0 * th[1] + 12.0 * th[2];
     if (th[1] + 1 > b->vcap || th[2] + 1 > b->tricap) batch_drop_graph(b);  // it writes the old buffers
     if (th[1] + 1 > b->vcap) {
       if (b->d_verts) (void)hipFree(b->d_verts);
@@ -401,8 +403,8 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
       b->tricap = th[2] + 1;
     }
   }
-  return mc_emit_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->G, b->tcap, iso, b->d_verts, b->d_tris, 0.0,
-                       (long long)b->vcap, (long long)b->tricap);
+  return mc_emit_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->G, b->tcap, iso, b->d_verts, b->d_tris,
+                       (double)b->ltot, mesh_bytes, (long long)b->vcap, (long long)b->tricap);
 }
 
 static int batch_read_ranges(vr_ctx* ctx, vr_batch* b) {

@@ -61,7 +61,8 @@ struct MCMulti {
 };
 
 // Kernel classes timed with HIP events when timing is on.
-enum TimingClass { TC_ADVECT = 0, TC_CROSSINGS, TC_REBUILD, TC_COMPACT, TC_MC, TC_GRAPH, TC_COUNT };
+// TC_MC: the marching-cubes count pass; TC_MC_EMIT: its emit pass.
+enum TimingClass { TC_ADVECT = 0, TC_CROSSINGS, TC_REBUILD, TC_COMPACT, TC_MC, TC_GRAPH, TC_MC_EMIT, TC_COUNT };
 
 // bytes = per_leaf * (leaf count) + fixed.  A rollout keeps leaf counts on
 // the device; its records carry the segment (and, for mesh writes, the mesh)
@@ -95,9 +96,9 @@ struct vr_ctx {
   int group_cls = -1, group_n = 0;
   hipEvent_t group_a = nullptr;
   double group_per_leaf = 0.0, group_fixed = 0.0, group_leaves = 0.0;
-  double t_ms[TC_COUNT] = {0, 0, 0, 0, 0, 0};
-  double t_bytes[TC_COUNT] = {0, 0, 0, 0, 0, 0};
-  uint64_t t_n[TC_COUNT] = {0, 0, 0, 0, 0, 0};
+  double t_ms[TC_COUNT] = {};
+  double t_bytes[TC_COUNT] = {};
+  uint64_t t_n[TC_COUNT] = {};
   ~vr_ctx();
 };
 
@@ -184,8 +185,10 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
 int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
                     int tcap);
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso);
+// Timed (class TC_MC_EMIT) with algorithmic bytes 2128 * leaf_count (the
+// leaves it reads) + mesh_bytes (24 per vertex + 12 per triangle written).
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int tcap, float iso, double* verts,
-                  uint32_t* tris, double algorithmic_bytes, long long vcap = -1, long long tricap = -1,
+                  uint32_t* tris, double leaf_count, double mesh_bytes, long long vcap = -1, long long tricap = -1,
                   int* counts_out = nullptr);
 // active_out (optional): receives += the active voxels after the rebuild.
 int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact = true,

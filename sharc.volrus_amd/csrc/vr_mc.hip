@@ -991,12 +991,13 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long l
 // Phase D: emit into verts/tris (capacities from the counts), and the
 // per-grid ranges (s.ranges; counts_out, optional: the totals).
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int tcap, float iso, double* verts,
-                  uint32_t* tris, double algorithmic_bytes, long long vcap, long long tricap, int* counts_out) {
+                  uint32_t* tris, double leaf_count, double mesh_bytes, long long vcap, long long tricap,
+                  int* counts_out) {
   hipError_t e;
   if (vcap < 0) vcap = 0x7fffffffLL;
   if (tricap < 0) tricap = 0x7fffffffLL;
   {
-    TimedLaunch tl(ctx, TC_MC, 0.0, algorithmic_bytes, 0.0, true);
+    TimedLaunch tl(ctx, TC_MC_EMIT, (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
     hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, (const void*)k_mc_emit, MC_EWAVES, tcap)), dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(), iso,
                        s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
@@ -1038,7 +1039,8 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
       m->cap_v = cv;
       m->cap_t = 2 * cv;
     }
-    return mc_emit_multi(ctx, s, s.views.as<GridView>(), 1, tcap, iso, m->d_v, m->d_t, 0.0, (long long)m->cap_v,
+    return mc_emit_multi(ctx, s, s.views.as<GridView>(), 1, tcap, iso, m->d_v, m->d_t, (double)n, 0.0,
+                         (long long)m->cap_v,
                          (long long)m->cap_t, d_counts);
   }
   int nt = 0;
@@ -1070,5 +1072,6 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
       return vr_hip_fail(ctx, e, "hipMalloc mesh");
     m->cap_t = (uint64_t)th[2] + 1;
   }
-  return mc_emit_multi(ctx, s, s.views.as<GridView>(), 1, nt, iso, m->d_v, m->d_t, 24.0 * th[1] + 12.0 * th[2]);
+  return mc_emit_multi(ctx, s, s.views.as<GridView>(), 1, nt, iso, m->d_v, m->d_t, (double)n,
+                       24.0 * th[1] + 12.0 * th[2]);
 }
