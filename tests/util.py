@@ -107,3 +107,34 @@ def mesh_equal(vg, tg, vw, tw, vs, what=""):
         err = np.abs(vg - vw).max()
         assert err <= 1e-5 * vs, f"{what}: vertex error {err}"
     return float(np.abs(vg - vw).max()) if len(vw) else 0.0
+
+
+def config4_problem(orc, seed=0):
+    """The differentiable-rollout workload: sphere r = 0.3 at 1/256, target the
+    sphere moved by (0.1, 0, 0), dense 64^3 field of N(0, 0.5^2) (numpy
+    default_rng(seed)), dt = 0.5 vs / max|u|, 20 RK1 steps."""
+    vs, n = 1.0 / 256, 64
+    u = np.random.default_rng(seed).normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32)
+    g0 = orc.sphere(0.3, (0.5, 0.5, 0.5), vs, 3.0)
+    tg = orc.sphere(0.3, (0.6, 0.5, 0.5), vs, 3.0)
+    dt = 0.5 * vs / float(np.abs(u).max())
+    return u, g0, tg, dt, n, 20
+
+
+def config4_fd64(args):
+    """Central difference (FP64 oracle loss) of the config-4 loss in one
+    velocity entry: (L(u + h e_i) - L(u - h e_i)) / 2h.  Runs in a worker
+    process (CPU only)."""
+    index, h = args
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import oracle as orc
+    u, g0, tg, dt, n, steps = config4_problem(orc)
+    u64 = u.astype(np.float64).reshape(-1)
+    up, dn = u64.copy(), u64.copy()
+    up[index] += h
+    dn[index] -= h
+    lu = orc.rollout_loss_f64(g0, up, n, 1.0 / n, dt, 1, steps, tg)
+    ld = orc.rollout_loss_f64(g0, dn, n, 1.0 / n, dt, 1, steps, tg)
+    return (lu - ld) / (2 * h)
