@@ -135,6 +135,21 @@ int vr_mesh_destroy(vr_ctx* ctx, vr_mesh* m);
 /* An empty mesh object for vr_rollout to fill (and reuse across calls). */
 int vr_mesh_create(vr_ctx* ctx, vr_mesh** out);
 
+/* No counterpart in the reference: volume_to_mesh for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  It calls no
+ * hipMalloc or hipFree and synchronises nothing: its scratch is a fixed
+ * array in device memory (about 27 MB per device, held until the process
+ * ends) for grids of at most VR_NOALLOC_MAX_LEAVES leaves (VR_ERR_CAPACITY
+ * above that).  The caller passes device buffers: xyz (3 * vcap doubles),
+ * tri (3 * tcap uint32) and counts (2 int32: vertices, triangles, written on
+ * the context's stream).  Vertices and triangles past vcap / tcap are not
+ * written, the counts are always the full ones; the output equals
+ * vr_marching_cubes.  Calls that overlap on one device (two contexts on
+ * different streams) share the array and race. */
+#define VR_NOALLOC_MAX_LEAVES 4096
+int vr_marching_cubes_noalloc(vr_ctx* ctx, const vr_grid* g, float iso, double* xyz, uint64_t vcap, uint32_t* tri,
+                              uint64_t tcap, int32_t* counts);
+
 /* ---- rollout ------------------------------------------------------------ */
 /* No counterpart in the reference (nothing there chains its tools).  Runs
  * `steps` advection steps from time t0 (step s starts at t0 + s * dt), calls

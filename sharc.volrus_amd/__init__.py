@@ -75,6 +75,8 @@ _SIGS = {
     "vr_mesh_copy": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t]),
     "vr_mesh_destroy": (ctypes.c_int, [_P, _P]),
     "vr_mesh_create": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "vr_marching_cubes_noalloc": (ctypes.c_int, [_P, _P, ctypes.c_float, _P, ctypes.c_uint64, _P, ctypes.c_uint64,
+                                                 _P]),
     "vr_grid_clone": (ctypes.c_int, [_P, _P, ctypes.POINTER(_P)]),
     "vr_batch_create": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(_P)]),
     "vr_batch_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),

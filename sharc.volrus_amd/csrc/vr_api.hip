@@ -36,10 +36,11 @@ int vr_hip_fail(vr_ctx* ctx, hipError_t e, const char* where) {
 
 // ------------------------------------------------------------------ device buffers
 DevBuf::~DevBuf() {
-  if (p) (void)hipFree(p);
+  if (p && owned) (void)hipFree(p);
 }
 hipError_t DevBuf::ensure(size_t n) {
   if (n <= bytes && p) return hipSuccess;
+  if (!owned) return hipErrorOutOfMemory;  // borrowed memory is never reallocated
   if (p) {
     (void)hipFree(p);
     p = nullptr;
@@ -143,7 +144,7 @@ extern "C" int vr_ctx_create(int device, vr_ctx** out) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) c->cus = cus;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return VR_ERR_HIP;
-  if (upload_tables_once(device) != hipSuccess) {
+  if (upload_tables_once(device) != hipSuccess || mc_noalloc_setup(device) != hipSuccess) {
     (void)hipStreamDestroy(c->stream);
     return VR_ERR_HIP;
   }

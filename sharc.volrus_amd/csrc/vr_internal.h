@@ -12,15 +12,23 @@
 
 #define NANO_HEADER_BYTES 320
 
-// A growable device allocation reused across calls.
+// A growable device allocation reused across calls, or (borrow) a fixed
+// range of memory owned elsewhere, which ensure() never reallocates: it
+// fails with hipErrorOutOfMemory when the range is too small.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
   hipError_t ensure(size_t n);
+  void borrow(void* ptr, size_t n) {
+    p = ptr;
+    bytes = n;
+    owned = false;
+  }
   template <class T>
   T* as() const { return static_cast<T*>(p); }
 };
@@ -180,6 +188,9 @@ int crossings_into(vr_ctx* ctx, const vr_grid* g, uint64_t** cross, int** any, i
 // d_counts[0..1] (vertices, triangles) instead of being read back.
 int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* s, vr_mesh* m, bool deferred = false,
            int* d_counts = nullptr);
+// Resolves the device address of the allocation-free marching-cubes scratch
+// on `device` (the current device) once per device; vr_ctx_create calls it.
+hipError_t mc_noalloc_setup(int device);
 int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
                    const GridView* single = nullptr);
 int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,

@@ -21,6 +21,8 @@
 // writes vertices in the owner's edge orientation and triangles in
 // TRI_TABLE order at the scanned offsets.  The result is the reference's
 // vertex and triangle lists, bit for bit.
+#include <mutex>
+
 #include "vr_internal.h"
 
 // ------------------------------------------------------------------ tables
@@ -1074,4 +1076,105 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
   }
   return mc_emit_multi(ctx, s, s.views.as<GridView>(), 1, nt, iso, m->d_v, m->d_t, (double)n,
                        24.0 * th[1] + 12.0 * th[2]);
+}
+
+// ------------------------------------------------------------------ allocation-free entry point
+// vr_marching_cubes_noalloc keeps the passes' scratch (tile lists,
+// neighbour slots, face meta, per-tile counts, scan partials) in one
+// file-scope device array laid out for grids of up to VR_NOALLOC_MAX_LEAVES
+// leaves (7 tiles per leaf at most), instead of context buffers that grow
+// with hipMalloc.  Each device holds its own copy until the process ends;
+// its address comes from hipGetSymbolAddress, taken once per device.
+#define VR_NOALLOC_ALIGN 256
+
+// Byte offsets of the MCMulti ranges for n leaves (tile capacity 7 n); the
+// same function sizes the array and carves it.
+struct MCNoAllocLayout {
+  size_t off[18];
+  size_t len[18];
+  size_t total;
+};
+static constexpr size_t vr_align_up(size_t x) { return (x + VR_NOALLOC_ALIGN - 1) / VR_NOALLOC_ALIGN * VR_NOALLOC_ALIGN; }
+static constexpr size_t vr_scan_ints(size_t n) { return (n + 1023) / 1024 + 1; }  // vr_scan_scratch_ints
+static constexpr MCNoAllocLayout mc_noalloc_layout(size_t n) {
+  MCNoAllocLayout L{};
+  const size_t n7 = 7 * n, tc = 7 * (n < 1 ? 1 : n) + 1;
+  const size_t len[18] = {sizeof(GridView), 2 * sizeof(int), sizeof(int) * (n7 + 1), sizeof(int) * (n7 + 1),
+                          sizeof(int) * (vr_scan_ints(n7) + 1), sizeof(int) * (vr_scan_ints(tc) + 1), 4 * sizeof(int),
+                          2 * sizeof(int), 16 * tc, sizeof(int) * tc, sizeof(int) * VR_NBR * tc,
+                          sizeof(int) * VR_NBR * tc, sizeof(uint32_t) * MC_FACE * tc, sizeof(int) * tc,
+                          sizeof(int) * tc, sizeof(int) * tc, sizeof(int) * tc, 8 * sizeof(int)};
+  size_t at = 0;
+  for (int i = 0; i < 18; ++i) {
+    L.off[i] = at;
+    L.len[i] = vr_align_up(len[i]);
+    at += L.len[i];
+  }
+  L.total = at;
+  return L;
+}
+static constexpr size_t VR_NOALLOC_BYTES = mc_noalloc_layout(VR_NOALLOC_MAX_LEAVES).total;
+
+static __device__ __align__(256) unsigned char scratch[VR_NOALLOC_BYTES];
+
+#define VR_NOALLOC_DEVICES 64
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[VR_NOALLOC_DEVICES] = {};
+
+hipError_t mc_noalloc_setup(int device) {
+  if (device < 0 || device >= VR_NOALLOC_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  g_noalloc_ptr[device] = ptr;
+  return hipSuccess;
+}
+
+static void* mc_noalloc_ptr(int device) {
+  if (device < 0 || device >= VR_NOALLOC_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  return g_noalloc_ptr[device];
+}
+
+extern "C" int vr_marching_cubes_noalloc(vr_ctx* ctx, const vr_grid* g, float iso, double* xyz, uint64_t vcap,
+                                         uint32_t* tri, uint64_t tcap, int32_t* counts) {
+  if (!ctx || !g || !counts || (vcap && !xyz) || (tcap && !tri))
+    return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "null argument");
+  const int n = g->topo->count;
+  if (n > VR_NOALLOC_MAX_LEAVES)
+    return vr_fail(ctx, VR_ERR_CAPACITY, "grid has %d leaves; the allocation-free marching cubes takes at most %d", n,
+                   VR_NOALLOC_MAX_LEAVES);
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
+  unsigned char* base = static_cast<unsigned char*>(mc_noalloc_ptr(ctx->device));
+  if (!base) return vr_fail(ctx, VR_ERR_HIP, "no scratch address for device %d", ctx->device);
+  if (n == 0) {  // no tiles: the counts are zero (a fill kernel, no memset node)
+    e = vr_fill_int(counts, 0, 2, ctx->stream);
+    return e == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "fill");
+  }
+  // The passes' buffers as borrowed ranges of this device's array, laid out
+  // for this grid's leaf count (every range within the array: n <= max).
+  constexpr int kSlots = 18;
+  const MCNoAllocLayout L = mc_noalloc_layout((size_t)n);
+  MCMulti s;
+  DevBuf* slot[kSlots] = {&s.views, &s.leaf_base, &s.keep, &s.pos, &s.sums, &s.sums2, &s.totals, &s.tile_base,
+                          &s.tiles, &s.tanchor, &s.tleaf, &s.ttile, &s.meta, &s.tv, &s.tt, &s.vb, &s.tb, &s.ranges};
+  for (int i = 0; i < kSlots; ++i) slot[i]->borrow(base + L.off[i], L.len[i]);
+  // No event records (they would create events) inside this call.
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  const GridView v = make_view(g);
+  const int tc = 7 * n;
+  int rc = mc_tiles_multi(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, &v);
+  if (rc == VR_OK) rc = mc_tiles_finish(ctx, s, s.views.as<GridView>(), s.leaf_base.as<int>(), 1, n, tc);
+  if (rc == VR_OK) rc = mc_count_multi(ctx, s, s.views.as<GridView>(), n, tc, iso);
+  if (rc == VR_OK)
+    rc = mc_emit_multi(ctx, s, s.views.as<GridView>(), 1, tc, iso, xyz, tri, (double)n, 0.0,
+                       (long long)(vcap < 0x7fffffffull ? vcap : 0x7fffffffull),
+                       (long long)(tcap < 0x7fffffffull ? tcap : 0x7fffffffull), counts);
+  ctx->timing = timing;
+  return rc;
 }
