@@ -39,6 +39,16 @@ __constant__ uint32_t c_mc_cand4[12 * 4];
 __constant__ uint32_t c_mc_edgeinfo[12];
 // Candidate of each local edge that is the cell itself.
 __constant__ int c_mc_self[12];
+// Per cell class and local edge: bit set of the neighbour tiles holding a
+// candidate ordered before the first candidate in the cell's own tile.  The
+// edge's owner lies in another tile when one of them is processed.
+__constant__ uint32_t c_mc_fbefore[27 * 12];
+// Per local edge: its slot in a tile's lattice-edge table for the cell at
+// local (x, y, z) is const + x * mx + y * my + z, packed const | mx << 11 |
+// my << 18.  The table holds the 1944 edges of the tile's 8^3 cells: x edges
+// at (0..7, 0..8, 0..8), then y edges at (0..8, 0..7, 0..8), then z edges at
+// (0..8, 0..8, 0..7), each by its lower endpoint.
+__constant__ uint32_t c_mc_eslot[12];
 // Per cell class (27: per axis local 0 / 1..6 / 7) and local edge: the four
 // candidates in ascending visiting-key order, one byte each: j << 5 | the
 // neighbour-tile index (0..26) that candidate lies in.
@@ -63,7 +73,7 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
       tri4[c * 4 + w] = v;
     }
   }
-  uint32_t cand4[48], einfo[12], ownsort[27 * 12], before[27 * 12];
+  uint32_t cand4[48], einfo[12], ownsort[27 * 12], before[27 * 12], fbefore[27 * 12], eslot[12];
   int self[12];
   for (int e = 0; e < 12; ++e) {
     self[e] = 0;
@@ -78,6 +88,23 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
     const uint32_t ab = (uint32_t)(corner[3 * ca] | (corner[3 * ca + 1] << 1) | (corner[3 * ca + 2] << 2));
     const uint32_t bb = (uint32_t)(corner[3 * cb] | (corner[3 * cb + 1] << 1) | (corner[3 * cb + 2] << 2));
     einfo[e] = (uint32_t)ca | ((uint32_t)cb << 3) | (ab << 6) | (bb << 9);
+    const uint32_t lo = ab & bb, ax = ab ^ bb;  // lower endpoint offset bits, axis bit
+    const uint32_t lx = lo & 1u, ly = (lo >> 1) & 1u, lz = lo >> 2;
+    uint32_t k0, mx, my;
+    if (ax == 1u) {
+      k0 = lx * 81 + ly * 9 + lz;
+      mx = 81;
+      my = 9;
+    } else if (ax == 2u) {
+      k0 = 648 + lx * 72 + ly * 9 + lz;
+      mx = 72;
+      my = 9;
+    } else {
+      k0 = 1296 + lx * 72 + ly * 8 + lz;
+      mx = 72;
+      my = 8;
+    }
+    eslot[e] = k0 | (mx << 11) | (my << 18);
   }
   const int rep[3] = {0, 3, 7};
   for (int c = 0; c < 27; ++c)
@@ -96,15 +123,18 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
           ord[k] = ord[k - 1];
           ord[k - 1] = tmp;
         }
-      uint32_t v = 0, b = 0;
-      bool seen_self = false;
+      uint32_t v = 0, b = 0, fb = 0;
+      bool seen_self = false, seen_tile = false;
       for (int k = 0; k < 4; ++k) {
         v |= (uint32_t)((ord[k] << 5) | tile[ord[k]]) << (8 * k);
         if (ord[k] == self[e]) seen_self = true;
+        if (tile[ord[k]] == 13) seen_tile = true;
         if (!seen_self) b |= 1u << tile[ord[k]];
+        if (!seen_tile) fb |= 1u << tile[ord[k]];
       }
       ownsort[c * 12 + e] = v;
       before[c * 12 + e] = b;
+      fbefore[c * 12 + e] = fb;
     }
   hipError_t e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_emn), emn, sizeof(emn))) != hipSuccess) return e;
@@ -113,6 +143,8 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_edgeinfo), einfo, sizeof(einfo))) != hipSuccess) return e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_self), self, sizeof(self))) != hipSuccess) return e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_before), before, sizeof(before))) != hipSuccess) return e;
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_fbefore), fbefore, sizeof(fbefore))) != hipSuccess) return e;
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_eslot), eslot, sizeof(eslot))) != hipSuccess) return e;
   return hipMemcpyToSymbol(HIP_SYMBOL(c_mc_ownsort), ownsort, sizeof(ownsort));
 }
 
@@ -567,35 +599,35 @@ __device__ __forceinline__ uint32_t mc_case_rows(const uint32_t* rows, int x, in
          (((r10 >> 1) & 1u) << 5) | (((r11 >> 1) & 1u) << 6) | (((r01 >> 1) & 1u) << 7);
 }
 
-// Global index of the vertex on local edge e of cell (x, y, z) of tile t,
-// relative to the tile's grid.  The owner is the first candidate of
-// (class, edge) in visiting-key order whose tile is processed; its meta
-// gives its first vertex and owned-edge mask (LDS for this tile, the face
-// meta of pass 1 for an earlier tile).  0 if no owner (not reached for
-// edges of a crossing cell of a processed tile).
-__device__ __forceinline__ int mc_edge_vertex(int x, int y, int z, int e, int cls, uint32_t processed,
-                                              const uint32_t* s_ownsort, const uint32_t* s_cand,
-                                              const uint32_t* s_meta, const int* ttile, const uint32_t* meta,
-                                              const int* vbase, int vb, int vrel) {
-  const int j = mc_first_processed(s_ownsort[cls * 12 + e], processed);
+// Slot of local edge e of cell (x, y, z) in the tile's lattice-edge table
+// (c_mc_eslot): every cell containing the same lattice edge gets the same slot.
+__device__ __forceinline__ int mc_edge_slot(int x, int y, int z, uint32_t es) {
+  return (int)(es & 0x7FFu) + x * (int)((es >> 11) & 0x7Fu) + y * (int)(es >> 18) + z;
+}
+#define MC_EDGES 1944
+#define MC_FOREIGN 432  // lattice edges on a tile's three low planes: 3 x 144
+
+// Index (relative to the grid) of the vertex on local edge e of cell
+// (x, y, z) whose owner lies in an earlier tile: the first processed
+// candidate in visiting order (c_mc_ownsort) names that tile and cell, whose
+// face meta from pass 1 gives the vertex.  Such an owner always has a face
+// index: its tile precedes the cell's, so its first differing tile offset is
+// -1 and its local coordinate on that axis is 7.
+__device__ __forceinline__ int mc_foreign_index(int x, int y, int z, int e, uint32_t processed,
+                                                const uint32_t* s_ownsort, const uint32_t* s_cand, const int* ttile,
+                                                const uint32_t* meta, const int* vbase, int vrel) {
+  const int j = mc_first_processed(s_ownsort[mc_cell_class(x, y, z) * 12 + e], processed);
   const uint32_t cw = s_cand[e * 4 + (j < 0 ? 0 : j)];
   const int nx = x + (int)(int8_t)(cw & 0xFFu), ny = y + (int)(int8_t)((cw >> 8) & 0xFFu),
             nz = z + (int)(int8_t)((cw >> 16) & 0xFFu);
-  const int ep = (int)(int8_t)(cw >> 24);
+  const uint32_t ep = (uint32_t)(int8_t)(cw >> 24);
   const int d = ((nx >> 3) + 1) * 9 + ((ny >> 3) + 1) * 3 + ((nz >> 3) + 1);
-  // Both sources are read unconditionally (the global one at index 0 when
-  // the owner is in this tile) so that the lookups of a triangle overlap.
-  const bool ext = d != 13;
   const int ot = ttile[d];
   const int fi = mc_face_index(nx & 7, ny & 7, nz & 7);
-  const bool ok = j >= 0 && (!ext || (ot >= 0 && fi >= 0));
-  const int go = ext && ok ? ot : 0;
-  const uint32_t omg = meta[(size_t)go * MC_FACE + (ext && ok ? fi : 0)];
-  const int obg = vbase[go];
-  const uint32_t oml = s_meta[((nx & 7) << 6) + ((ny & 7) << 3) + (nz & 7)];
-  const uint32_t om = ext ? omg : oml;
-  const int obase = ext ? obg : vb;
-  return ok ? obase - vrel + (int)(om >> 12) + __popc((om & 0xFFFu) & ((1u << ep) - 1u)) : 0;
+  const bool ok = j >= 0 && d != 13 && ot >= 0 && fi >= 0;
+  const uint32_t om = meta[(size_t)(ok ? ot : 0) * MC_FACE + (ok ? fi : 0)];
+  const int ob = vbase[ok ? ot : 0];
+  return ok ? ob - vrel + (int)(om >> 12) + __popc(om & 0xFFFu & ((1u << ep) - 1u)) : 0;
 }
 
 // Corner value at tile-local (x, y, z), 0..8 each (leaf 0 is read and
@@ -732,35 +764,55 @@ __device__ __forceinline__ void mc_ranges(const int* tile_base, int G, int nt, c
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
 // Mixed cells go in rounds of 64, one per lane: a wave scan places the
-// round's vertices and triangles, the cells' meta goes to LDS, and lists of
-// the round's vertices (cell, edge) and triangles (cell, row entry, case)
-// spread the writes one item per lane.  A triangle's vertices come from the
-// owners' meta: this tile's in LDS (an owner precedes the cell in visiting
-// order, so its round is done), earlier tiles' face meta from pass 1.  A
-// round with more items than the lists hold is written cell by cell.
-// Vertex indices are relative to the tile's grid.
+// round's vertices and triangles, and lists of the round's vertices
+// (cell, edge) and triangles (cell, row entry, case) spread the writes one
+// item per lane.  Vertex indices go through the tile's lattice-edge table
+// in LDS (16 bits per edge): an owned vertex stores its index relative to
+// the tile's first vertex at its edge's slot; an edge owned in an earlier
+// tile (it lies on one of the tile's low planes) is resolved once, by the
+// first cell of this tile that contains it, from that tile's face meta of
+// pass 1 into a side table, and its slot holds 0x8000 | the side-table
+// entry.  Owners precede the cells that share their edges in visiting
+// order, so every slot a triangle reads was written in this or an earlier
+// round, before the round's triangles.  A round with more items than the
+// lists hold is written cell by cell.  Vertex indices are relative to the
+// tile's grid.
 #define MC_EWAVES 8
 #define MC_ETHREADS (64 * MC_EWAVES)
 #define MC_RV 192
 #define MC_RT 256
 struct MCEmitLDS {
-  uint32_t meta[512];     // (vertex prefix << 12) | owned edges, mixed cells
-  uint32_t tlist[MC_RT];  // cell | triangle of the cell << 9 | case << 12
-  uint16_t vlist[MC_RV];  // cell << 4 | edge
+  uint16_t edge[MC_EDGES];       // lattice-edge table of the tile
+  int foreign[MC_FOREIGN];       // vertex index (relative to the grid) of edges owned in an earlier tile
+  union {
+    uint32_t tlist[MC_RT];       // cell | triangle of the cell << 9 | case << 12
+    uint16_t flist[MC_FOREIGN];  // cell << 4 | edge, edges owned in an earlier tile to resolve
+  };
+  uint16_t vlist[MC_RV];         // cell << 4 | edge
+  uint16_t fitem[28];            // per cell class: edges owned in an earlier tile that this cell resolves
 };
 
-__global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_mc_emit(const GridView* views, const int4* tiles, const int* ntiles,
-                                                         const int* tleaf, const int* ttile, const int* tile_base,
-                                                         float iso, const uint32_t* meta, const int* tv, const int* tt,
-                                                         const int* vbase, const int* tbase, const int* totals,
-                                                         double* verts, uint32_t* tris, long long vcap,
-                                                         long long tricap, int G, int* ranges, int* counts_out) {
+// Vertex index (relative to the grid) on local edge e of cell (x, y, z).
+__device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const MCEmitLDS& el,
+                                               const uint32_t* s_eslot, int vbrel) {
+  const uint32_t v = el.edge[mc_edge_slot(x, y, z, s_eslot[e])];
+  const int f = el.foreign[(v & 0x1FFu) < MC_FOREIGN ? (v & 0x1FFu) : 0];
+  return (v & 0x8000u) ? f : vbrel + (int)v;
+}
+
+__global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mc_emit(
+    const GridView* views, const int4* tiles, const int* ntiles, const int* tleaf, const int* ttile,
+    const int* tile_base, float iso, const uint32_t* meta, const int* tv, const int* tt, const int* vbase,
+    const int* tbase, const int* totals, double* verts, uint32_t* tris, long long vcap, long long tricap, int G,
+    int* ranges, int* counts_out) {
   __shared__ uint16_t s_emn[256];
   __shared__ __align__(16) int8_t s_tri[256 * 16];  // TRI_TABLE rows
   __shared__ uint32_t s_ownsort[27 * 12];
   __shared__ uint32_t s_before[27 * 12];
+  __shared__ uint32_t s_fbefore[27 * 12];
   __shared__ uint32_t s_cand[48];
   __shared__ uint32_t s_einfo[12];
+  __shared__ uint32_t s_eslot[12];
   __shared__ MCTileLDS s_w[MC_EWAVES];
   __shared__ MCEmitLDS s_e[MC_EWAVES];
   const int tid = threadIdx.x, lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -769,9 +821,13 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
   for (int i = tid; i < 27 * 12; i += MC_ETHREADS) {
     s_ownsort[i] = c_mc_ownsort[i];
     s_before[i] = c_mc_before[i];
+    s_fbefore[i] = c_mc_fbefore[i];
   }
   if (tid < 48) s_cand[tid] = c_mc_cand4[tid];
-  if (tid < 12) s_einfo[tid] = c_mc_edgeinfo[tid];
+  if (tid < 12) {
+    s_einfo[tid] = c_mc_edgeinfo[tid];
+    s_eslot[tid] = c_mc_eslot[tid];
+  }
   __syncthreads();
   MCTileLDS& w = s_w[wv];
   MCEmitLDS& el = s_e[wv];
@@ -818,17 +874,32 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     const double vs = g.vs;
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
+    const int vbrel = vb - vrel;
     const int n = mc_tile_setup(leaves, bg, iso, tl, ttv, processed, s_before, w);
-    int vcarry = 0, tcarry = 0;
+    // Per class: the edges owned in an earlier tile that this cell resolves
+    // (it is the first cell of this tile containing them: no candidate of
+    // this tile precedes it).
+    if (lane < 27) {
+      uint32_t m = 0;
+#pragma unroll
+      for (int e = 0; e < 12; ++e)
+        m |= (uint32_t)((s_fbefore[lane * 12 + e] & processed) != 0u && ((s_before[lane * 12 + e] >> 13) & 1u) == 0u)
+             << e;
+      el.fitem[lane] = (uint16_t)m;
+    }
+    wave_lds_sync();
+    int vcarry = 0, tcarry = 0, fcarry = 0;
     for (int base = 0; base < n; base += 64) {
       const int i = base + lane;
       int c = 0, packed = 0, ntr = 0;
-      uint32_t owned = 0u, cube = 0u;
+      uint32_t owned = 0u, cube = 0u, fm = 0u;
       if (i < n) {
         c = w.mlist[i];
+        const int ccls = mc_cell_class(c >> 6, (c >> 3) & 7, c & 7);
         cube = mc_case_rows(w.rows, c >> 6, (c >> 3) & 7, c & 7);
         const uint32_t emn = s_emn[cube];
-        owned = (emn & 0xFFFu) & w.self[mc_cell_class(c >> 6, (c >> 3) & 7, c & 7)];
+        owned = (emn & 0xFFFu) & w.self[ccls];
+        fm = (emn & 0xFFFu) & el.fitem[ccls];
         ntr = (int)(emn >> 12);
         packed = __popc(owned) | (ntr << 16);
       }
@@ -836,8 +907,26 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
       const int excl = incl - packed, tot = __builtin_amdgcn_readlane(incl, 63);
       const int rv = tot & 0xFFFF, rt = tot >> 16;
       const int vr = excl & 0xFFFF, tr = excl >> 16;
-      if (i < n) el.meta[c] = ((uint32_t)(vcarry + vr) << 12) | owned;
       const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
+      // Edges owned in an earlier tile, one resolved per lane (each lattice
+      // edge once per tile, so at most MC_FOREIGN per tile).
+      const int nf = __popc(fm);
+      const int fincl = wave_incl_scan(nf);
+      const int ftot = __builtin_amdgcn_readlane(fincl, 63);
+      if (ftot > 0) {
+        int k = fincl - nf;
+        for (uint32_t m = fm; m; m &= m - 1u, ++k) el.flist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
+        wave_lds_sync();
+        for (int k2 = lane; k2 < ftot; k2 += 64) {
+          const int it = el.flist[k2], cc = it >> 4, fe = it & 15;
+          const int fx = cc >> 6, fy = (cc >> 3) & 7, fz = cc & 7;
+          const int slot = fcarry + k2;
+          el.foreign[slot] = mc_foreign_index(fx, fy, fz, fe, processed, s_ownsort, s_cand, w.ttile, meta, vbase, vrel);
+          el.edge[mc_edge_slot(fx, fy, fz, s_eslot[fe])] = (uint16_t)(0x8000u | (uint32_t)slot);
+        }
+        fcarry += ftot;
+        wave_lds_sync();
+      }
       if (rv <= MC_RV && rt <= MC_RT) {
         if (i < n) {
           int k = vr;
@@ -847,41 +936,49 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
         wave_lds_sync();
         for (int k = lane; k < rv; k += 64) {
           const int vi = vb + vcarry + k;
-          const int it = el.vlist[k], cc = it >> 4;
+          const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
+          const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
+          el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
           if (vi < vmax)
-            mc_write_vertex(leaves, bg, w.tleaf, cc >> 6, (cc >> 3) & 7, cc & 7, it & 15, to, vs, iso, s_einfo,
-                            verts + 3 * (size_t)vi);
+            mc_write_vertex(leaves, bg, w.tleaf, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
         }
+        wave_lds_sync();
         for (int k = lane; k < rt; k += 64) {
           const int ti = tb + tcarry + k;
           const uint32_t it = el.tlist[k];
           const int cc = (int)(it & 511u), q = (int)((it >> 9) & 7u);
           const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
           const int8_t* row = s_tri + (it >> 12) * 16 + 3 * q;
-          const int cls = mc_cell_class(ux, uy, uz);
+          const int i0 = mc_corner_index(ux, uy, uz, row[0], el, s_eslot, vbrel);
+          const int i1 = mc_corner_index(ux, uy, uz, row[1], el, s_eslot, vbrel);
+          const int i2 = mc_corner_index(ux, uy, uz, row[2], el, s_eslot, vbrel);
           if (ti < tmax) {
             uint32_t* o = tris + 3 * (size_t)ti;
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-              o[j] = (uint32_t)mc_edge_vertex(ux, uy, uz, row[j], cls, processed, s_ownsort, s_cand, el.meta, w.ttile,
-                                              meta, vbase, vb, vrel);
+            o[0] = (uint32_t)i0;
+            o[1] = (uint32_t)i1;
+            o[2] = (uint32_t)i2;
           }
         }
       } else {
-        // Dense round: cell by cell.
+        // Dense round: cell by cell, every vertex (and its table slot)
+        // before any triangle of the round.
         wave_lds_sync();
         if (i < n) {
-          int vi = vb + vcarry + vr;
-          for (uint32_t m = owned; m && vi < vmax; m &= m - 1u, ++vi)
-            mc_write_vertex(leaves, bg, w.tleaf, cx, cy, cz, __ffs(m) - 1, to, vs, iso, s_einfo,
-                            verts + 3 * (size_t)vi);
-          const int cls = mc_cell_class(cx, cy, cz);
+          int vk = vcarry + vr;
+          for (uint32_t m = owned; m; m &= m - 1u, ++vk) {
+            const int ve = __ffs(m) - 1;
+            el.edge[mc_edge_slot(cx, cy, cz, s_eslot[ve])] = (uint16_t)vk;
+            if (vb + vk < vmax)
+              mc_write_vertex(leaves, bg, w.tleaf, cx, cy, cz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)(vb + vk));
+          }
+        }
+        wave_lds_sync();
+        if (i < n) {
           int ti = tb + tcarry + tr;
           for (int q = 0; q < ntr && ti < tmax; ++q, ++ti)
-            for (int j = 0; j < 3; ++j)
-              tris[3 * (size_t)ti + j] = (uint32_t)mc_edge_vertex(cx, cy, cz, s_tri[cube * 16 + 3 * q + j], cls, processed,
-                                                                  s_ownsort, s_cand, el.meta, w.ttile, meta, vbase, vb,
-                                                                  vrel);
+            for (int jj = 0; jj < 3; ++jj)
+              tris[3 * (size_t)ti + jj] =
+                  (uint32_t)mc_corner_index(cx, cy, cz, s_tri[cube * 16 + 3 * q + jj], el, s_eslot, vbrel);
         }
       }
       wave_lds_sync();
