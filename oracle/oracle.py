@@ -72,6 +72,9 @@ _SIGS = {
     "or_rollout_adjoint_f64": (_D, [_P, _I, _D, _P, _D, _I, _I, _P, _P]),
     "or_rollout_dense": (_P, [_P, _I, _D, _P, _D, _I, _I]),
     "or_time_advect_mc": (_D, [_P, _I, _P, _D, _F, _P]),
+    "or_vol_write": (_S, [_P, _P, _S]),
+    "or_vol_read": (_P, [_P, _S, ctypes.c_char_p, _S]),
+    "or_extend_velocity": (None, [_P, _P, _P, _P, _F]),
     "or_time_step": (_D, [_P, _I, _P, _D, _I, _P, _I, _F, _I, _F]),
 }
 
@@ -339,3 +342,24 @@ def time_step(g, kind, params, dt, scheme, tf, do_rebuild, hw, do_mc, iso=0.0):
     """Times one oracle step in place on g (advect [+ rebuild] [+ MC]); seconds."""
     p, t = _dvec(params, 6), _dvec(tf, 3)
     return lib().or_time_step(g.h, kind, _ptr(p), dt, scheme, _ptr(t), int(do_rebuild), hw, int(do_mc), iso)
+
+
+def vol_write(g):
+    """write_vol (src/io/vol_format.rs:53-85), leaves in ascending origin order."""
+    n = lib().or_vol_write(g.h, None, 0)
+    out = np.empty(n, dtype=np.uint8)
+    lib().or_vol_write(g.h, _ptr(out), n)
+    return out.tobytes()
+
+
+def vol_read(b):
+    """read_vol (src/io/vol_format.rs:88-147): (grid, "") or (None, error text)."""
+    a = np.frombuffer(bytes(b), dtype=np.uint8) if len(b) else np.zeros(1, dtype=np.uint8)
+    msg = ctypes.create_string_buffer(256)
+    h = lib().or_vol_read(_ptr(a), len(b), msg, 256)
+    return (OGrid(h) if h else None), msg.value.decode()
+
+
+def extend_velocity(sdf, vx, vy, vz, half_width):
+    """extend_velocity (src/tools/level_set_track.rs:127-169), in place on vx, vy, vz."""
+    lib().or_extend_velocity(sdf.h, vx.h, vy.h, vz.h, half_width)

@@ -7,6 +7,7 @@
 #include <cstdio>
 
 #include "volrus_oracle_ext.hpp"
+#include "volrus_oracle_io.hpp"
 
 using namespace vo;
 
@@ -235,6 +236,23 @@ double or_time_step(void* g, int kind, const double* a6, double dt, int scheme, 
   const auto t1 = std::chrono::steady_clock::now();
   *G(g) = std::move(out);
   return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// ---------------------------------------------------------------- .vol, velocity extension
+size_t or_vol_write(void* g, uint8_t* out, size_t cap) {
+  const std::vector<uint8_t> b = vol_write(*G(g));
+  if (out && cap >= b.size()) std::memcpy(out, b.data(), b.size());
+  return b.size();
+}
+// The grid read, or null with the error text in msg.
+void* or_vol_read(const uint8_t* b, size_t len, char* msg, size_t cap) {
+  Grid<float> g(0.0f, 1.0);
+  const std::string e = vol_read(b, len, &g);
+  if (msg && cap) std::snprintf(msg, cap, "%s", e.c_str());
+  return e.empty() ? new Grid<float>(std::move(g)) : nullptr;
+}
+void or_extend_velocity(void* sdf, void* vx, void* vy, void* vz, float half_width) {
+  extend_velocity(*G(sdf), *G(vx), *G(vy), *G(vz), half_width);
 }
 
 }  // extern "C"
