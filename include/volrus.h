@@ -31,7 +31,8 @@ enum {
   VR_ERR_OUT_OF_MEMORY = 3,
   VR_ERR_HIP = 4,
   VR_ERR_UNSUPPORTED = 5,
-  VR_ERR_CAPACITY = 6 /* caller buffer too small; required size reported */
+  VR_ERR_CAPACITY = 6, /* caller buffer too small; required size reported */
+  VR_ERR_INVALID_VOL = 7 /* read_vol would return Err (InvalidData / UnexpectedEof) */
 };
 
 /* Integration schemes for vr_advect. */
@@ -70,6 +71,16 @@ int vr_grid_upload_nano(vr_ctx* ctx, const uint8_t* bytes, size_t len, vr_grid**
  * grid's contents.  With out == NULL or cap too small, *len receives the size
  * and VR_ERR_CAPACITY is returned (NULL/0 is the size query, returns VR_OK). */
 int vr_grid_download_nano(vr_ctx* ctx, const vr_grid* g, uint8_t* out, size_t cap, size_t* len);
+/* read_vol (src/io/vol_format.rs:88-147) plus the upload: the same checks
+ * and messages (bad magic, version, grid class, short buffer ->
+ * VR_ERR_INVALID_VOL); each record becomes the leaf at origin & ~7, a later
+ * record of a leaf replacing an earlier one; uniform transform, zero origin,
+ * as Grid::new. */
+int vr_grid_upload_vol(vr_ctx* ctx, const uint8_t* bytes, size_t len, vr_grid** out);
+/* write_vol (src/io/vol_format.rs:53-85) for a device grid, leaves in
+ * ascending origin order (the reference writes its HashMap order).  Size
+ * query and VR_ERR_CAPACITY as vr_grid_download_nano. */
+int vr_grid_download_vol(vr_ctx* ctx, const vr_grid* g, uint8_t* out, size_t cap, size_t* len);
 /* Grid::leaf_count / active_voxel_count (src/grid.rs:505-513). */
 int vr_grid_counts(vr_ctx* ctx, const vr_grid* g, uint64_t* leaf_count, uint64_t* active_count);
 int vr_grid_destroy(vr_ctx* ctx, vr_grid* g);
@@ -125,6 +136,15 @@ int vr_trim_band(vr_ctx* ctx, vr_grid* g, float half_width);
 /* rebuild_narrow_band(&mut Grid<f32>, half_width) (src/tools/level_set_track.rs:80-114).
  * Requires ceil(half_width) <= 8 (VR_ERR_UNSUPPORTED otherwise). */
 int vr_rebuild_band(vr_ctx* ctx, vr_grid* g, float half_width);
+/* extend_velocity(&sdf, &mut vx, &mut vy, &mut vz, half_width)
+ * (src/tools/level_set_track.rs:127-169): every band voxel of sdf
+ * (vs <= |phi| < half_width * vs, active) takes, in vx, vy and vz (activated,
+ * with a new leaf where a grid has none), the values at the interface voxel
+ * (|phi| < vs, active) at the least squared index distance; among equals the
+ * first in iteration order (internal-node origin ascending, then leaf slot,
+ * then voxel offset: the reference's order inside one 128^3 node).  vx, vy,
+ * vz must be distinct grids. */
+int vr_extend_velocity(vr_ctx* ctx, const vr_grid* sdf, vr_grid* vx, vr_grid* vy, vr_grid* vz, float half_width);
 /* volume_to_mesh(&Grid<f32>, iso) -> Mesh (src/tools/volume_to_mesh.rs:46-101). */
 int vr_marching_cubes(vr_ctx* ctx, const vr_grid* g, float iso, vr_mesh** out);
 /* Mesh::vertex_count / tri_count (src/tools/volume_to_mesh.rs:18-25). */

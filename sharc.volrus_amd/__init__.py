@@ -28,6 +28,7 @@ VR_ERR_OUT_OF_MEMORY = 3
 VR_ERR_HIP = 4
 VR_ERR_UNSUPPORTED = 5
 VR_ERR_CAPACITY = 6
+VR_ERR_INVALID_VOL = 7
 
 RK1 = 1
 RK3 = 3
@@ -57,6 +58,9 @@ _SIGS = {
     "vr_ctx_stream": (_P, [_P]),
     "vr_grid_upload_nano": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "vr_grid_download_nano": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "vr_grid_upload_vol": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "vr_grid_download_vol": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "vr_extend_velocity": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_float]),
     "vr_grid_counts": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "vr_grid_destroy": (ctypes.c_int, [_P, _P]),
     "vr_grid_from_primitives": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -199,6 +203,17 @@ class Context:
         self.check(self.lib.vr_grid_upload_nano(self.h, _ptr(b), b.size, ctypes.byref(h)))
         return Grid(self, h)
 
+    def upload_vol(self, vol_bytes):
+        """A grid from the bytes of a .vol file (read_vol)."""
+        b = np.frombuffer(bytes(vol_bytes), dtype=np.uint8) if len(vol_bytes) else np.zeros(1, dtype=np.uint8)
+        h = ctypes.c_void_p()
+        self.check(self.lib.vr_grid_upload_vol(self.h, _ptr(b), len(vol_bytes), ctypes.byref(h)))
+        return Grid(self, h)
+
+    def extend_velocity(self, sdf, vx, vy, vz, half_width):
+        """extend_velocity, in place on the velocity grids vx, vy, vz."""
+        self.check(self.lib.vr_extend_velocity(self.h, sdf.h, vx.h, vy.h, vz.h, half_width))
+
     def from_primitives(self, prims, voxel_size, half_width):
         """prims: list of (kind, op, params); op is ignored for the first."""
         a = np.zeros(len(prims), dtype=PRIM_DTYPE)
@@ -244,6 +259,14 @@ class Grid:
         self.ctx.check(self.ctx.lib.vr_grid_download_nano(self.ctx.h, self.h, None, 0, ctypes.byref(n)))
         out = np.empty(n.value, dtype=np.uint8)
         self.ctx.check(self.ctx.lib.vr_grid_download_nano(self.ctx.h, self.h, _ptr(out), out.size, ctypes.byref(n)))
+        return out.tobytes()
+
+    def download_vol(self):
+        """The grid as the bytes of a .vol file (write_vol), leaves in ascending origin order."""
+        n = ctypes.c_size_t()
+        self.ctx.check(self.ctx.lib.vr_grid_download_vol(self.ctx.h, self.h, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value, dtype=np.uint8)
+        self.ctx.check(self.ctx.lib.vr_grid_download_vol(self.ctx.h, self.h, _ptr(out), out.size, ctypes.byref(n)))
         return out.tobytes()
 
     def clone(self):
