@@ -75,6 +75,14 @@ _SIGS = {
     "or_vol_write": (_S, [_P, _P, _S]),
     "or_vol_read": (_P, [_P, _S, ctypes.c_char_p, _S]),
     "or_extend_velocity": (None, [_P, _P, _P, _P, _F]),
+    "or_gradient_at": (None, [_P, _I, _I, _I, _P]),
+    "or_laplacian_at": (_F, [_P, _I, _I, _I]),
+    "or_mean_curvature_at": (_F, [_P, _I, _I, _I]),
+    "or_adaptive_surface": (_P, [_P, _F, _D, _D, ctypes.c_uint32]),
+    "or_render_nv": (_S, [_P]),
+    "or_render_nt": (_S, [_P]),
+    "or_render_copy": (None, [_P, _P, _P, _P, _P]),
+    "or_render_free": (None, [_P]),
     "or_time_step": (_D, [_P, _I, _P, _D, _I, _P, _I, _F, _I, _F]),
 }
 
@@ -363,3 +371,30 @@ def vol_read(b):
 def extend_velocity(sdf, vx, vy, vz, half_width):
     """extend_velocity (src/tools/level_set_track.rs:127-169), in place on vx, vy, vz."""
     lib().or_extend_velocity(sdf.h, vx.h, vy.h, vz.h, half_width)
+
+
+def gradient_at(g, x, y, z):
+    out = np.zeros(3, dtype=np.float32)
+    lib().or_gradient_at(g.h, x, y, z, _ptr(out))
+    return out
+
+
+def laplacian_at(g, x, y, z):
+    return lib().or_laplacian_at(g.h, x, y, z)
+
+
+def mean_curvature_at(g, x, y, z):
+    return lib().or_mean_curvature_at(g.h, x, y, z)
+
+
+def adaptive_surface(g, iso=0.0, max_edge_length=1.0, curvature_threshold=0.5, max_depth=3):
+    """adaptive_surface (src/tools/adaptive_mesh.rs:100-241): (vertices, normals, triangles, curvatures)."""
+    m = lib().or_adaptive_surface(g.h, iso, max_edge_length, curvature_threshold, max_depth)
+    try:
+        nv, nt = lib().or_render_nv(m), lib().or_render_nt(m)
+        v = np.empty((max(nv, 1), 3)); n = np.empty((max(nv, 1), 3))
+        t = np.empty((max(nt, 1), 3), dtype=np.uint32); c = np.empty(max(nv, 1))
+        lib().or_render_copy(m, _ptr(v), _ptr(n), _ptr(t), _ptr(c))
+        return v[:nv], n[:nv], t[:nt], c[:nv]
+    finally:
+        lib().or_render_free(m)

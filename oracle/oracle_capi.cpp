@@ -8,6 +8,7 @@
 
 #include "volrus_oracle_ext.hpp"
 #include "volrus_oracle_io.hpp"
+#include "volrus_oracle_adaptive.hpp"
 
 using namespace vo;
 
@@ -254,5 +255,32 @@ void* or_vol_read(const uint8_t* b, size_t len, char* msg, size_t cap) {
 void or_extend_velocity(void* sdf, void* vx, void* vy, void* vz, float half_width) {
   extend_velocity(*G(sdf), *G(vx), *G(vy), *G(vz), half_width);
 }
+
+// ---------------------------------------------------------------- differential operators, adaptive meshing
+void or_gradient_at(void* g, int x, int y, int z, float* out) {
+  const std::array<float, 3> v = gradient_at(*G(g), Coord(x, y, z));
+  out[0] = v[0];
+  out[1] = v[1];
+  out[2] = v[2];
+}
+float or_laplacian_at(void* g, int x, int y, int z) { return laplacian_at(*G(g), Coord(x, y, z)); }
+float or_mean_curvature_at(void* g, int x, int y, int z) { return mean_curvature_at(*G(g), Coord(x, y, z)); }
+void* or_adaptive_surface(void* g, float iso, double max_edge, double threshold, uint32_t depth) {
+  return new RenderMesh(adaptive_surface(*G(g), iso, max_edge, threshold, depth));
+}
+size_t or_render_nv(void* m) { return static_cast<RenderMesh*>(m)->vertices.size(); }
+size_t or_render_nt(void* m) { return static_cast<RenderMesh*>(m)->triangles.size(); }
+void or_render_copy(void* m, double* v, double* n, uint32_t* t, double* c) {
+  const RenderMesh& r = *static_cast<RenderMesh*>(m);
+  for (size_t i = 0; i < r.vertices.size(); ++i)
+    for (int k = 0; k < 3; ++k) {
+      v[3 * i + k] = r.vertices[i][k];
+      n[3 * i + k] = r.normals[i][k];
+    }
+  for (size_t i = 0; i < r.triangles.size(); ++i)
+    for (int k = 0; k < 3; ++k) t[3 * i + k] = r.triangles[i][k];
+  for (size_t i = 0; i < r.curvatures.size(); ++i) c[i] = r.curvatures[i];
+}
+void or_render_free(void* m) { delete static_cast<RenderMesh*>(m); }
 
 }  // extern "C"
