@@ -170,6 +170,25 @@ int vr_mesh_create(vr_ctx* ctx, vr_mesh** out);
 int vr_marching_cubes_noalloc(vr_ctx* ctx, const vr_grid* g, float iso, double* xyz, uint64_t vcap, uint32_t* tri,
                               uint64_t tcap, int32_t* counts);
 
+/* ---- adaptive meshing ---------------------------------------------------- */
+/* adaptive_surface(&Grid<f32>, &AdaptiveMeshConfig) -> RenderMesh
+ * (src/tools/adaptive_mesh.rs:100-241): marching cubes, then max_depth rounds
+ * of 1-to-4 midpoint subdivision of the triangles whose largest vertex |mean
+ * curvature| exceeds curvature_threshold and whose longest edge exceeds
+ * max_edge_length / 2^(depth+1), new vertices projected onto the isosurface;
+ * angle-weighted vertex normals and signed curvatures.  Vertices, triangles
+ * and curvatures follow the reference's arithmetic and numbering; normals
+ * use the device's acos. */
+typedef struct vr_render_mesh vr_render_mesh;
+int vr_adaptive_surface(vr_ctx* ctx, const vr_grid* g, float isovalue, double max_edge_length,
+                        double curvature_threshold, uint32_t max_depth, vr_render_mesh** out);
+int vr_render_mesh_counts(vr_ctx* ctx, const vr_render_mesh* m, uint64_t* vertex_count, uint64_t* tri_count);
+/* RenderMesh::{vertices, normals, curvatures} (vcap entries each, any may be
+ * NULL) and triangles (tcap). */
+int vr_render_mesh_copy(vr_ctx* ctx, const vr_render_mesh* m, double* xyz, double* normals, double* curvatures,
+                        size_t vcap, uint32_t* tri, size_t tcap);
+int vr_render_mesh_destroy(vr_ctx* ctx, vr_render_mesh* m);
+
 /* ---- rollout ------------------------------------------------------------ */
 /* No counterpart in the reference (nothing there chains its tools).  Runs
  * `steps` advection steps from time t0 (step s starts at t0 + s * dt), calls

@@ -61,6 +61,11 @@ _SIGS = {
     "vr_grid_upload_vol": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "vr_grid_download_vol": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "vr_extend_velocity": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_float]),
+    "vr_adaptive_surface": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
+                                           ctypes.POINTER(_P)]),
+    "vr_render_mesh_counts": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "vr_render_mesh_copy": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+    "vr_render_mesh_destroy": (ctypes.c_int, [_P, _P]),
     "vr_grid_counts": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "vr_grid_destroy": (ctypes.c_int, [_P, _P]),
     "vr_grid_from_primitives": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -300,6 +305,24 @@ class Grid:
         self.ctx.check(self.ctx.lib.vr_adjoint_rollout(self.ctx.h, self.h, vel.h, dt, scheme, steps,
                                                        checkpoint_every, target.h, ctypes.byref(st), _ptr(grad)))
         return st, (grad.reshape(3, n, n, n) if want_grad else None)
+
+    def adaptive_surface(self, iso=0.0, max_edge_length=1.0, curvature_threshold=0.5, max_depth=3):
+        """adaptive_surface: (vertices, normals, triangles, curvatures)."""
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.vr_adaptive_surface(self.ctx.h, self.h, iso, max_edge_length, curvature_threshold,
+                                                        max_depth, ctypes.byref(h)))
+        try:
+            nv, nt = ctypes.c_uint64(), ctypes.c_uint64()
+            self.ctx.check(self.ctx.lib.vr_render_mesh_counts(self.ctx.h, h, ctypes.byref(nv), ctypes.byref(nt)))
+            v = np.empty((max(nv.value, 1), 3))
+            n = np.empty((max(nv.value, 1), 3))
+            c = np.empty(max(nv.value, 1))
+            t = np.empty((max(nt.value, 1), 3), dtype=np.uint32)
+            self.ctx.check(self.ctx.lib.vr_render_mesh_copy(self.ctx.h, h, _ptr(v), _ptr(n), _ptr(c), nv.value, _ptr(t),
+                                                            nt.value))
+            return v[: nv.value], n[: nv.value], t[: nt.value], c[: nv.value]
+        finally:
+            self.ctx.lib.vr_render_mesh_destroy(self.ctx.h, h)
 
     def zero_crossings(self):
         n = ctypes.c_size_t()

@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["vr_kernels.hip", "vr_api.hip", "vr_fixtures.hip", "vr_rollout.hip", "vr_adjoint.hip", "vr_mc.hip", "vr_batch.hip", "vr_io.hip", "vr_extend.hip"]
+SOURCES = ["vr_kernels.hip", "vr_api.hip", "vr_fixtures.hip", "vr_rollout.hip", "vr_adjoint.hip", "vr_mc.hip", "vr_batch.hip", "vr_io.hip", "vr_extend.hip", "vr_adaptive.hip"]
 LIB = os.path.join(HERE, "libvolrus_amd.so")
 
 # -ffp-contract=off: no fused multiply-add, so f32/f64 results round like the
