@@ -277,31 +277,29 @@ def adjoint_noisy_extra(vr, ctx, count=8):
     """8 level sets at 1/2048, each a sphere r = 0.35 displaced by seeded 4-octave value noise (amplitude
     0.02, base frequency 8), band 3; per grid a dense 128^3 field of seeded Gaussian noise (sigma 0.5);
     10 RK3 steps forward + adjoint with a checkpoint every 2 steps; loss against the undisplaced sphere.
-    Grids run one after another through vr_adjoint_rollout."""
+    All grids in one launch per step (vr_adjoint_rollout_batch)."""
     import numpy as np
 
     vs, n, steps, ckpt = 1.0 / 2048, 128, 10, 2
+    phis, vels, dts, active, leaves = [], [], [], 0, 0
     target = ctx.from_primitives([(vr.PRIM_SPHERE, 0, (0.5, 0.5, 0.5, 0.35))], vs, HW)
-    fwd = adj = 0.0
-    active = leaves = 0
     for i in range(count):
         phi = ctx.from_primitives([(vr.PRIM_NOISY_SPHERE, 0, (0.5, 0.5, 0.5, 0.35, 0.02, 8.0, 1000 + i))], vs, HW)
         lc, ac = phi.counts()
         u = np.random.default_rng(100 + i).normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32)
-        vel = ctx.velocity_dense(u, n, 1.0 / n)
-        dt = 0.5 * vs / float(np.abs(u).max())
-        if i == 0:
-            phi.adjoint_rollout(vel, n, dt, vr.RK3, steps, ckpt, target, False)  # warm
-        st, _ = phi.adjoint_rollout(vel, n, dt, vr.RK3, steps, ckpt, target)
-        fwd += st.forward_ms
-        adj += st.adjoint_ms
+        phis.append(phi)
+        vels.append(ctx.velocity_dense(u, n, 1.0 / n))
+        dts.append(0.5 * vs / float(np.abs(u).max()))
         active += ac
         leaves += lc
-        log(f"noisy sphere {i}: {lc} leaves, {ac} active voxels, forward {st.forward_ms:.2f} ms, "
-            f"adjoint {st.adjoint_ms:.2f} ms")
-        del phi, vel
+    log(f"noisy spheres: {count} grids, {leaves} leaves, {active} active voxels")
+    targets = [target] * count
+    vr.adjoint_rollout_batch(ctx, phis, vels, [n] * count, dts, vr.RK3, steps, ckpt, targets, False)  # warm
+    st, _ = vr.adjoint_rollout_batch(ctx, phis, vels, [n] * count, dts, vr.RK3, steps, ckpt, targets, False)
+    fwd, adj = st[0].forward_ms, st[0].adjoint_ms
     return {"grids": count, "leaves": leaves, "active_voxels": active, "steps": steps, "checkpoint_every": ckpt,
-            "forward_ms": round(fwd, 3), "adjoint_ms": round(adj, 3), "adjoint_over_forward": round(adj / fwd, 3)}
+            "launch": "one launch per step over all grids", "forward_ms": round(fwd, 3), "adjoint_ms": round(adj, 3),
+            "adjoint_over_forward": round(adj / fwd, 3)}
 
 
 def main():

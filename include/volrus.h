@@ -234,6 +234,15 @@ typedef struct vr_adjoint_stats {
 } vr_adjoint_stats;
 int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_velocity* vel, double dt, int scheme, int steps,
                        int checkpoint_every, const vr_grid* target, vr_adjoint_stats* stats, float* grad);
+/* vr_adjoint_rollout for `count` level sets at once (each with its own dense
+ * field, time step and target): every forward step and every backward step
+ * is one launch over the leaves of all grids.  stats[g] (count entries, may
+ * be NULL) holds grid g's loss and the whole batch's forward / adjoint
+ * times; grads[g] (may be NULL, or hold NULL entries) receives grid g's
+ * gradient. */
+int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* const* phi0, const vr_velocity* const* vels,
+                             const double* dts, int scheme, int steps, int checkpoint_every,
+                             const vr_grid* const* targets, vr_adjoint_stats* stats, float* const* grads);
 
 /* ---- batches of level sets ----------------------------------------------- */
 /* No counterpart in the reference.  A batch advects every grid with its own

@@ -102,6 +102,8 @@ _SIGS = {
     "vr_batch_destroy": (ctypes.c_int, [_P, _P]),
     "vr_adjoint_rollout": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           _P, _P, _P]),
+    "vr_adjoint_rollout_batch": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, _P, _P, _P]),
     "vr_rollout": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, _P, _P]),
     "vr_ctx_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -360,6 +362,22 @@ class Grid:
             self.close()
         except Exception:
             pass
+
+
+def adjoint_rollout_batch(ctx, phis, vels, ns, dts, scheme, steps, checkpoint_every, targets, want_grad=True):
+    """vr_adjoint_rollout_batch: one launch per step over all level sets.
+    Returns ([AdjointStats per grid], [grad[3, n, n, n] or None per grid])."""
+    G = len(phis)
+    ph = (ctypes.c_void_p * G)(*[g.h.value if hasattr(g.h, "value") else g.h for g in phis])
+    vh = (ctypes.c_void_p * G)(*[v.h.value if hasattr(v.h, "value") else v.h for v in vels])
+    th = (ctypes.c_void_p * G)(*[t.h.value if hasattr(t.h, "value") else t.h for t in targets])
+    d = np.ascontiguousarray(dts, dtype=np.float64)
+    stats = (AdjointStats * G)()
+    grads = [np.zeros(3 * n ** 3, dtype=np.float32) for n in ns] if want_grad else None
+    gp = (ctypes.c_void_p * G)(*[g.ctypes.data for g in grads]) if want_grad else None
+    ctx.check(ctx.lib.vr_adjoint_rollout_batch(ctx.h, G, ph, vh, _ptr(d), scheme, steps, checkpoint_every, th,
+                                                stats, gp))
+    return list(stats), ([g.reshape(3, n, n, n) for g, n in zip(grads, ns)] if want_grad else None)
 
 
 BATCH_LOOP = 0

@@ -240,7 +240,7 @@ __device__ __forceinline__ float merged_lambda(const float* ob, const float* far
 // |coefficient|.
 
 template <int SCHEME>
-__device__ __forceinline__ void adj_step(const AdjStep& P) {
+__device__ __forceinline__ void adj_step(const AdjStep& P, int li, bool valid) {
   // RK3 keeps 2 scatter records per voxel (y2, y1); the scatter at x goes
   // through the separable contraction, or straight into the window.
   constexpr int NREC = SCHEME == 3 ? 2 : 0;
@@ -273,13 +273,9 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
   __shared__ int s_sep;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const GridView& g = P.phi;
-  const int n = *g.count;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double tf[3] = {P.tf0, P.tf1, P.tf2};
-  int li_end, li_step;
-  // One leaf per block: the grid has at least ceil(n / 8) blocks per XCD.
-  const int li = xcd_first(n, &li_end, &li_step);
-  if (li < li_end) {
+  if (valid) {
     const NanoLeaf& L = g.leaves[li];
     if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
     int total = 0;
@@ -601,11 +597,65 @@ __device__ __forceinline__ void adj_step(const AdjStep& P) {
 
 // RK1: held to 128 VGPRs (4 waves per SIMD; a few bytes of spill) -- faster
 // than the unconstrained allocation at 3 waves.  RK3 keeps its registers.
+// One leaf per block: the grid has at least ceil(n / 8) blocks per XCD.
 __global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adj_step1(AdjStep P) {
-  adj_step<1>(P);
+  int li_end, li_step;
+  const int li = xcd_first(*P.phi.count, &li_end, &li_step);
+  adj_step<1>(P, li, li < li_end);
 }
 __global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adj_step3(AdjStep P) {
-  adj_step<3>(P);
+  int li_end, li_step;
+  const int li = xcd_first(*P.phi.count, &li_end, &li_step);
+  adj_step<3>(P, li, li < li_end);
+}
+
+// Grid g of global leaf i: leaf_base[g] <= i < leaf_base[g + 1] (G + 1 entries).
+__device__ __forceinline__ int adj_grid_of(const int* leaf_base, int G, int i) {
+  int lo = 0, hi = G;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (leaf_base[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Several level sets in one launch: the leaves of all grids form one list
+// (grid g owns [leaf_base[g], leaf_base[g + 1])), one leaf per block, each
+// with its grid's step (its state, field, bricks and gradient).
+template <int SCHEME>
+__device__ __forceinline__ void adj_step_multi(const AdjStep* Ps, const int* leaf_base, int G, int L) {
+  int end, stride;
+  const int gl = xcd_first(L, &end, &stride);
+  const bool valid = gl < end;
+  const int g = __builtin_amdgcn_readfirstlane(valid ? adj_grid_of(leaf_base, G, gl) : 0);
+  adj_step<SCHEME>(Ps[g], gl - leaf_base[g], valid);
+}
+__global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adj_step1_multi(
+    const AdjStep* Ps, const int* leaf_base, int G, int L) {
+  adj_step_multi<1>(Ps, leaf_base, G, L);
+}
+__global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adj_step3_multi(
+    const AdjStep* Ps, const int* leaf_base, int G, int L) {
+  adj_step_multi<3>(Ps, leaf_base, G, L);
+}
+
+// The forward step of several level sets in one launch (the dense field's
+// advection, one leaf per block of NW waves): grid g reads ins[g] and writes
+// outs[g] with its own field and time step.
+template <int SCHEME, int NW>
+__global__ __launch_bounds__(64 * NW) void k_adj_advect_multi(const GridView* ins, NanoLeaf* const* outs,
+                                                             const VelView* vels, const double* dts,
+                                                             const int* leaf_base, int G, int L) {
+  __shared__ AdvectLDS s;
+  int end, stride;
+  const int gl = xcd_first(L, &end, &stride);
+  if (gl >= end) return;
+  const int g = adj_grid_of(leaf_base, G, gl);
+  const GridView in = ins[g];
+  const VelView v = vels[g];
+  NanoLeaf* const out = (VR_GLOBAL NanoLeaf*)reinterpret_cast<const uintptr_t*>(outs)[g];
+  const double tf[3] = {1.0, 1.0, 1.0};
+  advect_leaf<VK_DENSE, SCHEME, NW>(in, v, out, gl - leaf_base[g], s, threadIdx.x, dts[g], tf);
 }
 
 // lambda_K = phi_K - target at active voxels (0 elsewhere), written as each
@@ -613,14 +663,11 @@ __global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(2, 
 // merges it like any other step's bricks; per-leaf loss in f64.  The target
 // value of a voxel is Grid::get: the target leaf with the same origin holds
 // all of the leaf's voxels, so it is looked up once per leaf.
-__global__ __launch_bounds__(256) void k_adj_terminal(GridView g, GridView target, float* ob, double* leaf_loss) {
+__device__ __forceinline__ void adj_terminal_leaf(const GridView& g, const GridView& target, float* ob,
+                                                  double* leaf_loss, int li, bool valid) {
   __shared__ double s_part[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int n = *g.count;
-  int li_end, li_step;
-  // One leaf per block: the grid has at least ceil(n / 8) blocks per XCD.
-  const int li = xcd_first(n, &li_end, &li_step);
-  if (li < li_end) {
+  if (valid) {
     const NanoLeaf& L = g.leaves[li];
     const int tl = find_leaf(target, L.origin[0], L.origin[1], L.origin[2]);
     double part = 0.0;
@@ -641,6 +688,25 @@ __global__ __launch_bounds__(256) void k_adj_terminal(GridView g, GridView targe
     if (tid == 0) leaf_loss[li] = ((s_part[0] + s_part[1]) + s_part[2]) + s_part[3];
     __syncthreads();
   }
+}
+
+// One leaf per block: the grid has at least ceil(n / 8) blocks per XCD.
+__global__ __launch_bounds__(256) void k_adj_terminal(GridView g, GridView target, float* ob, double* leaf_loss) {
+  int li_end, li_step;
+  const int li = xcd_first(*g.count, &li_end, &li_step);
+  adj_terminal_leaf(g, target, ob, leaf_loss, li, li < li_end);
+}
+
+// Several level sets: grid g's bricks and per-leaf losses at its leaf_base offset.
+__global__ __launch_bounds__(256) void k_adj_terminal_multi(const GridView* gs, const GridView* targets, float* ob,
+                                                            double* leaf_loss, const int* leaf_base, int G, int L) {
+  int end, stride;
+  const int gl = xcd_first(L, &end, &stride);
+  const bool valid = gl < end;
+  const int g = valid ? adj_grid_of(leaf_base, G, gl) : 0;
+  const GridView gv = gs[g], tv = targets[g];
+  const size_t b = (size_t)leaf_base[g];
+  adj_terminal_leaf(gv, tv, ob + b * 1000, leaf_loss + b, gl - leaf_base[g], valid);
 }
 
 // Sum of n doubles in a fixed order (one block).
@@ -826,5 +892,259 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   for (int i = 0; i < 3; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
   if (rc == VR_OK && stats) *stats = st;
+  return rc;
+}
+
+// ------------------------------------------------------------------ several level sets
+// vr_adjoint_rollout for `count` level sets at once: every forward step, the
+// terminal adjoint and every backward step is one launch over the leaves of
+// all grids.  States are kept as one leaf array over all grids (grid g at
+// leaf_base[g]); the per-grid arguments of every launch (views of the state
+// it reads, the buffers it writes) are laid out on the host in execution
+// order and uploaded once.
+extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* const* phi0,
+                                        const vr_velocity* const* vels, const double* dts, int scheme, int steps,
+                                        int checkpoint_every, const vr_grid* const* targets, vr_adjoint_stats* stats,
+                                        float* const* grads) {
+  if (!ctx || count < 1 || !phi0 || !vels || !dts || !targets || steps < 1 || checkpoint_every < 1)
+    return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "bad adjoint arguments");
+  if (scheme != VR_SCHEME_RK1 && scheme != VR_SCHEME_RK3)
+    return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "scheme must be 1 or 3");
+  const int G = count;
+  std::vector<int> leaf_base((size_t)G + 1, 0);
+  std::vector<size_t> goff((size_t)G + 1, 0);
+  for (int g = 0; g < G; ++g) {
+    if (!phi0[g] || !vels[g] || !targets[g]) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "null grid %d", g);
+    if (vels[g]->kind != VK_DENSE || vels[g]->period > 0.0)
+      return vr_fail(ctx, VR_ERR_UNSUPPORTED, "grid %d: the adjoint needs a static dense velocity field", g);
+    if (phi0[g]->topo->has_empty)
+      return vr_fail(ctx, VR_ERR_UNSUPPORTED, "grid %d: the adjoint needs leaves that all hold active voxels", g);
+    leaf_base[g + 1] = leaf_base[g] + phi0[g]->topo->count;
+    goff[g + 1] = goff[g] + 3 * (size_t)vels[g]->n * vels[g]->n * vels[g]->n;
+    if ((long long)leaf_base[g + 1] > 0x7fffffffLL / 1000) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "batch too large");
+  }
+  hipError_t e;
+  if ((e = hipSetDevice(ctx->device)) != hipSuccess) return vr_hip_fail(ctx, e, "hipSetDevice");
+  const int L = leaf_base[G];
+  const size_t cap = (size_t)(L < 1 ? 1 : L);
+  const int c = checkpoint_every;
+  const int nseg = (steps + c - 1) / c;
+  std::vector<NanoLeaf*> ck((size_t)nseg, nullptr), seg((size_t)c, nullptr);
+  NanoLeaf* fin = nullptr;
+  float *ob[2] = {nullptr, nullptr}, *far[2] = {nullptr, nullptr}, *grad = nullptr;
+  double *leaf_loss = nullptr, *loss_d = nullptr;
+  int* d_lb = nullptr;
+  VelView* d_vel = nullptr;
+  double* d_dts = nullptr;
+  GridView *d_fin = nullptr, *d_tgt = nullptr, *d_fwd_in = nullptr;
+  NanoLeaf** d_fwd_out = nullptr;
+  AdjStep* d_bwd = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  int rc = VR_OK;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (rc != VR_OK) return;
+    if ((e = hipMalloc(p, bytes)) != hipSuccess) rc = vr_hip_fail(ctx, e, "hipMalloc");
+  };
+  for (int k = 0; k < nseg; ++k) alloc((void**)&ck[k], sizeof(NanoLeaf) * cap);
+  for (int j = 1; j < c; ++j) alloc((void**)&seg[j], sizeof(NanoLeaf) * cap);
+  alloc((void**)&fin, sizeof(NanoLeaf) * cap);
+  for (int i = 0; i < 2; ++i) {
+    alloc((void**)&ob[i], sizeof(float) * 1000 * cap);
+    alloc((void**)&far[i], sizeof(float) * 512 * cap);
+  }
+  alloc((void**)&grad, sizeof(float) * (goff[G] ? goff[G] : 1));
+  alloc((void**)&leaf_loss, sizeof(double) * cap);
+  alloc((void**)&loss_d, sizeof(double) * G);
+  alloc((void**)&d_lb, sizeof(int) * (G + 1));
+  alloc((void**)&d_vel, sizeof(VelView) * G);
+  alloc((void**)&d_dts, sizeof(double) * G);
+  alloc((void**)&d_fin, sizeof(GridView) * G);
+  alloc((void**)&d_tgt, sizeof(GridView) * G);
+  for (int i = 0; i < 3 && rc == VR_OK; ++i)
+    if ((e = hipEventCreate(&ev[i])) != hipSuccess) rc = vr_hip_fail(ctx, e, "hipEventCreate");
+  // Views of grid g over a state buffer (phi0's topology).
+  std::vector<std::unique_ptr<vr_grid>> views;
+  std::vector<int> separable((size_t)G, 1);
+  std::vector<VelView> vv((size_t)G);
+  for (int g = 0; g < G; ++g) {
+    std::unique_ptr<vr_grid> v(new vr_grid());
+    copy_header_into(v.get(), phi0[g]);
+    v->topo = phi0[g]->topo;
+    views.push_back(std::move(v));
+    vv[g] = make_vel_view(vels[g]);
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q)
+        if (r == q ? !(phi0[g]->mat[r][q] > 0.0) : phi0[g]->mat[r][q] != 0.0) separable[g] = 0;
+  }
+  auto view_of = [&](int g, NanoLeaf* state) {
+    views[g]->d_leaves = state + leaf_base[g];
+    const GridView gv = make_view(views[g].get());
+    views[g]->d_leaves = nullptr;  // not owned
+    return gv;
+  };
+  // The launch sequence, laid out on the host in execution order.
+  std::vector<GridView> fwd_in, fin_v, tgt_v;
+  std::vector<NanoLeaf*> fwd_out;
+  std::vector<AdjStep> bwd;
+  struct Op {
+    int kind;  // 0 forward, 1 backward
+    int index;
+  };
+  std::vector<Op> ops;
+  auto add_forward = [&](NanoLeaf* src, NanoLeaf* dst) {
+    ops.push_back({0, (int)(fwd_in.size() / G)});
+    for (int g = 0; g < G; ++g) {
+      fwd_in.push_back(view_of(g, src));
+      fwd_out.push_back(dst + leaf_base[g]);
+    }
+  };
+  NanoLeaf* cur = ck[0];
+  for (int s = 0; s < steps; ++s) {
+    NanoLeaf* dst = s + 1 == steps ? fin : ((s + 1) % c == 0 ? ck[(s + 1) / c] : seg[(s + 1) % c]);
+    add_forward(cur, dst);
+    cur = dst;
+  }
+  const size_t n_forward_ops = ops.size();
+  for (int g = 0; g < G; ++g) {
+    fin_v.push_back(view_of(g, fin));
+    tgt_v.push_back(make_view(targets[g]));
+  }
+  int bi = 0;
+  for (int k = nseg - 1; k >= 0; --k) {
+    const int s0 = k * c, s1 = (k + 1) * c < steps ? (k + 1) * c : steps;
+    NanoLeaf* prev = ck[k];
+    for (int s = s0 + 1; s < s1; ++s) {
+      add_forward(prev, seg[s - s0]);
+      prev = seg[s - s0];
+    }
+    for (int s = s1 - 1; s >= s0; --s) {
+      NanoLeaf* state = (s == s0) ? ck[k] : seg[s - s0];
+      ops.push_back({1, (int)(bwd.size() / G)});
+      for (int g = 0; g < G; ++g) {
+        const size_t lb = (size_t)leaf_base[g];
+        AdjStep P;
+        P.phi = view_of(g, state);
+        P.vel = vv[g];
+        P.ob_in = ob[bi] + lb * 1000;
+        P.far_in = far[bi] + lb * 512;
+        P.ob_out = ob[bi ^ 1] + lb * 1000;
+        P.far_out = far[bi ^ 1] + lb * 512;
+        P.grad = grad + goff[g];
+        P.dt = dts[g];
+        P.tf0 = P.tf1 = P.tf2 = 1.0;
+        P.separable = separable[g];
+        bwd.push_back(P);
+      }
+      bi ^= 1;
+    }
+  }
+  alloc((void**)&d_fwd_in, sizeof(GridView) * (fwd_in.size() ? fwd_in.size() : 1));
+  alloc((void**)&d_fwd_out, sizeof(NanoLeaf*) * (fwd_out.size() ? fwd_out.size() : 1));
+  alloc((void**)&d_bwd, sizeof(AdjStep) * (bwd.size() ? bwd.size() : 1));
+  auto h2d = [&](void* dst, const void* src, size_t bytes) {
+    if (rc == VR_OK && bytes && (e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+      rc = vr_hip_fail(ctx, e, "hipMemcpyAsync");
+  };
+  std::vector<double> dtv(dts, dts + G);
+  h2d(d_lb, leaf_base.data(), sizeof(int) * (G + 1));
+  h2d(d_vel, vv.data(), sizeof(VelView) * G);
+  h2d(d_dts, dtv.data(), sizeof(double) * G);
+  h2d(d_fin, fin_v.data(), sizeof(GridView) * G);
+  h2d(d_tgt, tgt_v.data(), sizeof(GridView) * G);
+  h2d(d_fwd_in, fwd_in.data(), sizeof(GridView) * fwd_in.size());
+  h2d(d_fwd_out, fwd_out.data(), sizeof(NanoLeaf*) * fwd_out.size());
+  h2d(d_bwd, bwd.data(), sizeof(AdjStep) * bwd.size());
+  for (int g = 0; g < G && rc == VR_OK; ++g)
+    if (phi0[g]->topo->count &&
+        (e = hipMemcpyAsync(ck[0] + leaf_base[g], phi0[g]->d_leaves, sizeof(NanoLeaf) * phi0[g]->topo->count,
+                            hipMemcpyDeviceToDevice, ctx->stream)) != hipSuccess)
+      rc = vr_hip_fail(ctx, e, "copy");
+  if (rc == VR_OK) {
+    hipLaunchKernelGGL(k_zero_f32, dim3(grid_blocks(goff[G], 256)), dim3(256), 0, ctx->stream, grad, goff[G]);
+    for (int i = 0; i < 2; ++i)
+      hipLaunchKernelGGL(k_zero_f32, dim3(grid_blocks(512 * cap, 256)), dim3(256), 0, ctx->stream, far[i], 512 * cap);
+  }
+  const int nw = vr_advect_waves(L, ctx->cus) == 1 ? 1 : 4;
+  auto launch_forward = [&](int index) {
+    const GridView* in = d_fwd_in + (size_t)index * G;
+    NanoLeaf* const* out = d_fwd_out + (size_t)index * G;
+    const int blocks = leaf_blocks((size_t)L);
+    if (scheme == VR_SCHEME_RK3) {
+      if (nw == 1)
+        hipLaunchKernelGGL((k_adj_advect_multi<3, 1>), dim3(blocks), dim3(64), 0, ctx->stream, in, out, d_vel, d_dts,
+                           d_lb, G, L);
+      else
+        hipLaunchKernelGGL((k_adj_advect_multi<3, 4>), dim3(blocks), dim3(256), 0, ctx->stream, in, out, d_vel, d_dts,
+                           d_lb, G, L);
+    } else {
+      if (nw == 1)
+        hipLaunchKernelGGL((k_adj_advect_multi<1, 1>), dim3(blocks), dim3(64), 0, ctx->stream, in, out, d_vel, d_dts,
+                           d_lb, G, L);
+      else
+        hipLaunchKernelGGL((k_adj_advect_multi<1, 4>), dim3(blocks), dim3(256), 0, ctx->stream, in, out, d_vel, d_dts,
+                           d_lb, G, L);
+    }
+  };
+  if (rc == VR_OK) (void)hipEventRecord(ev[0], ctx->stream);
+  for (size_t o = 0; o < n_forward_ops && rc == VR_OK; ++o) launch_forward(ops[o].index);
+  if (rc == VR_OK) {
+    (void)hipEventRecord(ev[1], ctx->stream);
+    hipLaunchKernelGGL(k_adj_terminal_multi, dim3(leaf_blocks((size_t)L)), dim3(256), 0, ctx->stream, d_fin, d_tgt,
+                       ob[0], leaf_loss, d_lb, G, L);
+    for (int g = 0; g < G; ++g)
+      hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(256), 0, ctx->stream, leaf_loss + leaf_base[g], phi0[g]->topo->d_count,
+                         loss_d + g);
+  }
+  for (size_t o = n_forward_ops; o < ops.size() && rc == VR_OK; ++o) {
+    if (ops[o].kind == 0) {
+      launch_forward(ops[o].index);
+    } else {
+      const AdjStep* P = d_bwd + (size_t)ops[o].index * G;
+      if (scheme == VR_SCHEME_RK3)
+        hipLaunchKernelGGL(k_adj_step3_multi, dim3(leaf_blocks((size_t)L)), dim3(ADJ_THREADS), 0, ctx->stream, P, d_lb,
+                           G, L);
+      else
+        hipLaunchKernelGGL(k_adj_step1_multi, dim3(leaf_blocks((size_t)L)), dim3(ADJ_THREADS), 0, ctx->stream, P, d_lb,
+                           G, L);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) rc = vr_hip_fail(ctx, e, "adjoint launch");
+  }
+  if (rc == VR_OK) (void)hipEventRecord(ev[2], ctx->stream);
+  if (rc == VR_OK && (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) rc = vr_hip_fail(ctx, e, "adjoint sync");
+  std::vector<double> loss((size_t)G, 0.0);
+  float fms = 0.f, bms = 0.f;
+  if (rc == VR_OK) {
+    (void)hipEventElapsedTime(&fms, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&bms, ev[1], ev[2]);
+    if ((e = hipMemcpy(loss.data(), loss_d, sizeof(double) * G, hipMemcpyDeviceToHost)) != hipSuccess)
+      rc = vr_hip_fail(ctx, e, "loss readback");
+  }
+  for (int g = 0; g < G && rc == VR_OK; ++g) {
+    if (grads && grads[g] &&
+        (e = hipMemcpy(grads[g], grad + goff[g], sizeof(float) * (goff[g + 1] - goff[g]), hipMemcpyDeviceToHost)) !=
+            hipSuccess)
+      rc = vr_hip_fail(ctx, e, "grad readback");
+    if (rc == VR_OK && stats) {
+      uint64_t lc = 0, ac = 0;
+      rc = vr_grid_counts(ctx, phi0[g], &lc, &ac);
+      vr_adjoint_stats& st = stats[g];
+      std::memset(&st, 0, sizeof(st));
+      st.loss = loss[g];
+      st.forward_ms = fms;
+      st.adjoint_ms = bms;
+      st.steps = (uint64_t)steps;
+      st.active_voxels = ac;
+    }
+  }
+  for (NanoLeaf* p : ck)
+    if (p) (void)hipFree(p);
+  for (int j = 1; j < c; ++j)
+    if (seg[j]) (void)hipFree(seg[j]);
+  for (void* p : {(void*)fin, (void*)ob[0], (void*)ob[1], (void*)far[0], (void*)far[1], (void*)grad, (void*)leaf_loss,
+                  (void*)loss_d, (void*)d_lb, (void*)d_vel, (void*)d_dts, (void*)d_fin, (void*)d_tgt, (void*)d_fwd_in,
+                  (void*)d_fwd_out, (void*)d_bwd})
+    if (p) (void)hipFree(p);
+  for (int i = 0; i < 3; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
   return rc;
 }

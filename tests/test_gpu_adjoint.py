@@ -4,6 +4,8 @@ against central differences of the GPU's own forward loss."""
 import numpy as np
 import pytest
 
+from util import active_voxels
+
 pytestmark = pytest.mark.gpu
 
 
@@ -109,3 +111,40 @@ def test_adjoint_fine_field_uses_atomic_scatter(orc, ctx, vr, scheme):
     assert np.abs(grad - g64).max() <= 2e-3 * np.abs(g64).max()
     assert np.linalg.norm(grad - g64) / np.linalg.norm(g64) < 1e-3
     assert abs(st.loss - loss64) <= 1e-4 * loss64
+
+
+@pytest.mark.parametrize("scheme,ckpt,steps", [(3, 2, 6), (1, 1, 5)])
+def test_adjoint_batch_matches_per_grid(orc, ctx, vr, scheme, ckpt, steps):
+    """Several level sets in one launch per step (vr_adjoint_rollout_batch):
+    noisy spheres and a sphere of 137 leaves, each with its own dense field
+    (of its own size), time step and target; each grid's loss equals its own
+    f32 rollout's, and its gradient the FP64 CPU adjoint's and the per-grid
+    GPU call's."""
+    announce(f"adjoint batch scheme={scheme} ckpt={ckpt}")
+    vs = 1.0 / 64
+    grids = [orc.scene([(2, (0.5, 0.5, 0.5, 0.3, 0.02, 8.0, 1000 + i))], [0], vs, 3.0) for i in range(2)]
+    grids.append(orc.sphere(0.27, (0.5, 0.5, 0.5), vs, 3.0))
+    targets = [orc.sphere(0.3, (0.55, 0.5, 0.5), vs, 3.0) for _ in grids]
+    ns = [16, 12, 16]
+    us = [np.random.default_rng(200 + i).normal(0.0, 0.5, size=(3, n, n, n)).astype(np.float32) for i, n in enumerate(ns)]
+    dts = [0.5 * vs / float(np.abs(u).max()) for u in us]
+    dg = [ctx.upload(g.to_nano()) for g in grids]
+    dt_ = [ctx.upload(t.to_nano()) for t in targets]
+    vels = [ctx.velocity_dense(u, n, 1.0 / n) for u, n in zip(us, ns)]
+    stats, grads = vr.adjoint_rollout_batch(ctx, dg, vels, ns, dts, scheme, steps, ckpt, dt_)
+    for i in range(len(grids)):
+        fin = orc.rollout_dense(grids[i], us[i], ns[i], 1.0 / ns[i], dts[i], scheme, steps)
+        c, v = active_voxels(fin.to_nano())
+        t = np.array([targets[i].get(*xyz) for xyz in c.tolist()], dtype=np.float64)
+        loss_f32 = 0.5 * float(((v.astype(np.float64) - t) ** 2).sum())
+        assert abs(stats[i].loss - loss_f32) <= 1e-12 * loss_f32, i
+        st1, g1 = dg[i].adjoint_rollout(vels[i], ns[i], dts[i], scheme, steps, ckpt, dt_[i])
+        assert st1.loss == stats[i].loss
+        scale = np.abs(g1).max()
+        assert np.abs(grads[i] - g1).max() <= 1e-5 * scale, i
+        loss64, g64 = orc.rollout_adjoint_f64(grids[i], us[i].astype(np.float64), ns[i], 1.0 / ns[i], dts[i], scheme,
+                                              steps, targets[i])
+        g64 = g64.reshape(3, ns[i], ns[i], ns[i])
+        assert np.abs(grads[i] - g64).max() <= 2e-3 * np.abs(g64).max(), i
+        assert np.linalg.norm(grads[i] - g64) / np.linalg.norm(g64) < 1e-3, i
+    assert stats[0].forward_ms > 0 and stats[0].adjoint_ms > 0
