@@ -124,6 +124,7 @@ struct AdvectLDS {
   uint16_t list[512];
   int nb[VR_NBR];
   uint64_t act[8];  // the leaf's active mask
+  uint32_t sgn[16];  // sign mask of the results (SIGNS)
   double vtab[32];  // single-vortex row tables: sin x, cos x, sin y, cos y (8 each)
 };
 
@@ -135,10 +136,15 @@ struct AdvectLDS {
 // Output origin and mask are the input's (the topology does not change,
 // src/tools/advection.rs:34-53).  The background stores follow the
 // results, so each output cache line is written within a short window and
-// leaves the L2 once.
-template <int VK, int SCHEME, int NW>
+// leaves the L2 once.  SIGNS: also writes the leaf's sign mask for `iso`
+// to sg[0..15] (bit (o & 31) of word o >> 5 = value at offset o < iso,
+// background at the inactive offsets), gathered with LDS atomics as the
+// results come, so marching cubes over the output can classify cells
+// without reading values.
+template <int VK, int SCHEME, int NW, bool SIGNS = false>
 __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
-                                            AdvectLDS& s, int tid, double dt, const double tf[3]) {
+                                            AdvectLDS& s, int tid, double dt, const double tf[3],
+                                            uint32_t* sg = nullptr, float iso = 0.0f) {
   constexpr int NT = 64 * NW;
   const int lane = tid & 63, wave = tid >> 6;
   const NanoLeaf& L = in.leaves[li];
@@ -149,6 +155,7 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   constexpr int OQ = (128 + NT - 1) / NT;
   const int nbv = in.nbr[(size_t)li * VR_NBR + (tid < VR_NBR ? tid : 0)];
   const uint64_t mword = L.mask[tid & 7];  // copied to the output by threads 0..7
+  const uint32_t sword = SIGNS ? reinterpret_cast<const uint32_t*>(L.mask)[tid & 15] : 0u;
   float4 own[OQ];
 #pragma unroll
   for (int q = 0; q < OQ; ++q) {
@@ -197,17 +204,21 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
     O.mask[tid] = mword;
     s.act[tid] = mword;
   }
+  if (SIGNS && tid < 16) s.sgn[tid] = bg < iso ? ~sword : 0u;
   if (tid < 3) O.origin[tid] = tid == 0 ? ox : (tid == 1 ? oy : oz);
   if (tid == 3) O.pad = 0;
   stage_halo<NT>(in, s.nb, s.brick, tid);
   __syncthreads();
   for (int i = tid; i < total; i += NT) {
     const int off = s.list[i];
-    O.values[off] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt,
+    const float r = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt,
                                              tf, tab, s.vtab);
+    O.values[off] = r;
+    if (SIGNS && r < iso) atomicOr(&s.sgn[off >> 5], 1u << (off & 31));
   }
 #pragma unroll
   for (int x = 0; x < 8; ++x)
     if ((x % NW) == wave && !((s.act[x] >> lane) & 1ull)) O.values[x * 64 + lane] = bg;
   __syncthreads();
+  if (SIGNS && tid < 16) sg[tid] = s.sgn[tid];
 }

@@ -183,6 +183,7 @@ GridView make_view(const vr_grid* g) {
   v.roots = (decltype(v.roots))g->topo->d_roots;
   v.child = (decltype(v.child))g->topo->d_child;
   v.nbr = (decltype(v.nbr))g->topo->d_nbr;
+  v.signs = nullptr;
   v.root_count = (int)g->topo->roots.size();
   v.background = g->background;
   v.vs = g->voxel_size;

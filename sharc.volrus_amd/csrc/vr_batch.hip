@@ -37,6 +37,7 @@ struct vr_batch {
   int vk = VK_MIXED;                          // velocity kind shared by all grids
   int cus = 256;
   DevBuf d_in, d_out_views, d_vel, d_outs, d_leaf_base, d_counters, d_ranges, d_dts, d_chunks;
+  DevBuf d_signs;  // 16 words per leaf (global leaf index): sign masks of the outputs
   int chunk_lpg = 0;  // leaves per chunk the chunk table was built for
   int max_range = 0;  // most chunks in one XCD range
   bool has_dts = false;
@@ -86,25 +87,26 @@ static void batch_drop_graph(vr_batch* b) {
 template <int VK, int SCHEME, int NW, bool ONE, class LDS>
 __device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* vels, NanoLeaf* const* outs,
                                             const int4& ck, LDS& s, int tid, const double* dts, double dt,
-                                            const double tf[3]) {
+                                            const double tf[3], uint32_t* signs, const int* leaf_base, float iso) {
   const GridView in = ins[ck.x];
   const VelView v = vels[ck.x];
   // Typed global where loaded (kernel parameter types stay unqualified: the
   // host pass has no address spaces and must name the same kernel).
   NanoLeaf* const out = (VR_GLOBAL NanoLeaf*)reinterpret_cast<const uintptr_t*>(outs)[ck.x];
   const double gdt = dts ? dts[ck.x] : dt;
+  uint32_t* const sg = signs + (size_t)leaf_base[ck.x] * 16;
   // ONE: chunks of one leaf, so no loop (nothing is kept live across leaves).
   const int li_end = ONE ? ck.y + 1 : ck.y + ck.z;
   for (int li = ck.y; li < li_end; ++li) {
     if constexpr (VK != VK_MIXED) {
-      advect_leaf<VK, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf);
+      advect_leaf<VK, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso);
     } else {
       switch (v.kind) {
-        case VK_UNIFORM: advect_leaf<VK_UNIFORM, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
-        case VK_RIGID: advect_leaf<VK_RIGID, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
-        case VK_VORTEX: advect_leaf<VK_VORTEX, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
-        case VK_DENSE: advect_leaf<VK_DENSE, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
-        default: advect_leaf<VK_GRIDS3, SCHEME, NW>(in, v, out, li, s, tid, gdt, tf); break;
+        case VK_UNIFORM: advect_leaf<VK_UNIFORM, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
+        case VK_RIGID: advect_leaf<VK_RIGID, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
+        case VK_VORTEX: advect_leaf<VK_VORTEX, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
+        case VK_DENSE: advect_leaf<VK_DENSE, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
+        default: advect_leaf<VK_GRIDS3, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
       }
     }
   }
@@ -118,7 +120,8 @@ template <int VK, int SCHEME, int NW, bool ONE>
 __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, const VelView* vels,
                                                           NanoLeaf* const* outs, const int4* chunks, int* counters,
                                                           const int* ranges, int persistent, const double* dts,
-                                                          double dt, double tf0, double tf1, double tf2) {
+                                                          double dt, double tf0, double tf1, double tf2,
+                                                          uint32_t* signs, const int* leaf_base, float iso) {
   __shared__ AdvectLDS s;
   __shared__ int s_item;
   const int tid = threadIdx.x;
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, c
   const double tf[3] = {tf0, tf1, tf2};
   if (ONE || !persistent) {
     const int c = lo + (int)(blockIdx.x >> 3);
-    if (c < hi) batch_chunk<VK, SCHEME, NW, ONE>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf);
+    if (c < hi) batch_chunk<VK, SCHEME, NW, ONE>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf, signs, leaf_base, iso);
     return;
   }
   // Chunk indices read from LDS are block-uniform; readfirstlane tells the
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, c
     // Grab the next chunk now; its latency overlaps this chunk's loads.
     int next = 0;
     if (tid == 0) next = atomicAdd(&counters[xcd], 1);
-    batch_chunk<VK, SCHEME, NW, false>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf);
+    batch_chunk<VK, SCHEME, NW, false>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf, signs, leaf_base, iso);
     if (tid == 0) s_item = next;
     __syncthreads();
     c = __builtin_amdgcn_readfirstlane(lo + s_item);
@@ -151,36 +154,39 @@ __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, c
 template <int VK, int SCHEME, int NW, bool ONE>
 static void launch_batch_k(int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
                            NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges, int persistent,
-                           const double* dts, double dt, const double tf[3]) {
+                           const double* dts, double dt, const double tf[3], uint32_t* signs,
+                           const int* leaf_base, float iso) {
   hipLaunchKernelGGL((k_batch_advect<VK, SCHEME, NW, ONE>), dim3(blocks), dim3(64 * NW), 0, st, ins, vels, outs,
-                     chunks, counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2]);
+                     chunks, counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2], signs, leaf_base, iso);
 }
 
 template <int VK, int SCHEME>
 static void launch_batch_t(int nw, bool one, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
                            NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges, int persistent,
-                           const double* dts, double dt, const double tf[3]) {
+                           const double* dts, double dt, const double tf[3], uint32_t* signs,
+                           const int* leaf_base, float iso) {
   if (nw == 1) {
     if (one)
-      launch_batch_k<VK, SCHEME, 1, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf);
+      launch_batch_k<VK, SCHEME, 1, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, iso);
     else
       launch_batch_k<VK, SCHEME, 1, false>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt,
-                                           tf);
+                                           tf, signs, leaf_base, iso);
   } else {
     if (one)
-      launch_batch_k<VK, SCHEME, 4, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf);
+      launch_batch_k<VK, SCHEME, 4, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, iso);
     else
       launch_batch_k<VK, SCHEME, 4, false>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt,
-                                           tf);
+                                           tf, signs, leaf_base, iso);
   }
 }
 
 template <int SCHEME>
 static void launch_batch_s(int vk, int nw, bool one, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
                            NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges,
-                           int persistent, const double* dts, double dt, const double tf[3]) {
+                           int persistent, const double* dts, double dt, const double tf[3], uint32_t* signs,
+                           const int* leaf_base, float iso) {
 #define VR_BK(K) \
-  launch_batch_t<K, SCHEME>(nw, one, blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf)
+  launch_batch_t<K, SCHEME>(nw, one, blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, iso)
   switch (vk) {
     case VK_UNIFORM: VR_BK(VK_UNIFORM); break;
     case VK_RIGID: VR_BK(VK_RIGID); break;
@@ -282,8 +288,11 @@ extern "C" int vr_batch_create(vr_ctx* ctx, vr_grid* const* grids, const vr_velo
       (e = b->d_vel.ensure(sizeof(VelView) * count)) != hipSuccess ||
       (e = b->d_outs.ensure(sizeof(NanoLeaf*) * count)) != hipSuccess ||
       (e = b->d_leaf_base.ensure(sizeof(int) * (count + 1))) != hipSuccess ||
-      (e = b->d_counters.ensure(sizeof(int) * 8)) != hipSuccess)
+      (e = b->d_counters.ensure(sizeof(int) * 8)) != hipSuccess ||
+      (e = b->d_signs.ensure(sizeof(uint32_t) * 16 * (size_t)(b->ltot ? b->ltot : 1))) != hipSuccess)
     return vr_hip_fail(ctx, e, "hipMalloc");
+  for (int i = 0; i < count; ++i)
+    vout[i].signs = (decltype(vout[i].signs))(b->d_signs.as<uint32_t>() + (size_t)b->leaf_base[i] * 16);
   if ((e = hipMemcpyAsync(b->d_in.p, vin.data(), sizeof(GridView) * count, hipMemcpyHostToDevice, ctx->stream)) !=
           hipSuccess ||
       (e = hipMemcpyAsync(b->d_out_views.p, vout.data(), sizeof(GridView) * count, hipMemcpyHostToDevice,
@@ -328,7 +337,9 @@ extern "C" int vr_batch_set_tuning(vr_batch* b, int leaves_per_grab, int waves_p
   return VR_OK;
 }
 
-static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int scheme) {
+// Also writes the outputs' sign masks for `iso` (d_signs), which the
+// batch's marching cubes classifies cells from.
+static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int scheme, float iso) {
   if (b->chunk_lpg != b->lpg) return vr_fail(ctx, VR_ERR_INVALID_ARGUMENT, "chunk table out of date");
   double tf[3];
   // one field per grid; time factors from the first field's period (uniform fields use none)
@@ -345,14 +356,16 @@ static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int schem
   {
     TimedLaunch tl(ctx, TC_ADVECT, 0.0, bytes, 0.0);
     const double* dts = b->has_dts ? b->d_dts.as<double>() : nullptr;
+    uint32_t* signs = b->d_signs.as<uint32_t>();
+    const int* leaf_base = b->d_leaf_base.as<int>();
     if (scheme == VR_SCHEME_RK3)
       launch_batch_s<3>(b->vk, nw, one, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
                         b->d_outs.as<NanoLeaf*>(), b->d_chunks.as<int4>(), b->d_counters.as<int>(),
-                        b->d_ranges.as<int>(), persistent, dts, dt, tf);
+                        b->d_ranges.as<int>(), persistent, dts, dt, tf, signs, leaf_base, iso);
     else
       launch_batch_s<1>(b->vk, nw, one, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
                         b->d_outs.as<NanoLeaf*>(), b->d_chunks.as<int4>(), b->d_counters.as<int>(),
-                        b->d_ranges.as<int>(), persistent, dts, dt, tf);
+                        b->d_ranges.as<int>(), persistent, dts, dt, tf, signs, leaf_base, iso);
   }
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "batch advect");
@@ -378,7 +391,7 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
     if (rc != VR_OK) return rc;
     b->tiles_ready = true;
   }
-  rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->ltot, b->tcap, iso);
+  rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->ltot, b->tcap, iso, true);
   if (rc != VR_OK) return rc;
   double mesh_bytes = 0.0;  // known to the host after a sizing run only (timing is off in graphs)
   if (sizing) {
@@ -404,7 +417,7 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
     }
   }
   return mc_emit_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->G, b->tcap, iso, b->d_verts, b->d_tris,
-                       (double)b->ltot, mesh_bytes, (long long)b->vcap, (long long)b->tricap);
+                       (double)b->ltot, mesh_bytes, (long long)b->vcap, (long long)b->tricap, nullptr, true);
 }
 
 static int batch_read_ranges(vr_ctx* ctx, vr_batch* b) {
@@ -450,7 +463,7 @@ extern "C" int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int 
   if (b->sized && (b->s_dt != dt || b->s_t != t || b->s_scheme != scheme || b->s_iso != iso)) b->sized = false;
   if (!b->sized || mode == VR_BATCH_EAGER) {
     b->sized = false;
-    int rc = batch_advect(ctx, b, dt, t, scheme);
+    int rc = batch_advect(ctx, b, dt, t, scheme, iso);
     if (rc == VR_OK) rc = batch_mc(ctx, b, iso, true);
     if (rc == VR_OK) rc = batch_read_ranges(ctx, b);
     if (rc != VR_OK) return rc;
@@ -471,7 +484,7 @@ extern "C" int vr_batch_step(vr_ctx* ctx, vr_batch* b, double dt, double t, int 
       ctx->timing = timing;
       return vr_hip_fail(ctx, e, "hipStreamBeginCapture");
     }
-    int rc = batch_advect(ctx, b, dt, t, scheme);
+    int rc = batch_advect(ctx, b, dt, t, scheme, iso);
     if (rc == VR_OK) rc = batch_mc(ctx, b, iso, false);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(ctx->stream, &g);
@@ -517,9 +530,9 @@ extern "C" int vr_batch_autotune(vr_ctx* ctx, vr_batch* b, double dt, int scheme
       b->lpg = lp;
       b->wpc = wp;
       rc = batch_chunks(ctx, b);
-      if (rc == VR_OK) rc = batch_advect(ctx, b, dt, 0.0, scheme);  // warm
+      if (rc == VR_OK) rc = batch_advect(ctx, b, dt, 0.0, scheme, 0.0f);  // warm
       (void)hipEventRecord(a, ctx->stream);
-      for (int r = 0; r < 3 && rc == VR_OK; ++r) rc = batch_advect(ctx, b, dt, 0.0, scheme);
+      for (int r = 0; r < 3 && rc == VR_OK; ++r) rc = batch_advect(ctx, b, dt, 0.0, scheme, 0.0f);
       (void)hipEventRecord(z, ctx->stream);
       if (rc != VR_OK || (e = hipEventSynchronize(z)) != hipSuccess) break;
       float ms = 0.f;

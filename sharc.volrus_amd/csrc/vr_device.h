@@ -49,6 +49,10 @@ struct GridView {
   const VR_GLOBAL int4* roots;
   const VR_GLOBAL int* child;
   const VR_GLOBAL int* nbr;
+  // Optional sign masks, 16 words per leaf: bit (offset & 31) of word
+  // offset >> 5 = value at `offset` < the iso value they were made for
+  // (written by the batched advection for its marching cubes; else null).
+  const VR_GLOBAL uint32_t* signs;
   int root_count;
   float background;
   int diag;         // mat has zero off-diagonal terms and a positive diagonal

@@ -195,12 +195,16 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
                    const GridView* single = nullptr);
 int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
                     int tcap);
-int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso);
-// Timed (class TC_MC_EMIT) with algorithmic bytes 2128 * leaf_count (the
-// leaves it reads) + mesh_bytes (24 per vertex + 12 per triangle written).
+// masks: the views carry sign masks for `iso` (GridView::signs), which the
+// cells are classified from instead of the values.
+int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso,
+                   bool masks = false);
+// Timed (class TC_MC_EMIT) with algorithmic bytes per leaf (2128, the leaf,
+// or 64, its sign mask, with masks) * leaf_count + mesh_bytes (24 per vertex
+// + 12 per triangle written).
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int tcap, float iso, double* verts,
                   uint32_t* tris, double leaf_count, double mesh_bytes, long long vcap = -1, long long tricap = -1,
-                  int* counts_out = nullptr);
+                  int* counts_out = nullptr, bool masks = false);
 // active_out (optional): receives += the active voxels after the rebuild.
 int rebuild_impl(vr_ctx* ctx, vr_grid* g, float half_width, NanoLeaf* tmp, bool exact = true,
                  int* count_copy = nullptr, unsigned long long* active_out = nullptr);

@@ -546,26 +546,80 @@ __device__ __forceinline__ void mc_half_bits(const MCHalf& q, float bg, float is
   if (!odd) rows[r] = bits;
 }
 
+// Sign rows from the grid's sign masks (16 words per leaf), requested one
+// tile ahead: lane l holds the mask words of rows l and l + 64 (< 81).  Row
+// r = x * 9 + y has z = 0..7 in byte y & 3 of word (x & 7) * 2 + (y & 7) / 4
+// of its leaf and z = 8 in the lowest bit of that byte in the +z neighbour;
+// an absent leaf has the background's sign (mc_mask_rows).  tlr: the tile's
+// neighbour leaves, lane d holding entry d (d < 27).  Whole wave.
+struct MCMaskQ {
+  uint32_t a0, b0, a1, b1;  // row leaf and +z neighbour words of the two rows
+  uint32_t ok;              // bit k: the leaf of word k (a0, b0, a1, b1) is present
+};
+__device__ __forceinline__ MCMaskQ mc_mask_issue(const VR_GLOBAL uint32_t* sg, int tlr, int lane) {
+  MCMaskQ q;
+  q.ok = 0u;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int r = k == 0 ? lane : (lane < 81 - 64 ? lane + 64 : 80);
+    const int x = (r * 57) >> 9, y = r - 9 * x;  // r / 9 for r < 81
+    const int d = ((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + 1;
+    const int la = __shfl(tlr, d), lb = __shfl(tlr, d + 1);
+    const int wi = (x & 7) * 2 + ((y & 7) >> 2);
+    const uint32_t wa = sg[(size_t)(la < 0 ? 0 : la) * 16 + wi];
+    const uint32_t wb = sg[(size_t)(lb < 0 ? 0 : lb) * 16 + wi];
+    if (k == 0) {
+      q.a0 = wa;
+      q.b0 = wb;
+    } else {
+      q.a1 = wa;
+      q.b1 = wb;
+    }
+    q.ok |= ((uint32_t)(la >= 0) | ((uint32_t)(lb >= 0) << 1)) << (2 * k);
+  }
+  return q;
+}
+// bgs: 0x1FF when the background is below iso, else 0.
+__device__ __forceinline__ void mc_mask_rows(const MCMaskQ& q, uint32_t bgs, int lane, uint32_t* rows) {
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int r = lane + 64 * k;
+    const int x = (r * 57) >> 9, y = r - 9 * x;
+    const int sh = (y & 3) * 8;
+    const uint32_t wa = k == 0 ? q.a0 : q.a1, wb = k == 0 ? q.b0 : q.b1;
+    const uint32_t lo = ((q.ok >> (2 * k)) & 1u) ? (wa >> sh) & 0xFFu : bgs & 0xFFu;
+    const uint32_t hi = ((q.ok >> (2 * k + 1)) & 1u) ? (wb >> sh) & 1u : bgs >> 8;
+    if (k == 0 || lane < 81 - 64) rows[r] = lo | (hi << 8);
+  }
+}
+
 // Loads the tile's neighbour lists (fetched into lanes < 27), stages its
-// sign rows, lists its mixed cells (corners on both sides of iso: a case
+// sign rows (MASKS: from the mask words q, else from the values), lists
+// its mixed cells (corners on both sides of iso: a case
 // other than 0 or 255) in visiting order and builds the per-class self
 // masks (no candidate ordered before the cell lies in a processed tile;
 // s_before is c_mc_before in LDS).  Returns the mixed-cell count.  Whole
 // wave, uniform control flow.
-__device__ __forceinline__ int mc_tile_setup(const VR_GLOBAL NanoLeaf* leaves, float bg, float iso, int tl, int tt,
-                                             uint32_t processed, const uint32_t* s_before, MCTileLDS& w) {
+template <bool MASKS>
+__device__ __forceinline__ int mc_tile_setup(const VR_GLOBAL NanoLeaf* leaves, const MCMaskQ& q, float bg, float iso,
+                                             int tl, int tt, uint32_t processed, const uint32_t* s_before,
+                                             MCTileLDS& w) {
   const int lane = lane_id();
   if (lane < VR_NBR) {
     w.tleaf[lane] = tl;
     w.ttile[lane] = tt;
   }
   wave_lds_sync();
-  const MCHalf q0 = mc_half_load(leaves, w.tleaf, lane), q1 = mc_half_load(leaves, w.tleaf, lane + 64);
-  const MCHalf q2 = mc_half_load(leaves, w.tleaf, lane < 162 - 128 ? lane + 128 : 161);
-  mc_use6(q0.a.w, q1.a.w, q2.a.w, q0.c, q1.c, q2.c);
-  mc_half_bits(q0, bg, iso, lane, w.rows);
-  mc_half_bits(q1, bg, iso, lane + 64, w.rows);
-  if (lane < 162 - 128) mc_half_bits(q2, bg, iso, lane + 128, w.rows);
+  if constexpr (MASKS) {
+    mc_mask_rows(q, bg < iso ? 0x1FFu : 0u, lane, w.rows);
+  } else {
+    const MCHalf q0 = mc_half_load(leaves, w.tleaf, lane), q1 = mc_half_load(leaves, w.tleaf, lane + 64);
+    const MCHalf q2 = mc_half_load(leaves, w.tleaf, lane < 162 - 128 ? lane + 128 : 161);
+    mc_use6(q0.a.w, q1.a.w, q2.a.w, q0.c, q1.c, q2.c);
+    mc_half_bits(q0, bg, iso, lane, w.rows);
+    mc_half_bits(q1, bg, iso, lane + 64, w.rows);
+    if (lane < 162 - 128) mc_half_bits(q2, bg, iso, lane + 128, w.rows);
+  }
   if (lane < 27) {
     uint32_t m = 0;
 #pragma unroll
@@ -630,29 +684,44 @@ __device__ __forceinline__ int mc_foreign_index(int x, int y, int z, int e, uint
   return ok ? ob - vrel + (int)(om >> 12) + __popc(om & 0xFFFu & ((1u << ep) - 1u)) : 0;
 }
 
-// Corner value at tile-local (x, y, z), 0..8 each (leaf 0 is read and
-// discarded where the leaf is absent, so the two corner loads overlap).
-__device__ __forceinline__ float mc_corner(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, int x, int y,
-                                           int z) {
-  const int l = tl[((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + ((z >> 3) + 1)];
-  const float v = leaves[l < 0 ? 0 : l].values[voxel_offset(x, y, z)];
-  return l < 0 ? bg : v;
-}
-
-// Vertex on owned local edge e of cell (x, y, z) (volume_to_mesh.rs:162-186):
-// t = (iso - va) / (vb - va) in f32 (0.5 when |vb - va| <= 1e-10), position
-// pa + t (pb - pa) in f64 from the corners' world positions (index * vs).
-// An edge spans one axis; on the other two pb - pa is +0 exactly, so
-// pa + t (pb - pa) is pa there and only the spanned axis is interpolated.
-__device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, int x, int y,
-                                                int z, int e, const int4& to, double vs, float iso,
-                                                const uint32_t* s_einfo, double* out) {
+// The two corner values of local edge e of cell (x, y, z), requested with
+// their leaves' presence (an absent leaf's slot reads leaf 0 and is
+// replaced by the background in mc_vertex_store), so that a round's loads
+// can be issued together and consumed later.
+struct MCCorner {
+  float va, vb;
+  int ok;  // bit 0: corner a's leaf present, bit 1: corner b's
+};
+__device__ __forceinline__ MCCorner mc_corner_load(const VR_GLOBAL NanoLeaf* leaves, const int* tl, int x, int y, int z,
+                                                   int e, const uint32_t* s_einfo) {
   const uint32_t ei = s_einfo[e];
   const int ao = (int)((ei >> 6) & 7u), bo = (int)((ei >> 9) & 7u);
   const int ax = x + (ao & 1), ay = y + ((ao >> 1) & 1), az = z + (ao >> 2);
   const int bx = x + (bo & 1), by = y + ((bo >> 1) & 1), bz = z + (bo >> 2);
-  const float va = mc_corner(leaves, bg, tl, ax, ay, az);
-  const float vbv = mc_corner(leaves, bg, tl, bx, by, bz);
+  const int la = tl[((ax >> 3) + 1) * 9 + ((ay >> 3) + 1) * 3 + ((az >> 3) + 1)];
+  const int lb = tl[((bx >> 3) + 1) * 9 + ((by >> 3) + 1) * 3 + ((bz >> 3) + 1)];
+  MCCorner c;
+  c.va = leaves[la < 0 ? 0 : la].values[voxel_offset(ax, ay, az)];
+  c.vb = leaves[lb < 0 ? 0 : lb].values[voxel_offset(bx, by, bz)];
+  c.ok = (int)(la >= 0) | ((int)(lb >= 0) << 1);
+  return c;
+}
+
+// Vertex on owned local edge e of cell (x, y, z) (volume_to_mesh.rs:162-186)
+// from its corner values: t = (iso - va) / (vb - va) in f32 (0.5 when
+// |vb - va| <= 1e-10), position pa + t (pb - pa) in f64 from the corners'
+// world positions (index * vs).  An edge spans one axis; on the other two
+// pb - pa is +0 exactly, so pa + t (pb - pa) is pa there and only the
+// spanned axis is interpolated.
+__device__ __forceinline__ void mc_vertex_store(const MCCorner& cv, float bg, int x, int y, int z, int e,
+                                                const int4& to, double vs, float iso, const uint32_t* s_einfo,
+                                                double* out) {
+  const uint32_t ei = s_einfo[e];
+  const int ao = (int)((ei >> 6) & 7u), bo = (int)((ei >> 9) & 7u);
+  const int ax = x + (ao & 1), ay = y + ((ao >> 1) & 1), az = z + (ao >> 2);
+  const int bx = x + (bo & 1), by = y + ((bo >> 1) & 1), bz = z + (bo >> 2);
+  const float va = (cv.ok & 1) ? cv.va : bg;
+  const float vbv = (cv.ok & 2) ? cv.vb : bg;
   double tpar;
   if (fabsf(vbv - va) > 1e-10f)
     tpar = (double)((iso - va) / (vbv - va));
@@ -660,22 +729,39 @@ __device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves
     tpar = 0.5;
   const int span = ao ^ bo;  // 1, 2 or 4: the axis of the edge
   const double pax = (double)(to.x + ax) * vs, pay = (double)(to.y + ay) * vs, paz = (double)(to.z + az) * vs;
-  const int ia = span == 1 ? to.x + ax : (span == 2 ? to.y + ay : to.z + az);
-  const int ib = span == 1 ? to.x + bx : (span == 2 ? to.y + by : to.z + bz);
-  const double pa = (double)ia * vs, pb = (double)ib * vs;
+  // pa on the spanned axis is that axis's corner-a coordinate
+  const bool sx = span == 1, sy = span == 2, sz = span == 4;
+  const int ib = (sx ? to.x + bx : 0) + (sy ? to.y + by : 0) + (sz ? to.z + bz : 0);
+  const double pa = sx ? pax : (sy ? pay : paz), pb = (double)ib * vs;
   const double p = pa + tpar * (pb - pa);
-  out[0] = span == 1 ? p : pax;
-  out[1] = span == 2 ? p : pay;
-  out[2] = span == 4 ? p : paz;
+  out[0] = sx ? p : pax;
+  out[1] = sy ? p : pay;
+  out[2] = sz ? p : paz;
+}
+
+__device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves, float bg, const int* tl, int x, int y,
+                                                int z, int e, const int4& to, double vs, float iso,
+                                                const uint32_t* s_einfo, double* out) {
+  mc_vertex_store(mc_corner_load(leaves, tl, x, y, z, e, s_einfo), bg, x, y, z, e, to, vs, iso, s_einfo, out);
 }
 
 // ------------------------------------------------------------------ pass 1
+// Tile t's lists in one vector load per array: lanes < 27 get its neighbour
+// leaves (*l) and tiles (*m), lane 27 of *l its grid (tiles[t].w).
+__device__ __forceinline__ void mc_count_lists(const int* tleaf, const int* ttile, const int4* tiles, int t, int lane,
+                                               int* l, int* m) {
+  const int* p = lane < VR_NBR ? tleaf + (size_t)t * VR_NBR + lane : reinterpret_cast<const int*>(tiles) + (size_t)t * 4 + 3;
+  if (lane <= VR_NBR) *l = p[0];
+  if (lane < VR_NBR) *m = ttile[(size_t)t * VR_NBR + lane];
+}
+
 // Per mixed cell (in visiting order): case, owned crossing edges, triangle
 // count.  The face cells (mc_face_index >= 0) store meta = (vertex prefix
 // inside the tile << 12) | owned-edge mask for the emit pass of later tiles;
 // per-tile vertex and triangle totals.  Cells with case 0 or 255 add nothing
 // and are never visited.  Each wave loops over tiles; the next tile's
 // neighbour lists load while the current one is processed.
+template <bool MASKS>
 __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
                                                          const int* tleaf, const int* ttile, float iso,
                                                          uint32_t* meta, int* tv, int* tt, int tcap) {
@@ -695,21 +781,27 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   }
   int t_end, t_step;
   int t = xcd_first_sub(nt, wv, MC_WAVES, &t_end, &t_step);
-  int ntl = -1, ntt = -1;
-  if (t < t_end && lane < VR_NBR) {
-    ntl = tleaf[(size_t)t * VR_NBR + lane];
-    ntt = ttile[(size_t)t * VR_NBR + lane];
-  }
+  // Lists of tile t (ntl, ntt) and of the tile after it (ftl, ftt) are in
+  // flight, loaded by mc_count_lists; MASKS: so are tile t's sign-mask words
+  // (q).  Vector loads throughout: a scalar load of a later tile would hold
+  // up every LDS wait of this one.
+  int ntl = -1, ntt = -1, ftl = -1, ftt = -1;
+  if (t < t_end) mc_count_lists(tleaf, ttile, tiles, t, lane, &ntl, &ntt);
+  if (t + t_step < t_end) mc_count_lists(tleaf, ttile, tiles, t + t_step, lane, &ftl, &ftt);
+  MCMaskQ q{};
+  if (MASKS && t < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(ntl, VR_NBR)].signs, ntl, lane);
   for (; t < t_end; t += t_step) {
     const int tl = ntl, ttv = ntt;
     const uint32_t processed = (uint32_t)__ballot(lane < VR_NBR && ttv >= 0);
-    const int tn = t + t_step;
-    if (tn < t_end && lane < VR_NBR) {
-      ntl = tleaf[(size_t)tn * VR_NBR + lane];
-      ntt = ttile[(size_t)tn * VR_NBR + lane];
-    }
-    const GridView& g = views[tiles[t].w];
-    const int n = mc_tile_setup(g.leaves, g.background, iso, tl, ttv, processed, s_before, w);
+    const int tn = t + t_step, tf = tn + t_step;
+    ntl = ftl;
+    ntt = ftt;
+    if (tf < t_end) mc_count_lists(tleaf, ttile, tiles, tf, lane, &ftl, &ftt);
+    const GridView& g = views[__builtin_amdgcn_readlane(tl, VR_NBR)];
+    const MCMaskQ cq = q;
+    const int n = mc_tile_setup<MASKS>(g.leaves, cq, g.background, iso, tl, ttv, processed, s_before, w);
+    // the next tile's mask words load while this one is counted
+    if (MASKS && tn < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(ntl, VR_NBR)].signs, ntl, lane);
     int vcarry = 0, tcarry = 0;
     for (int base = 0; base < n; base += 64) {
       const int i = base + lane;
@@ -792,6 +884,23 @@ struct MCEmitLDS {
   uint16_t fitem[28];            // per cell class: edges owned in an earlier tile that this cell resolves
 };
 
+// Tile t's lists and header in one vector load per array: lanes < 27 get
+// its neighbour leaves (*l) and tiles (*m); lanes 27..34 of *l its vertex
+// and triangle counts, first vertex and triangle, and tiles[t] (x, y, z,
+// grid).
+__device__ __forceinline__ void mc_emit_lists(const int* tleaf, const int* ttile, const int* tv, const int* tt,
+                                              const int* vbase, const int* tbase, const int4* tiles, int t, int lane,
+                                              int* l, int* m) {
+  const int* p = lane < VR_NBR ? tleaf + (size_t)t * VR_NBR + lane
+                 : lane == VR_NBR     ? tv + t
+                 : lane == VR_NBR + 1 ? tt + t
+                 : lane == VR_NBR + 2 ? vbase + t
+                 : lane == VR_NBR + 3 ? tbase + t
+                                      : reinterpret_cast<const int*>(tiles) + (size_t)t * 4 + (lane - VR_NBR - 4);
+  if (lane < VR_NBR + 8) *l = p[0];
+  if (lane < VR_NBR) *m = ttile[(size_t)t * VR_NBR + lane];
+}
+
 // Vertex index (relative to the grid) on local edge e of cell (x, y, z).
 __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const MCEmitLDS& el,
                                                const uint32_t* s_eslot, int vbrel) {
@@ -800,6 +909,7 @@ __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const
   return (v & 0x8000u) ? f : vbrel + (int)v;
 }
 
+template <bool MASKS>
 __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mc_emit(
     const GridView* views, const int4* tiles, const int* ntiles, const int* tleaf, const int* ttile,
     const int* tile_base, float iso, const uint32_t* meta, const int* tv, const int* tt, const int* vbase,
@@ -837,36 +947,32 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
   const int vmax = (int)min((long long)totals[1], vcap), tmax = (int)min((long long)totals[2], tricap);
   int t_end, t_step;
   int t = xcd_first_sub(nt, wv, MC_EWAVES, &t_end, &t_step);
-  // Header of the next tile, loaded while the current one is processed.
-  int h_nv = 0, h_nt = 0, h_vb = 0, h_tb = 0, h_tl = -1, h_tt = -1;
-  int4 h_to = make_int4(0, 0, 0, 0);
-  if (t < t_end) {
-    h_nv = tv[t];
-    h_nt = tt[t];
-    h_vb = vbase[t];
-    h_tb = tbase[t];
-    h_to = tiles[t];
-    if (lane < VR_NBR) {
-      h_tl = tleaf[(size_t)t * VR_NBR + lane];
-      h_tt = ttile[(size_t)t * VR_NBR + lane];
-    }
-  }
+  // Lists and header of tile t (h_tl, h_tt) and of the tile after it (ftl,
+  // ftt) are in flight, loaded by mc_emit_lists, so that the next tile's
+  // sign-mask words (MASKS, q) can be requested in the middle of the
+  // current tile.  Vector loads throughout: a scalar load of a later tile
+  // would hold up every LDS wait of this one.
+  int h_tl = -1, h_tt = -1, ftl = -1, ftt = -1;
+  if (t < t_end) mc_emit_lists(tleaf, ttile, tv, tt, vbase, tbase, tiles, t, lane, &h_tl, &h_tt);
+  if (t + t_step < t_end) mc_emit_lists(tleaf, ttile, tv, tt, vbase, tbase, tiles, t + t_step, lane, &ftl, &ftt);
+  MCMaskQ q{};
+  if (MASKS && t < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
   for (; t < t_end; t += t_step) {
-    const int c_nv = h_nv, c_nt = h_nt, vb = h_vb, tb = h_tb, tl = h_tl, ttv = h_tt;
-    const int4 to = h_to;
-    const int tn = t + t_step;
-    if (tn < t_end) {
-      h_nv = tv[tn];
-      h_nt = tt[tn];
-      h_vb = vbase[tn];
-      h_tb = tbase[tn];
-      h_to = tiles[tn];
-      if (lane < VR_NBR) {
-        h_tl = tleaf[(size_t)tn * VR_NBR + lane];
-        h_tt = ttile[(size_t)tn * VR_NBR + lane];
-      }
+    const int tl = h_tl, ttv = h_tt;
+    const int c_nv = __builtin_amdgcn_readlane(tl, VR_NBR), c_nt = __builtin_amdgcn_readlane(tl, VR_NBR + 1);
+    const int vb = __builtin_amdgcn_readlane(tl, VR_NBR + 2), tb = __builtin_amdgcn_readlane(tl, VR_NBR + 3);
+    const int4 to = make_int4(__builtin_amdgcn_readlane(tl, VR_NBR + 4), __builtin_amdgcn_readlane(tl, VR_NBR + 5),
+                              __builtin_amdgcn_readlane(tl, VR_NBR + 6), __builtin_amdgcn_readlane(tl, VR_NBR + 7));
+    const int tn = t + t_step, tf = tn + t_step;
+    h_tl = ftl;
+    h_tt = ftt;
+    if (tf < t_end) mc_emit_lists(tleaf, ttile, tv, tt, vbase, tbase, tiles, tf, lane, &ftl, &ftt);
+    const MCMaskQ cq = q;
+    if (c_nv == 0 && c_nt == 0) {  // nothing to write (uniform over the wave)
+      if (MASKS && tn < t_end)
+        q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
+      continue;
     }
-    if (c_nv == 0 && c_nt == 0) continue;  // nothing to write (uniform over the wave)
     const uint32_t processed = (uint32_t)__ballot(lane < VR_NBR && ttv >= 0);
     const GridView& g = views[to.w];
     const VR_GLOBAL NanoLeaf* leaves = g.leaves;
@@ -875,7 +981,8 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
     const int vbrel = vb - vrel;
-    const int n = mc_tile_setup(leaves, bg, iso, tl, ttv, processed, s_before, w);
+    const int n = mc_tile_setup<MASKS>(leaves, cq, bg, iso, tl, ttv, processed, s_before, w);
+    if (MASKS && tn < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
     // Per class: the edges owned in an earlier tile that this cell resolves
     // (it is the first cell of this tile containing them: no candidate of
     // this tile precedes it).
@@ -908,6 +1015,25 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
       const int rv = tot & 0xFFFF, rt = tot >> 16;
       const int vr = excl & 0xFFFF, tr = excl >> 16;
       const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
+      const bool listed = rv <= MC_RV && rt <= MC_RT;  // uniform
+      MCCorner cv[MC_RV / 64] = {};
+      if (listed) {
+        // The round's vertices are listed and their corner values requested
+        // first, so that the loads overlap the resolution of foreign edges.
+        if (i < n) {
+          int k = vr;
+          for (uint32_t m = owned; m; m &= m - 1u, ++k) el.vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < MC_RV / 64; ++j) {
+          const int k = lane + 64 * j;
+          if (k < rv) {
+            const int it = el.vlist[k], cc = it >> 4;
+            cv[j] = mc_corner_load(leaves, w.tleaf, cc >> 6, (cc >> 3) & 7, cc & 7, it & 15, s_einfo);
+          }
+        }
+      }
       // Edges owned in an earlier tile, one resolved per lane (each lattice
       // edge once per tile, so at most MC_FOREIGN per tile).
       const int nf = __popc(fm);
@@ -927,28 +1053,28 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
         fcarry += ftot;
         wave_lds_sync();
       }
-      if (rv <= MC_RV && rt <= MC_RT) {
-        if (i < n) {
-          int k = vr;
-          for (uint32_t m = owned; m; m &= m - 1u, ++k) el.vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
-          for (int q = 0; q < ntr; ++q) el.tlist[tr + q] = (uint32_t)c | ((uint32_t)q << 9) | (cube << 12);
-        }
-        wave_lds_sync();
-        for (int k = lane; k < rv; k += 64) {
-          const int vi = vb + vcarry + k;
-          const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
-          const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
-          el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
-          if (vi < vmax)
-            mc_write_vertex(leaves, bg, w.tleaf, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
+      if (listed) {
+        // (the triangle list shares its LDS with the foreign list above)
+        if (i < n)
+          for (int q2 = 0; q2 < ntr; ++q2) el.tlist[tr + q2] = (uint32_t)c | ((uint32_t)q2 << 9) | (cube << 12);
+#pragma unroll
+        for (int j = 0; j < MC_RV / 64; ++j) {
+          const int k = lane + 64 * j;
+          if (k < rv) {
+            const int vi = vb + vcarry + k;
+            const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
+            const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
+            el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
+            if (vi < vmax) mc_vertex_store(cv[j], bg, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
+          }
         }
         wave_lds_sync();
         for (int k = lane; k < rt; k += 64) {
           const int ti = tb + tcarry + k;
           const uint32_t it = el.tlist[k];
-          const int cc = (int)(it & 511u), q = (int)((it >> 9) & 7u);
+          const int cc = (int)(it & 511u), q2 = (int)((it >> 9) & 7u);
           const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
-          const int8_t* row = s_tri + (it >> 12) * 16 + 3 * q;
+          const int8_t* row = s_tri + (it >> 12) * 16 + 3 * q2;
           const int i0 = mc_corner_index(ux, uy, uz, row[0], el, s_eslot, vbrel);
           const int i1 = mc_corner_index(ux, uy, uz, row[1], el, s_eslot, vbrel);
           const int i2 = mc_corner_index(ux, uy, uz, row[2], el, s_eslot, vbrel);
@@ -975,10 +1101,10 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
         wave_lds_sync();
         if (i < n) {
           int ti = tb + tcarry + tr;
-          for (int q = 0; q < ntr && ti < tmax; ++q, ++ti)
+          for (int q2 = 0; q2 < ntr && ti < tmax; ++q2, ++ti)
             for (int jj = 0; jj < 3; ++jj)
               tris[3 * (size_t)ti + jj] =
-                  (uint32_t)mc_corner_index(cx, cy, cz, s_tri[cube * 16 + 3 * q + jj], el, s_eslot, vbrel);
+                  (uint32_t)mc_corner_index(cx, cy, cz, s_tri[cube * 16 + 3 * q2 + jj], el, s_eslot, vbrel);
         }
       }
       wave_lds_sync();
@@ -1071,14 +1197,15 @@ int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int*
 }
 
 // Phase C: count, scans (the tile lists of phases A and B in s).
-int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso) {
+int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso, bool masks) {
   hipError_t e;
   int* totals = s.totals.as<int>();
   {
-    TimedLaunch tl(ctx, TC_MC, (double)VR_LEAF_BYTES, 0.0, (double)ltot);
-    hipLaunchKernelGGL(k_mc_count, dim3(mc_blocks(ctx, (const void*)k_mc_count, MC_WAVES, tcap)), dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
-                       totals, s.tleaf.as<int>(), s.ttile.as<int>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(),
-                       s.tt.as<int>(), tcap);
+    TimedLaunch tl(ctx, TC_MC, masks ? 64.0 : (double)VR_LEAF_BYTES, 0.0, (double)ltot);
+    const void* k = masks ? (const void*)k_mc_count<true> : (const void*)k_mc_count<false>;
+    hipLaunchKernelGGL(masks ? k_mc_count<true> : k_mc_count<false>, dim3(mc_blocks(ctx, k, MC_WAVES, tcap)),
+                       dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(), totals, s.tleaf.as<int>(),
+                       s.ttile.as<int>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), tcap);
   }
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = vr_exclusive_scan2(s.tv.as<int>(), s.vb.as<int>(), totals + 1, s.tt.as<int>(), s.tb.as<int>(),
@@ -1091,13 +1218,15 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long l
 // per-grid ranges (s.ranges; counts_out, optional: the totals).
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int tcap, float iso, double* verts,
                   uint32_t* tris, double leaf_count, double mesh_bytes, long long vcap, long long tricap,
-                  int* counts_out) {
+                  int* counts_out, bool masks) {
   hipError_t e;
   if (vcap < 0) vcap = 0x7fffffffLL;
   if (tricap < 0) tricap = 0x7fffffffLL;
   {
-    TimedLaunch tl(ctx, TC_MC_EMIT, (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
-    hipLaunchKernelGGL(k_mc_emit, dim3(mc_blocks(ctx, (const void*)k_mc_emit, MC_EWAVES, tcap)), dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
+    TimedLaunch tl(ctx, TC_MC_EMIT, masks ? 64.0 : (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
+    const void* k = masks ? (const void*)k_mc_emit<true> : (const void*)k_mc_emit<false>;
+    hipLaunchKernelGGL(masks ? k_mc_emit<true> : k_mc_emit<false>, dim3(mc_blocks(ctx, k, MC_EWAVES, tcap)),
+                       dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
                        s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(), iso,
                        s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
                        s.totals.as<int>(), verts, tris, vcap, tricap, G, s.ranges.as<int>(), counts_out);
