@@ -35,6 +35,12 @@ struct AdjStep {
   float* ob_out;       // [cap][1000] bricks of this step
   float* far_out;      // [cap][512], corners beyond the brick (large steps)
   float* grad;         // 3 n^3
+  // Per active voxel (leaf li's i-th active voxel in offset order at
+  // abase[li] + i): the sum over the steps of lambda_{n+1} grad S(phi_n, p),
+  // p the voxel's departure point; component a at gacc + a * gstride.
+  const int* abase;
+  float* gacc;
+  long long gstride;
   double dt, tf0, tf1, tf2;
   int separable;   // grid transform diagonal with positive scales (host check)
 };
@@ -219,120 +225,107 @@ __device__ __forceinline__ float merged_lambda(const float* ob, const float* far
   return s + fv;
 }
 
-// One backward step, one leaf per block of ADJ_WAVES waves.
+// The leaf's active voxels in offset order (list[0..total)), by wave.
+__device__ __forceinline__ int adj_active_list(const NanoLeaf& L, uint16_t* list, int lane, int wave) {
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int total = 0;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    const uint64_t m = L.mask[x];
+    if ((x % ADJ_WAVES) == wave && ((m >> lane) & 1ull)) list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
+    total += __popcll(m);
+  }
+  return total;
+}
+
+// The dense field's window of the leaf at (ox, oy, oz): base node wb (two
+// nodes below the leaf origin's cell) and its nodes wb + (0..VW-1)^3 in s_u
+// (0 outside the field, as dense_at).  A thread's nodes all load before any
+// is stored (nodes outside the field read node 0).
+__device__ __forceinline__ void adj_window(const GridView& g, const VelView& v, int ox, int oy, int oz, int wb[3],
+                                           float* s_u, int tid) {
+  double ow[3];
+  index_to_world(g.mat, ox, oy, oz, ow);
+  for (int a = 0; a < 3; ++a) wb[a] = f64_to_i32(floor(ow[a] * v.inv_h)) - 2;
+  constexpr int UQ = (3 * VW3 + ADJ_THREADS - 1) / ADJ_THREADS;
+  const int nn = v.n;
+  const size_t n3 = (size_t)nn * nn * nn;
+  float uv[UQ];
+  bool inside[UQ];
+#pragma unroll
+  for (int q = 0; q < UQ; ++q) {
+    const int i0 = tid + q * ADJ_THREADS, i = i0 < 3 * VW3 ? i0 : 0;
+    const int c = i / VW3, r = i - c * VW3;
+    const int gi = wb[0] + r / (VW * VW), gj = wb[1] + (r / VW) % VW, gk = wb[2] + r % VW;
+    inside[q] = (unsigned)gi < (unsigned)nn && (unsigned)gj < (unsigned)nn && (unsigned)gk < (unsigned)nn;
+    uv[q] = v.dense[inside[q] ? c * n3 + ((size_t)gi * nn + gj) * nn + gk : 0];
+  }
+#pragma unroll
+  for (int q = 0; q < UQ; ++q) {
+    const int i0 = tid + q * ADJ_THREADS;
+    if (i0 < 3 * VW3) s_u[i0] = inside[q] ? uv[q] : 0.0f;
+  }
+}
+
+// The departure point of voxel (x, y, z) of the leaf at (ox, oy, oz): RK1 or
+// RK3 through the static field (the forward step's arithmetic, so the same
+// point), with the RK3 stage positions y1, y2.
+template <int SCHEME>
+__device__ __forceinline__ void adj_departure(const AdjStep& P, const float* s_u, const int wb[3], int px, int py,
+                                              int pz, const double tf[3], double xw[3], double y1[3], double y2[3],
+                                              double p[3]) {
+  double u[3];
+  index_to_world(P.phi.mat, px, py, pz, xw);
+  win_vel(P.vel, s_u, wb, xw, u);
+  for (int a = 0; a < 3; ++a) y1[a] = xw[a] - P.dt * (u[a] * tf[0]);
+  if (SCHEME == 1) {
+    for (int a = 0; a < 3; ++a) p[a] = y1[a];
+  } else {
+    win_vel(P.vel, s_u, wb, y1, u);
+    for (int a = 0; a < 3; ++a) y2[a] = 0.75 * xw[a] + 0.25 * (y1[a] - P.dt * (u[a] * tf[1]));
+    win_vel(P.vel, s_u, wb, y2, u);
+    for (int a = 0; a < 3; ++a) p[a] = (1.0 / 3.0) * xw[a] + (2.0 / 3.0) * (y2[a] - P.dt * (u[a] * tf[2]));
+  }
+}
+
+// One backward step for the state adjoint, one leaf per block of ADJ_WAVES
+// waves.
 //
 // lambda_{n+1} of the leaf's voxels is first merged from the previous step's
-// bricks (so no separate merge launch), then:
+// bricks (so no separate merge launch), then scattered with the trilinear
+// weights of the departure cell into this leaf's brick: each contribution
+// (float)(lambda w) is quantised to an integer with the leaf's scale
+// 2^(21 - E), max |lambda_{n+1}| over the leaf < 2^E, and summed in 32 bits
+// (a brick cell gets at most one per voxel, 512, so the sum stays below
+// 2^30).  Integer sums are exact, so lambda_n does not depend on the order of
+// the adds, and integer LDS atomics run far faster than float ones.
 //
-// lambda scatter: each contribution (float)(lambda w) is quantised to an
-// integer with the leaf's scale 2^(21 - E), max |lambda_{n+1}| over the
-// leaf < 2^E, and summed in 32 bits (a brick cell gets at most one per
-// voxel, 512, so the sum stays below 2^30).  Integer sums are exact, so
-// lambda_n does not depend on the order of the adds, and integer LDS
-// atomics run far faster than float ones.
-//
-// Velocity scatter at the voxel's own position (the RK1 term and the last
-// RK3 term): for a diagonal transform the node weights factor per axis, so
-// the per-voxel coefficients are staged in LDS and contracted axis by axis
-// into the leaf's node window (no atomics); otherwise float atomics into the
-// window.  The two RK3 stage positions are kept as records and added in a
-// second pass as int32-quantised 64-bit sums scaled by the leaf's max
-// |coefficient|.
-
+// The velocity gradient of the step is (dp/dU)^T g_n per voxel, with
+// g_n = lambda_{n+1} grad S(phi_n, p) and dp/dU the departure point's
+// derivative, which depends on the voxel, the static field and dt only: the
+// same every step.  So each voxel's g_n is summed over the steps (gacc) and
+// pulled back through dp/dU once, after the last backward step
+// (adj_vel_step), instead of once per step.
 template <int SCHEME>
-__device__ __forceinline__ void adj_step(const AdjStep& P, int li, bool valid) {
-  // RK3 keeps 2 scatter records per voxel (y2, y1); the scatter at x goes
-  // through the separable contraction, or straight into the window.
-  constexpr int NREC = SCHEME == 3 ? 2 : 0;
-  constexpr int REC_WORDS = NREC * 6 * 512;
-  constexpr int T_WORDS = 3 * 64 * VW + 3 * 8 * VW * VW;
+__device__ __forceinline__ void adj_lambda_step(const AdjStep& P, int li, bool valid) {
   __shared__ float brick[1000];
   __shared__ int acc[1000];
-  __shared__ float coefx[3][512];
-  __shared__ float vwin[3 * VW3];
   __shared__ float s_u[3 * VW3];  // the field's nodes over the window
-  __shared__ long long vwin_i[SCHEME == 3 ? 3 * VW3 : 1];
-  // records (pass 1 -> pass 2), then the separable contraction stages
-  __shared__ float s_raw[REC_WORDS > T_WORDS ? REC_WORDS : T_WORDS];
-  float* const t1 = s_raw;
-  float* const t2 = s_raw + 3 * 64 * VW;
-  // record r of a voxel: per axis the coefficient, and the cell's base node
-  // relative to the window (10 bits) with the fraction in 22-bit fixed point
-  // above it; all-zero coefficients mark no record.
-  float* const rec_c = s_raw;                                  // [NREC][3][512]
-  uint32_t* const rec_w = (uint32_t*)(s_raw + NREC * 3 * 512);  // [NREC][3][512]
-  __shared__ float s_cmax[ADJ_WAVES];
   __shared__ uint16_t list[512];
   __shared__ int nb[VR_NBR];
   __shared__ float s_max[ADJ_WAVES];
   __shared__ float s_lam[512];
-  // per axis and local coordinate: base node relative to the window, and the
-  // weight of each window node
-  __shared__ int s_bn[3][8];
-  __shared__ float s_wt[3][8][VW];
-  __shared__ int s_sep;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const GridView& g = P.phi;
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const double tf[3] = {P.tf0, P.tf1, P.tf2};
   if (valid) {
     const NanoLeaf& L = g.leaves[li];
     if (tid < VR_NBR) nb[tid] = g.nbr[(size_t)li * VR_NBR + tid];
-    int total = 0;
-#pragma unroll
-    for (int x = 0; x < 8; ++x) {
-      const uint64_t m = L.mask[x];
-      if ((x % ADJ_WAVES) == wave && ((m >> lane) & 1ull)) list[total + __popcll(m & below)] = (uint16_t)(x * 64 + lane);
-      total += __popcll(m);
-    }
+    const int total = adj_active_list(L, list, lane, wave);
     for (int i = tid; i < 1000; i += ADJ_THREADS) acc[i] = 0;
-    for (int i = tid; i < 3 * 512; i += ADJ_THREADS) (&coefx[0][0])[i] = 0.0f;
-    for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin[i] = 0.0f;
-    if constexpr (SCHEME == 3) {
-      for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin_i[i] = 0;
-      for (int i = tid; i < NREC * 3 * 512; i += ADJ_THREADS) rec_c[i] = 0.0f;
-    }
     const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
     int wb[3];
-    {
-      double ow[3];
-      index_to_world(g.mat, ox, oy, oz, ow);
-      for (int a = 0; a < 3; ++a) wb[a] = f64_to_i32(floor(ow[a] * P.vel.inv_h)) - 2;
-    }
-    if (tid < 24) {
-      // Node base and fraction per axis of the voxel positions (diagonal
-      // transform: the axis coordinate alone fixes them).
-      const int a = tid >> 3, l = tid & 7;
-      const int o = a == 0 ? ox : (a == 1 ? oy : oz);
-      const double w = g.mat[5 * a] * (double)(o + l) + g.mat[4 * a + 3];
-      const double idx = w * P.vel.inv_h;
-      const double fl = floor(idx);
-      const int bn = f64_to_i32(fl) - wb[a];
-      s_bn[a][l] = bn;
-      for (int k = 0; k < VW; ++k) s_wt[a][l][k] = (float)axis_weight(k, bn, idx - fl);
-    }
-    {
-      // A thread's window nodes all load before any is stored (nodes
-      // outside the field read node 0 and store 0).
-      constexpr int UQ = (3 * VW3 + ADJ_THREADS - 1) / ADJ_THREADS;
-      const int nn = P.vel.n;
-      const size_t n3 = (size_t)nn * nn * nn;
-      float uv[UQ];
-      bool inside[UQ];
-#pragma unroll
-      for (int q = 0; q < UQ; ++q) {
-        const int i0 = tid + q * ADJ_THREADS, i = i0 < 3 * VW3 ? i0 : 0;
-        const int c = i / VW3, r = i - c * VW3;
-        const int gi = wb[0] + r / (VW * VW), gj = wb[1] + (r / VW) % VW, gk = wb[2] + r % VW;
-        inside[q] = (unsigned)gi < (unsigned)nn && (unsigned)gj < (unsigned)nn && (unsigned)gk < (unsigned)nn;
-        uv[q] = P.vel.dense[inside[q] ? c * n3 + ((size_t)gi * nn + gj) * nn + gk : 0];
-      }
-#pragma unroll
-      for (int q = 0; q < UQ; ++q) {
-        const int i0 = tid + q * ADJ_THREADS;
-        if (i0 < 3 * VW3) s_u[i0] = inside[q] ? uv[q] : 0.0f;
-      }
-    }
+    adj_window(g, P.vel, ox, oy, oz, wb, s_u, tid);
     stage_interior<ADJ_THREADS>(g, li, brick, tid);
     __syncthreads();
     // lambda_{n+1} of the active voxels (merged), its max |.|; the leaf's
@@ -349,11 +342,6 @@ __device__ __forceinline__ void adj_step(const AdjStep& P, int li, bool valid) {
     for (int o = 32; o > 0; o >>= 1) lm = fmaxf(lm, __shfl_xor(lm, o));
     if (lane == 0) s_max[wave] = lm;
     stage_halo<ADJ_THREADS>(g, nb, brick, tid);
-    if (tid == 0) {
-      bool ok = P.separable != 0;
-      for (int a = 0; a < 3; ++a) ok = ok && s_bn[a][0] >= 0 && s_bn[a][7] + 1 < VW;
-      s_sep = ok ? 1 : 0;
-    }
     __syncthreads();
     float mx = s_max[0];
 #pragma unroll
@@ -365,53 +353,14 @@ __device__ __forceinline__ void adj_step(const AdjStep& P, int li, bool valid) {
     // below 2^30 in int32
     const float scale = ldexpf(1.0f, 21 - ex);
     const double inv_scale = ldexp(1.0, ex - 21);
-    const bool sep = s_sep != 0;
-    float cmax = 0.0f;  // max |coefficient| over this thread's records
-    // Stores a scatter record (RK3), or adds far points straight to dL/dU.
-    auto rec_put = [&](int r, int off, const double pt[3], const double c[3]) {
-      int b[3];
-      double f[3];
-      dense_weights(P.vel, pt, b, f);
-      const int r0 = b[0] - wb[0], r1 = b[1] - wb[1], r2 = b[2] - wb[2];
-      if (r0 < -512 || r0 > 511 || r1 < -512 || r1 > 511 || r2 < -512 || r2 > 511) {
-        const int nn = P.vel.n;
-        const size_t n3 = (size_t)nn * nn * nn;
-        for (int q = 0; q < 8; ++q) {
-          const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
-          const int i = b[0] + dx, j = b[1] + dy, k = b[2] + dz;
-          if ((unsigned)i >= (unsigned)nn || (unsigned)j >= (unsigned)nn || (unsigned)k >= (unsigned)nn) continue;
-          const double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
-          for (int a = 0; a < 3; ++a) atomicAdd(&P.grad[a * n3 + ((size_t)i * nn + j) * nn + k], (float)(c[a] * w));
-        }
-        return;
-      }
-      const int rr[3] = {r0, r1, r2};
-      for (int a = 0; a < 3; ++a) {
-        // f in [0, 1): 22-bit fixed point, truncated
-        const uint32_t fq = (uint32_t)(f[a] * 4194304.0);
-        rec_w[(r * 3 + a) * 512 + off] = (fq << 10) | (uint32_t)(rr[a] & 1023);
-        const float cf = (float)c[a];
-        rec_c[(r * 3 + a) * 512 + off] = cf;
-        cmax = fmaxf(cmax, fabsf(cf));
-      }
-    };
+    float* const ga = P.gacc + P.abase[li];
     for (int ii = tid; ii < total && mx > 0.0f; ii += ADJ_THREADS) {
       const int off = list[ii];
       const float lam = s_lam[off];
       if (lam == 0.0f) continue;
       const int x = off >> 6, y = (off >> 3) & 7, z = off & 7;
-      double xw[3], y1[3], y2[3], p[3], u[3];
-      index_to_world(g.mat, ox + x, oy + y, oz + z, xw);
-      win_vel(P.vel, s_u, wb, xw, u);
-      for (int a = 0; a < 3; ++a) y1[a] = xw[a] - P.dt * (u[a] * tf[0]);
-      if (SCHEME == 1) {
-        for (int a = 0; a < 3; ++a) p[a] = y1[a];
-      } else {
-        win_vel(P.vel, s_u, wb, y1, u);
-        for (int a = 0; a < 3; ++a) y2[a] = 0.75 * xw[a] + 0.25 * (y1[a] - P.dt * (u[a] * tf[1]));
-        win_vel(P.vel, s_u, wb, y2, u);
-        for (int a = 0; a < 3; ++a) p[a] = (1.0 / 3.0) * xw[a] + (2.0 / 3.0) * (y2[a] - P.dt * (u[a] * tf[2]));
-      }
+      double xw[3], y1[3], y2[3], p[3];
+      adj_departure<SCHEME>(P, s_u, wb, ox + x, oy + y, oz + z, tf, xw, y1, y2, p);
       const double ix = p[0] * g.inv_vs, iy = p[1] * g.inv_vs, iz = p[2] * g.inv_vs;
       const double flx = floor(ix), fly = floor(iy), flz = floor(iz);
       const double fx = ix - flx, fy = iy - fly, fz = iz - flz;
@@ -447,7 +396,129 @@ __device__ __forceinline__ void adj_step(const AdjStep& P, int li, bool valid) {
                          (c[6] - c[4]) * (1 - fx) * fz + (c[7] - c[5]) * fx * fz;
       const double dfz = (c[4] - c[0]) * (1 - fx) * (1 - fy) + (c[5] - c[1]) * fx * (1 - fy) +
                          (c[6] - c[2]) * (1 - fx) * fy + (c[7] - c[3]) * fx * fy;
-      const double gp[3] = {(double)lam * dfx * g.inv_vs, (double)lam * dfy * g.inv_vs, (double)lam * dfz * g.inv_vs};
+      // this step's g_n, summed into the voxel's gacc entry (one thread per
+      // voxel and step, so the sum is in step order)
+      ga[ii] += (float)((double)lam * dfx * g.inv_vs);
+      ga[P.gstride + ii] += (float)((double)lam * dfy * g.inv_vs);
+      ga[2 * P.gstride + ii] += (float)((double)lam * dfz * g.inv_vs);
+    }
+    __syncthreads();
+    float* ob = P.ob_out + (size_t)li * 1000;
+    for (int i = tid; i < 1000; i += ADJ_THREADS) ob[i] = (float)((double)acc[i] * inv_scale);
+    __syncthreads();
+  }
+}
+
+// The velocity gradient of the whole rollout, one leaf per block: each
+// active voxel's summed g (gacc) pulled back through its departure point's
+// derivative dp/dU.
+//
+// Scatter at the voxel's own position (the RK1 term and the last RK3 term):
+// for a diagonal transform the node weights factor per axis, so the
+// per-voxel coefficients are staged in LDS and contracted axis by axis into
+// the leaf's node window (no atomics); otherwise float atomics into the
+// window.  The two RK3 stage positions are kept as records and added in a
+// second pass as int32-quantised 64-bit sums scaled by the leaf's max
+// |coefficient|.  The window is then added to dL/dU with global atomics.
+template <int SCHEME>
+__device__ __forceinline__ void adj_vel_step(const AdjStep& P, int li, bool valid) {
+  // RK3 keeps 2 scatter records per voxel (y2, y1); the scatter at x goes
+  // through the separable contraction, or straight into the window.
+  constexpr int NREC = SCHEME == 3 ? 2 : 0;
+  constexpr int REC_WORDS = NREC * 6 * 512;
+  constexpr int T_WORDS = 3 * 64 * VW + 3 * 8 * VW * VW;
+  __shared__ float coefx[3][512];
+  __shared__ float vwin[3 * VW3];
+  __shared__ float s_u[3 * VW3];  // the field's nodes over the window
+  __shared__ long long vwin_i[SCHEME == 3 ? 3 * VW3 : 1];
+  // records (pass 1 -> pass 2), then the separable contraction stages
+  __shared__ float s_raw[REC_WORDS > T_WORDS ? REC_WORDS : T_WORDS];
+  float* const t1 = s_raw;
+  float* const t2 = s_raw + 3 * 64 * VW;
+  // record r of a voxel: per axis the coefficient, and the cell's base node
+  // relative to the window (10 bits) with the fraction in 22-bit fixed point
+  // above it; all-zero coefficients mark no record.
+  float* const rec_c = s_raw;                                  // [NREC][3][512]
+  uint32_t* const rec_w = (uint32_t*)(s_raw + NREC * 3 * 512);  // [NREC][3][512]
+  __shared__ float s_cmax[ADJ_WAVES];
+  __shared__ uint16_t list[512];
+  // per axis and local coordinate: base node relative to the window, and the
+  // weight of each window node
+  __shared__ int s_bn[3][8];
+  __shared__ float s_wt[3][8][VW];
+  __shared__ int s_sep;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const GridView& g = P.phi;
+  const double tf[3] = {P.tf0, P.tf1, P.tf2};
+  if (valid) {
+    const NanoLeaf& L = g.leaves[li];
+    const int total = adj_active_list(L, list, lane, wave);
+    for (int i = tid; i < 3 * 512; i += ADJ_THREADS) (&coefx[0][0])[i] = 0.0f;
+    for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin[i] = 0.0f;
+    if constexpr (SCHEME == 3) {
+      for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) vwin_i[i] = 0;
+      for (int i = tid; i < NREC * 3 * 512; i += ADJ_THREADS) rec_c[i] = 0.0f;
+    }
+    const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
+    int wb[3];
+    adj_window(g, P.vel, ox, oy, oz, wb, s_u, tid);
+    if (tid < 24) {
+      // Node base and fraction per axis of the voxel positions (diagonal
+      // transform: the axis coordinate alone fixes them).
+      const int a = tid >> 3, l = tid & 7;
+      const int o = a == 0 ? ox : (a == 1 ? oy : oz);
+      const double w = g.mat[5 * a] * (double)(o + l) + g.mat[4 * a + 3];
+      const double idx = w * P.vel.inv_h;
+      const double fl = floor(idx);
+      const int bn = f64_to_i32(fl) - wb[a];
+      s_bn[a][l] = bn;
+      for (int k = 0; k < VW; ++k) s_wt[a][l][k] = (float)axis_weight(k, bn, idx - fl);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      bool ok = P.separable != 0;
+      for (int a = 0; a < 3; ++a) ok = ok && s_bn[a][0] >= 0 && s_bn[a][7] + 1 < VW;
+      s_sep = ok ? 1 : 0;
+    }
+    __syncthreads();
+    const bool sep = s_sep != 0;
+    float cmax = 0.0f;  // max |coefficient| over this thread's records
+    // Stores a scatter record (RK3), or adds far points straight to dL/dU.
+    auto rec_put = [&](int r, int off, const double pt[3], const double c[3]) {
+      int b[3];
+      double f[3];
+      dense_weights(P.vel, pt, b, f);
+      const int r0 = b[0] - wb[0], r1 = b[1] - wb[1], r2 = b[2] - wb[2];
+      if (r0 < -512 || r0 > 511 || r1 < -512 || r1 > 511 || r2 < -512 || r2 > 511) {
+        const int nn = P.vel.n;
+        const size_t n3 = (size_t)nn * nn * nn;
+        for (int q = 0; q < 8; ++q) {
+          const int dx = q & 1, dy = (q >> 1) & 1, dz = (q >> 2) & 1;
+          const int i = b[0] + dx, j = b[1] + dy, k = b[2] + dz;
+          if ((unsigned)i >= (unsigned)nn || (unsigned)j >= (unsigned)nn || (unsigned)k >= (unsigned)nn) continue;
+          const double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
+          for (int a = 0; a < 3; ++a) atomicAdd(&P.grad[a * n3 + ((size_t)i * nn + j) * nn + k], (float)(c[a] * w));
+        }
+        return;
+      }
+      const int rr[3] = {r0, r1, r2};
+      for (int a = 0; a < 3; ++a) {
+        // f in [0, 1): 22-bit fixed point, truncated
+        const uint32_t fq = (uint32_t)(f[a] * 4194304.0);
+        rec_w[(r * 3 + a) * 512 + off] = (fq << 10) | (uint32_t)(rr[a] & 1023);
+        const float cf = (float)c[a];
+        rec_c[(r * 3 + a) * 512 + off] = cf;
+        cmax = fmaxf(cmax, fabsf(cf));
+      }
+    };
+    const float* const ga = P.gacc + P.abase[li];
+    for (int ii = tid; ii < total; ii += ADJ_THREADS) {
+      const int off = list[ii];
+      const double gp[3] = {(double)ga[ii], (double)ga[P.gstride + ii], (double)ga[2 * P.gstride + ii]};
+      if (gp[0] == 0.0 && gp[1] == 0.0 && gp[2] == 0.0) continue;
+      const int x = off >> 6, y = (off >> 3) & 7, z = off & 7;
+      double xw[3], y1[3], y2[3], p[3];
+      adj_departure<SCHEME>(P, s_u, wb, ox + x, oy + y, oz + z, tf, xw, y1, y2, p);
       double cself[3];  // coefficient of the scatter at the voxel's own position
       if (SCHEME == 1) {
         for (int a = 0; a < 3; ++a) cself[a] = -P.dt * tf[0] * gp[a];
@@ -578,8 +649,6 @@ __device__ __forceinline__ void adj_step(const AdjStep& P, int li, bool valid) {
       }
       __syncthreads();
     }
-    float* ob = P.ob_out + (size_t)li * 1000;
-    for (int i = tid; i < 1000; i += ADJ_THREADS) ob[i] = (float)((double)acc[i] * inv_scale);
     const int nv = P.vel.n;
     const size_t n3 = (size_t)nv * nv * nv;
     for (int i = tid; i < 3 * VW3; i += ADJ_THREADS) {
@@ -595,18 +664,23 @@ __device__ __forceinline__ void adj_step(const AdjStep& P, int li, bool valid) {
   }
 }
 
-// RK1: held to 128 VGPRs (4 waves per SIMD; a few bytes of spill) -- faster
-// than the unconstrained allocation at 3 waves.  RK3 keeps its registers.
 // One leaf per block: the grid has at least ceil(n / 8) blocks per XCD.
-__global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adj_step1(AdjStep P) {
-  int li_end, li_step;
-  const int li = xcd_first(*P.phi.count, &li_end, &li_step);
-  adj_step<1>(P, li, li < li_end);
+// PASS 0: the state adjoint step (adj_lambda_step), PASS 1: the velocity
+// gradient (adj_vel_step).  The RK3 velocity pass keeps its registers (2
+// waves per SIMD); the others are held to 128 VGPRs (4 waves).
+template <int SCHEME, int PASS>
+__device__ __forceinline__ void adj_run(const AdjStep& P, int li, bool valid) {
+  if constexpr (PASS == 0)
+    adj_lambda_step<SCHEME>(P, li, valid);
+  else
+    adj_vel_step<SCHEME>(P, li, valid);
 }
-__global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adj_step3(AdjStep P) {
+#define ADJ_WPE(SCHEME, PASS) __attribute__((amdgpu_waves_per_eu((SCHEME == 3 && PASS == 1) ? 2 : 4, 8)))
+template <int SCHEME, int PASS>
+__global__ __launch_bounds__(ADJ_THREADS) ADJ_WPE(SCHEME, PASS) void k_adj(AdjStep P) {
   int li_end, li_step;
   const int li = xcd_first(*P.phi.count, &li_end, &li_step);
-  adj_step<3>(P, li, li < li_end);
+  adj_run<SCHEME, PASS>(P, li, li < li_end);
 }
 
 // Grid g of global leaf i: leaf_base[g] <= i < leaf_base[g + 1] (G + 1 entries).
@@ -622,21 +696,24 @@ __device__ __forceinline__ int adj_grid_of(const int* leaf_base, int G, int i) {
 // Several level sets in one launch: the leaves of all grids form one list
 // (grid g owns [leaf_base[g], leaf_base[g + 1])), one leaf per block, each
 // with its grid's step (its state, field, bricks and gradient).
-template <int SCHEME>
-__device__ __forceinline__ void adj_step_multi(const AdjStep* Ps, const int* leaf_base, int G, int L) {
+template <int SCHEME, int PASS>
+__global__ __launch_bounds__(ADJ_THREADS) ADJ_WPE(SCHEME, PASS) void k_adj_multi(const AdjStep* Ps,
+                                                                                 const int* leaf_base, int G, int L) {
   int end, stride;
   const int gl = xcd_first(L, &end, &stride);
   const bool valid = gl < end;
   const int g = __builtin_amdgcn_readfirstlane(valid ? adj_grid_of(leaf_base, G, gl) : 0);
-  adj_step<SCHEME>(Ps[g], gl - leaf_base[g], valid);
+  adj_run<SCHEME, PASS>(Ps[g], gl - leaf_base[g], valid);
 }
-__global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adj_step1_multi(
-    const AdjStep* Ps, const int* leaf_base, int G, int L) {
-  adj_step_multi<1>(Ps, leaf_base, G, L);
-}
-__global__ __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_adj_step3_multi(
-    const AdjStep* Ps, const int* leaf_base, int G, int L) {
-  adj_step_multi<3>(Ps, leaf_base, G, L);
+
+// Per-leaf active voxel counts (for the gacc offsets).
+__global__ void k_adj_leaf_active(const NanoLeaf* leaves, int n, int* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int c = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) c += __popcll(leaves[i].mask[w]);
+    out[i] = c;
+  }
 }
 
 // The forward step of several level sets in one launch (the dense field's
@@ -758,14 +835,23 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
   // Buffers: checkpoints (segment starts 1..nseg-1), segment states, final state.
   std::vector<NanoLeaf*> ck((size_t)nseg, nullptr), seg((size_t)c, nullptr);
   NanoLeaf* fin = nullptr;
-  float *ob[2] = {nullptr, nullptr}, *far[2] = {nullptr, nullptr}, *grad = nullptr;
+  float *ob[2] = {nullptr, nullptr}, *far[2] = {nullptr, nullptr}, *grad = nullptr, *gacc = nullptr;
   double *leaf_loss = nullptr, *loss_d = nullptr;
+  int *abase = nullptr, *acnt = nullptr, *ascan = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  int rc = VR_OK;
+  uint64_t lc0 = 0, ac0 = 0;
+  int rc = vr_grid_counts(ctx, phi0, &lc0, &ac0);
+  if (rc != VR_OK) return rc;
+  if (ac0 > 0x7fffffffULL) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "too many active voxels");
+  const size_t na = ac0 ? (size_t)ac0 : 1;
   auto alloc = [&](void** p, size_t bytes) {
     if (rc != VR_OK) return;
     if ((e = hipMalloc(p, bytes)) != hipSuccess) rc = vr_hip_fail(ctx, e, "hipMalloc");
   };
+  alloc((void**)&gacc, sizeof(float) * 3 * na);
+  alloc((void**)&abase, sizeof(int) * (cap + 1));
+  alloc((void**)&acnt, sizeof(int) * cap);
+  alloc((void**)&ascan, sizeof(int) * (vr_scan_scratch_ints(cap) + 1));
   ck[0] = phi0->d_leaves;
   for (int k = 1; k < nseg; ++k) alloc((void**)&ck[k], sizeof(NanoLeaf) * cap);
   for (int j = 1; j < c; ++j) alloc((void**)&seg[j], sizeof(NanoLeaf) * cap);
@@ -819,8 +905,13 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
     cur = dst;
   }
   if (rc == VR_OK) (void)hipEventRecord(ev[1], ctx->stream);
-  // Terminal adjoint and loss.
+  // Terminal adjoint and loss; the per-voxel sums start at 0.
   if (rc == VR_OK) {
+    hipLaunchKernelGGL(k_adj_leaf_active, dim3(grid_blocks(cap, 256)), dim3(256), 0, ctx->stream, phi0->d_leaves, L,
+                       acnt);
+    if ((e = vr_exclusive_scan(acnt, abase, (size_t)L, ascan, abase + cap, ctx->stream)) != hipSuccess)
+      rc = vr_hip_fail(ctx, e, "scan");
+    hipLaunchKernelGGL(k_zero_f32, dim3(grid_blocks(3 * na, 256)), dim3(256), 0, ctx->stream, gacc, 3 * na);
     view.d_leaves = fin;
     hipLaunchKernelGGL(k_adj_terminal, dim3(leaf_blocks(L)), dim3(256), 0, ctx->stream, make_view(&view),
                        make_view(target), ob[0], leaf_loss);
@@ -847,13 +938,23 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
       P.ob_out = ob[bi ^ 1];
       P.far_out = far[bi ^ 1];
       P.grad = grad;
+      P.abase = abase;
+      P.gacc = gacc;
+      P.gstride = (long long)na;
       P.dt = dt;
       P.tf0 = P.tf1 = P.tf2 = 1.0;
       P.separable = separable;
       if (scheme == VR_SCHEME_RK3)
-        hipLaunchKernelGGL(k_adj_step3, dim3(leaf_blocks(L)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+        hipLaunchKernelGGL((k_adj<3, 0>), dim3(leaf_blocks(L)), dim3(ADJ_THREADS), 0, ctx->stream, P);
       else
-        hipLaunchKernelGGL(k_adj_step1, dim3(leaf_blocks(L)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+        hipLaunchKernelGGL((k_adj<1, 0>), dim3(leaf_blocks(L)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+      // after the last backward step: the velocity gradient of all steps
+      if (s == 0) {
+        if (scheme == VR_SCHEME_RK3)
+          hipLaunchKernelGGL((k_adj<3, 1>), dim3(leaf_blocks(L)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+        else
+          hipLaunchKernelGGL((k_adj<1, 1>), dim3(leaf_blocks(L)), dim3(ADJ_THREADS), 0, ctx->stream, P);
+      }
       if ((e = hipGetLastError()) != hipSuccess) rc = vr_hip_fail(ctx, e, "adjoint step");
       bi ^= 1;
     }
@@ -877,17 +978,13 @@ extern "C" int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_vel
         (e = hipMemcpy(grad_out, grad, sizeof(float) * 3 * nv3, hipMemcpyDeviceToHost)) != hipSuccess)
       rc = vr_hip_fail(ctx, e, "grad readback");
   }
-  if (rc == VR_OK) {
-    uint64_t lc = 0, ac = 0;
-    rc = vr_grid_counts(ctx, phi0, &lc, &ac);
-    st.active_voxels = ac;
-  }
+  st.active_voxels = ac0;
   for (int k = 1; k < nseg; ++k)
     if (ck[k]) (void)hipFree(ck[k]);
   for (int j = 1; j < c; ++j)
     if (seg[j]) (void)hipFree(seg[j]);
   for (void* p : {(void*)fin, (void*)ob[0], (void*)ob[1], (void*)far[0], (void*)far[1], (void*)grad, (void*)leaf_loss,
-                  (void*)loss_d})
+                  (void*)loss_d, (void*)gacc, (void*)abase, (void*)acnt, (void*)ascan})
     if (p) (void)hipFree(p);
   for (int i = 0; i < 3; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
@@ -931,8 +1028,9 @@ extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* c
   const int nseg = (steps + c - 1) / c;
   std::vector<NanoLeaf*> ck((size_t)nseg, nullptr), seg((size_t)c, nullptr);
   NanoLeaf* fin = nullptr;
-  float *ob[2] = {nullptr, nullptr}, *far[2] = {nullptr, nullptr}, *grad = nullptr;
+  float *ob[2] = {nullptr, nullptr}, *far[2] = {nullptr, nullptr}, *grad = nullptr, *gacc = nullptr;
   double *leaf_loss = nullptr, *loss_d = nullptr;
+  int *abase = nullptr, *acnt = nullptr, *ascan = nullptr;
   int* d_lb = nullptr;
   VelView* d_vel = nullptr;
   double* d_dts = nullptr;
@@ -945,6 +1043,17 @@ extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* c
     if (rc != VR_OK) return;
     if ((e = hipMalloc(p, bytes)) != hipSuccess) rc = vr_hip_fail(ctx, e, "hipMalloc");
   };
+  // active voxels of all grids: the stride of the per-voxel sums
+  std::vector<uint64_t> act((size_t)G, 0);
+  uint64_t atot = 0;
+  for (int g = 0; g < G && rc == VR_OK; ++g) {
+    uint64_t lc = 0;
+    rc = vr_grid_counts(ctx, phi0[g], &lc, &act[g]);
+    atot += act[g];
+  }
+  if (rc != VR_OK) return rc;
+  if (atot > 0x7fffffffULL) return vr_fail(ctx, VR_ERR_UNSUPPORTED, "too many active voxels");
+  const size_t na = atot ? (size_t)atot : 1;
   for (int k = 0; k < nseg; ++k) alloc((void**)&ck[k], sizeof(NanoLeaf) * cap);
   for (int j = 1; j < c; ++j) alloc((void**)&seg[j], sizeof(NanoLeaf) * cap);
   alloc((void**)&fin, sizeof(NanoLeaf) * cap);
@@ -953,6 +1062,10 @@ extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* c
     alloc((void**)&far[i], sizeof(float) * 512 * cap);
   }
   alloc((void**)&grad, sizeof(float) * (goff[G] ? goff[G] : 1));
+  alloc((void**)&gacc, sizeof(float) * 3 * na);
+  alloc((void**)&abase, sizeof(int) * (cap + 1));
+  alloc((void**)&acnt, sizeof(int) * cap);
+  alloc((void**)&ascan, sizeof(int) * (vr_scan_scratch_ints(cap) + 1));
   alloc((void**)&leaf_loss, sizeof(double) * cap);
   alloc((void**)&loss_d, sizeof(double) * G);
   alloc((void**)&d_lb, sizeof(int) * (G + 1));
@@ -1030,6 +1143,9 @@ extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* c
         P.ob_out = ob[bi ^ 1] + lb * 1000;
         P.far_out = far[bi ^ 1] + lb * 512;
         P.grad = grad + goff[g];
+        P.abase = abase + lb;
+        P.gacc = gacc;
+        P.gstride = (long long)na;
         P.dt = dts[g];
         P.tf0 = P.tf1 = P.tf2 = 1.0;
         P.separable = separable[g];
@@ -1089,6 +1205,11 @@ extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* c
   for (size_t o = 0; o < n_forward_ops && rc == VR_OK; ++o) launch_forward(ops[o].index);
   if (rc == VR_OK) {
     (void)hipEventRecord(ev[1], ctx->stream);
+    // per-voxel sums: offsets over the leaves of all grids, zeroed
+    hipLaunchKernelGGL(k_adj_leaf_active, dim3(grid_blocks(cap, 256)), dim3(256), 0, ctx->stream, ck[0], L, acnt);
+    if ((e = vr_exclusive_scan(acnt, abase, (size_t)L, ascan, abase + cap, ctx->stream)) != hipSuccess)
+      rc = vr_hip_fail(ctx, e, "scan");
+    hipLaunchKernelGGL(k_zero_f32, dim3(grid_blocks(3 * na, 256)), dim3(256), 0, ctx->stream, gacc, 3 * na);
     hipLaunchKernelGGL(k_adj_terminal_multi, dim3(leaf_blocks((size_t)L)), dim3(256), 0, ctx->stream, d_fin, d_tgt,
                        ob[0], leaf_loss, d_lb, G, L);
     for (int g = 0; g < G; ++g)
@@ -1101,11 +1222,20 @@ extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* c
     } else {
       const AdjStep* P = d_bwd + (size_t)ops[o].index * G;
       if (scheme == VR_SCHEME_RK3)
-        hipLaunchKernelGGL(k_adj_step3_multi, dim3(leaf_blocks((size_t)L)), dim3(ADJ_THREADS), 0, ctx->stream, P, d_lb,
-                           G, L);
+        hipLaunchKernelGGL((k_adj_multi<3, 0>), dim3(leaf_blocks((size_t)L)), dim3(ADJ_THREADS), 0, ctx->stream, P,
+                           d_lb, G, L);
       else
-        hipLaunchKernelGGL(k_adj_step1_multi, dim3(leaf_blocks((size_t)L)), dim3(ADJ_THREADS), 0, ctx->stream, P, d_lb,
-                           G, L);
+        hipLaunchKernelGGL((k_adj_multi<1, 0>), dim3(leaf_blocks((size_t)L)), dim3(ADJ_THREADS), 0, ctx->stream, P,
+                           d_lb, G, L);
+      // after the last backward step (state 0): the velocity gradient of all steps
+      if (o + 1 == ops.size()) {
+        if (scheme == VR_SCHEME_RK3)
+          hipLaunchKernelGGL((k_adj_multi<3, 1>), dim3(leaf_blocks((size_t)L)), dim3(ADJ_THREADS), 0, ctx->stream, P,
+                             d_lb, G, L);
+        else
+          hipLaunchKernelGGL((k_adj_multi<1, 1>), dim3(leaf_blocks((size_t)L)), dim3(ADJ_THREADS), 0, ctx->stream, P,
+                             d_lb, G, L);
+      }
     }
     if ((e = hipGetLastError()) != hipSuccess) rc = vr_hip_fail(ctx, e, "adjoint launch");
   }
@@ -1125,8 +1255,7 @@ extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* c
             hipSuccess)
       rc = vr_hip_fail(ctx, e, "grad readback");
     if (rc == VR_OK && stats) {
-      uint64_t lc = 0, ac = 0;
-      rc = vr_grid_counts(ctx, phi0[g], &lc, &ac);
+      const uint64_t ac = act[g];
       vr_adjoint_stats& st = stats[g];
       std::memset(&st, 0, sizeof(st));
       st.loss = loss[g];
@@ -1142,7 +1271,7 @@ extern "C" int vr_adjoint_rollout_batch(vr_ctx* ctx, int count, const vr_grid* c
     if (seg[j]) (void)hipFree(seg[j]);
   for (void* p : {(void*)fin, (void*)ob[0], (void*)ob[1], (void*)far[0], (void*)far[1], (void*)grad, (void*)leaf_loss,
                   (void*)loss_d, (void*)d_lb, (void*)d_vel, (void*)d_dts, (void*)d_fin, (void*)d_tgt, (void*)d_fwd_in,
-                  (void*)d_fwd_out, (void*)d_bwd})
+                  (void*)d_fwd_out, (void*)d_bwd, (void*)gacc, (void*)abase, (void*)acnt, (void*)ascan})
     if (p) (void)hipFree(p);
   for (int i = 0; i < 3; ++i)
     if (ev[i]) (void)hipEventDestroy(ev[i]);
