@@ -11,11 +11,16 @@
 //     (leaf + one-voxel halo) that the previous backward step left for it
 //     in the (up to 8) leaves whose brick covers it, in a fixed order;
 //   - it is scattered with the trilinear weights into this leaf's brick as
-//     exact 64-bit fixed-point sums, so lambda_n is deterministic;
-//   - lambda * dS/dp is pulled back through the departure point (and, for
-//     RK3, through the stage positions with the field's Jacobian) onto the
-//     velocity nodes, reduced in an LDS window (separably for the voxel's
-//     own position) and added to dL/dU with global atomics.
+//     exact 32-bit fixed-point sums, so lambda_n is deterministic;
+//   - g_n = lambda_{n+1} dS/dp is added to the voxel's running sum over the
+//     steps.
+// The departure point depends on the voxel, the static field and dt only,
+// so dL/dU = sum over voxels of (dp/dU)^T (sum over steps of g_n): after the
+// last backward step one pass pulls each voxel's sum back through the
+// departure point (and, for RK3, through the stage positions with the
+// field's Jacobian) onto the velocity nodes, reduced in an LDS window
+// (separably for the voxel's own position) and added to dL/dU with global
+// atomics.
 // Forward states are kept every `checkpoint_every` steps; the others are
 // recomputed segment by segment during the reverse pass.
 #include <cstring>
