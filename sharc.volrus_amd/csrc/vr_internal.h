@@ -200,8 +200,8 @@ int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int*
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso,
                    bool masks = false);
 // Timed (class TC_MC_EMIT) with algorithmic bytes per leaf (2128, the leaf,
-// or 64, its sign mask, with masks) * leaf_count + mesh_bytes (24 per vertex
-// + 12 per triangle written).
+// or with masks its 2048 value bytes and 64-byte sign mask) * leaf_count +
+// mesh_bytes (24 per vertex + 12 per triangle written).
 int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int tcap, float iso, double* verts,
                   uint32_t* tris, double leaf_count, double mesh_bytes, long long vcap = -1, long long tricap = -1,
                   int* counts_out = nullptr, bool masks = false);

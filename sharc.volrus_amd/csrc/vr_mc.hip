@@ -1223,7 +1223,8 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int t
   if (vcap < 0) vcap = 0x7fffffffLL;
   if (tricap < 0) tricap = 0x7fffffffLL;
   {
-    TimedLaunch tl(ctx, TC_MC_EMIT, masks ? 64.0 : (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
+    // with masks: each leaf's sign mask and its 2048 value bytes (vertices interpolate values throughout the leaf)
+    TimedLaunch tl(ctx, TC_MC_EMIT, masks ? 2048.0 + 64.0 : (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
     const void* k = masks ? (const void*)k_mc_emit<true> : (const void*)k_mc_emit<false>;
     hipLaunchKernelGGL(masks ? k_mc_emit<true> : k_mc_emit<false>, dim3(mc_blocks(ctx, k, MC_EWAVES, tcap)),
                        dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
