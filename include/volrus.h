@@ -223,8 +223,11 @@ int vr_rollout(vr_ctx* ctx, vr_grid* g, const vr_velocity* vel, double t0, doubl
  * (phi_steps - target.get(x))^2 and its gradient with respect to the dense
  * field, written to grad (3 n^3 floats, the field's layout; may be NULL).
  * Forward states are kept every `checkpoint_every` steps and the others are
- * recomputed in the reverse pass.  The gradient is summed with float
- * atomics, so its last bits can vary from run to run. */
+ * recomputed in the reverse pass.  Each reverse step sums, per active voxel,
+ * lambda times the gradient of its departure-point sample (12 bytes per
+ * active voxel of device memory); one launch after the last reverse step
+ * pulls the sums back onto the field's nodes.  The gradient is summed with
+ * float atomics, so its last bits can vary from run to run. */
 typedef struct vr_adjoint_stats {
   double loss;
   double forward_ms; /* device time of the forward steps */
@@ -235,8 +238,9 @@ typedef struct vr_adjoint_stats {
 int vr_adjoint_rollout(vr_ctx* ctx, const vr_grid* phi0, const vr_velocity* vel, double dt, int scheme, int steps,
                        int checkpoint_every, const vr_grid* target, vr_adjoint_stats* stats, float* grad);
 /* vr_adjoint_rollout for `count` level sets at once (each with its own dense
- * field, time step and target): every forward step and every backward step
- * is one launch over the leaves of all grids.  stats[g] (count entries, may
+ * field, time step and target): every forward step, every backward step and
+ * the final pull-back onto the fields is one launch over the leaves of all
+ * grids.  stats[g] (count entries, may
  * be NULL) holds grid g's loss and the whole batch's forward / adjoint
  * times; grads[g] (may be NULL, or hold NULL entries) receives grid g's
  * gradient. */
