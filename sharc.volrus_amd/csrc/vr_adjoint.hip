@@ -728,7 +728,7 @@ template <int SCHEME, int NW>
 __global__ __launch_bounds__(64 * NW) void k_adj_advect_multi(const GridView* ins, NanoLeaf* const* outs,
                                                              const VelView* vels, const double* dts,
                                                              const int* leaf_base, int G, int L) {
-  __shared__ AdvectLDS s;
+  __shared__ AdvectLDS<VK_DENSE> s;
   int end, stride;
   const int gl = xcd_first(L, &end, &stride);
   if (gl >= end) return;

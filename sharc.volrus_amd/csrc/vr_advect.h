@@ -118,15 +118,46 @@ __device__ __forceinline__ void stage_halo(const GridView& in, const int* nb, fl
     if (tid + q * NT < 488) brick[e[q] & 1023u] = src[q] < 0 ? bg : v[q];
 }
 
-// LDS of one advecting block.
-struct AdvectLDS {
+// The halo as stage_halo, with the neighbour list in a register: lane d of
+// every wave holds entry d (d < 27), read with a lane permute, so the list
+// needs no LDS and no barrier before the halo loads.
+template <int NT>
+__device__ __forceinline__ void stage_halo_reg(const GridView& in, int nbv, float* brick, int tid) {
+  constexpr int HQ = (488 + NT - 1) / NT;
+  const float bg = in.background;
+  uint32_t e[HQ];
+#pragma unroll
+  for (int q = 0; q < HQ; ++q) {
+    const int h = tid + q * NT;
+    e[q] = c_advect_halo.e[h < 488 ? h : 0];
+  }
+  int src[HQ];
+  float v[HQ];
+#pragma unroll
+  for (int q = 0; q < HQ; ++q) {
+    src[q] = __shfl(nbv, (int)((e[q] >> 10) & 31u));
+    v[q] = in.leaves[src[q] < 0 ? 0 : src[q]].values[e[q] >> 15];
+  }
+#pragma unroll
+  for (int q = 0; q < HQ; ++q)
+    if (tid + q * NT < 488) brick[e[q] & 1023u] = src[q] < 0 ? bg : v[q];
+}
+
+// LDS of one advecting block (5088 bytes without the vortex tables, so 32
+// one-wave blocks fit a CU's 160 KB).  TAB: room for the single-vortex row
+// tables (sin x, cos x, sin y, cos y, 8 each).
+template <bool TAB>
+struct AdvectLDST {
   float brick[1000];
   uint16_t list[512];
-  int nb[VR_NBR];
-  uint64_t act[8];  // the leaf's active mask
   uint32_t sgn[16];  // sign mask of the results (SIGNS)
-  double vtab[32];  // single-vortex row tables: sin x, cos x, sin y, cos y (8 each)
+  double vtab[TAB ? 32 : 1];
 };
+// Whether kernels for velocity kind vk (negative: kinds mixed at run time)
+// may run the single-vortex path.
+constexpr bool advect_tab(int vk) { return vk == VK_VORTEX || vk < 0; }
+template <int VK>
+using AdvectLDS = AdvectLDST<advect_tab(VK)>;
 
 // Advects leaf li of `in` into out[li] with NW waves (the whole block,
 // tid in [0, 64 NW)): the active voxels are listed by a ballot/popcount
@@ -141,9 +172,9 @@ struct AdvectLDS {
 // background at the inactive offsets), gathered with LDS atomics as the
 // results come, so marching cubes over the output can classify cells
 // without reading values.
-template <int VK, int SCHEME, int NW, bool SIGNS = false>
+template <int VK, int SCHEME, int NW, bool SIGNS = false, class LDS>
 __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
-                                            AdvectLDS& s, int tid, double dt, const double tf[3],
+                                            LDS& s, int tid, double dt, const double tf[3],
                                             uint32_t* sg = nullptr, float iso = 0.0f) {
   constexpr int NT = 64 * NW;
   const int lane = tid & 63, wave = tid >> 6;
@@ -153,7 +184,8 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   // The independent loads go out first (every lane loads, so no branch
   // waits on them): the neighbour list entry and the leaf's own values.
   constexpr int OQ = (128 + NT - 1) / NT;
-  const int nbv = in.nbr[(size_t)li * VR_NBR + (tid < VR_NBR ? tid : 0)];
+  // every wave holds the neighbour list in lanes 0..26 (stage_halo_reg)
+  const int nbv = in.nbr[(size_t)li * VR_NBR + (lane < VR_NBR ? lane : 0)];
   const uint64_t mword = L.mask[tid & 7];  // copied to the output by threads 0..7
   const uint32_t sword = SIGNS ? reinterpret_cast<const uint32_t*>(L.mask)[tid & 15] : 0u;
   float4 own[OQ];
@@ -185,7 +217,6 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
     s.vtab[16 * a + l] = sn;
     s.vtab[16 * a + 8 + l] = cs;
   }
-  if (tid < VR_NBR) s.nb[tid] = nbv;
 #pragma unroll
   for (int q = 0; q < OQ; ++q) {
     const int e = tid + q * NT;
@@ -198,16 +229,12 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
       b[3] = own[q].w;
     }
   }
-  __syncthreads();
   // Output header: origin and mask are the input's.
-  if (tid < 8) {
-    O.mask[tid] = mword;
-    s.act[tid] = mword;
-  }
+  if (tid < 8) O.mask[tid] = mword;
   if (SIGNS && tid < 16) s.sgn[tid] = bg < iso ? ~sword : 0u;
   if (tid < 3) O.origin[tid] = tid == 0 ? ox : (tid == 1 ? oy : oz);
   if (tid == 3) O.pad = 0;
-  stage_halo<NT>(in, s.nb, s.brick, tid);
+  stage_halo_reg<NT>(in, nbv, s.brick, tid);
   __syncthreads();
   for (int i = tid; i < total; i += NT) {
     const int off = s.list[i];
@@ -218,7 +245,7 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   }
 #pragma unroll
   for (int x = 0; x < 8; ++x)
-    if ((x % NW) == wave && !((s.act[x] >> lane) & 1ull)) O.values[x * 64 + lane] = bg;
+    if ((x % NW) == wave && !((mk[x] >> lane) & 1ull)) O.values[x * 64 + lane] = bg;
   __syncthreads();
   if (SIGNS && tid < 16) sg[tid] = s.sgn[tid];
 }

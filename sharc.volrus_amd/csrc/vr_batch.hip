@@ -122,7 +122,7 @@ __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, c
                                                           const int* ranges, int persistent, const double* dts,
                                                           double dt, double tf0, double tf1, double tf2,
                                                           uint32_t* signs, const int* leaf_base, float iso) {
-  __shared__ AdvectLDS s;
+  __shared__ AdvectLDS<VK> s;
   __shared__ int s_item;
   const int tid = threadIdx.x;
   const int xcd = blockIdx.x & 7;

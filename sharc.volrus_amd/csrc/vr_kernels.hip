@@ -178,7 +178,7 @@ __global__ void k_scan_add(int* out, const int* sums, size_t n) {
 template <int VK, int SCHEME, int NW>
 __global__ __launch_bounds__(64 * NW) void k_advect(GridView in, VelView vel, NanoLeaf* out, double dt, double tf0,
                                                     double tf1, double tf2) {
-  __shared__ AdvectLDS s;
+  __shared__ AdvectLDS<VK> s;
   const int n = *in.count;
   const double tf[3] = {tf0, tf1, tf2};
   int li_end, li_step;
