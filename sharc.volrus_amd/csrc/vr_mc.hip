@@ -28,8 +28,8 @@
 // ------------------------------------------------------------------ tables
 // Per case: crossing-edge mask | triangle count << 12.
 __constant__ uint16_t c_mc_emn[256];
-// Per case: TRI_TABLE row, 16 int8 (-1 padded), as 4 words.
-__constant__ uint32_t c_mc_tri4[256 * 4];
+// Per case: TRI_TABLE row, one nibble per entry (entry k at bits 4k; 15 pads).
+__constant__ uint64_t c_mc_trin[256];
 // For local edge e and candidate j (0..3): offset (dx, dy, dz) of a cell that
 // contains the same lattice edge and that edge's local number there, packed
 // as 4 int8 (dx, dy, dz, e').
@@ -64,15 +64,7 @@ __constant__ int c_dir7[7][3] = {{0, 0, 0}, {-8, 0, 0}, {8, 0, 0}, {0, -8, 0}, {
 hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, const int8_t* tri,
                                const int8_t* corner, const int8_t* edge, const int8_t* cand) {
   uint16_t emn[256];
-  uint32_t tri4[256 * 4];
-  for (int c = 0; c < 256; ++c) {
-    emn[c] = (uint16_t)(edge_mask[c] | (ntri[c] << 12));
-    for (int w = 0; w < 4; ++w) {
-      uint32_t v = 0;
-      for (int k = 0; k < 4; ++k) v |= (uint32_t)(uint8_t)tri[c * 16 + 4 * w + k] << (8 * k);
-      tri4[c * 4 + w] = v;
-    }
-  }
+  for (int c = 0; c < 256; ++c) emn[c] = (uint16_t)(edge_mask[c] | (ntri[c] << 12));
   uint32_t cand4[48], einfo[12], ownsort[27 * 12], before[27 * 12], fbefore[27 * 12], eslot[12];
   int self[12];
   for (int e = 0; e < 12; ++e) {
@@ -138,7 +130,12 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
     }
   hipError_t e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_emn), emn, sizeof(emn))) != hipSuccess) return e;
-  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_tri4), tri4, sizeof(tri4))) != hipSuccess) return e;
+  uint64_t trin[256];
+  for (int c = 0; c < 256; ++c) {
+    trin[c] = 0;
+    for (int k = 0; k < 16; ++k) trin[c] |= (uint64_t)(tri[c * 16 + k] < 0 ? 15 : tri[c * 16 + k]) << (4 * k);
+  }
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_trin), trin, sizeof(trin))) != hipSuccess) return e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_cand4), cand4, sizeof(cand4))) != hipSuccess) return e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_edgeinfo), einfo, sizeof(einfo))) != hipSuccess) return e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_mc_self), self, sizeof(self))) != hipSuccess) return e;
@@ -594,16 +591,16 @@ __device__ __forceinline__ void mc_mask_rows(const MCMaskQ& q, uint32_t bgs, int
 }
 
 // Loads the tile's neighbour lists (fetched into lanes < 27), stages its
-// sign rows (MASKS: from the mask words q, else from the values), lists
-// its mixed cells (corners on both sides of iso: a case
-// other than 0 or 255) in visiting order and builds the per-class self
-// masks (no candidate ordered before the cell lies in a processed tile;
-// s_before is c_mc_before in LDS).  Returns the mixed-cell count.  Whole
-// wave, uniform control flow.
-template <bool MASKS>
-__device__ __forceinline__ int mc_tile_setup(const VR_GLOBAL NanoLeaf* leaves, const MCMaskQ& q, float bg, float iso,
-                                             int tl, int tt, uint32_t processed, const uint32_t* s_before,
-                                             MCTileLDS& w) {
+// sign rows (MASKS: from the mask words q, else from the values) and builds
+// the per-class self masks (no candidate ordered before the cell lies in a
+// processed tile; s_before is c_mc_before in LDS).  Returns the lane's mixed
+// cells (corners on both sides of iso: a case other than 0 or 255): bit z of
+// lane x * 8 + y is cell (x, y, z).  W: per-wave LDS with rows, tleaf,
+// ttile and self.  Whole wave, uniform control flow.
+template <bool MASKS, class W>
+__device__ __forceinline__ uint32_t mc_tile_rows(const VR_GLOBAL NanoLeaf* leaves, const MCMaskQ& q, float bg,
+                                                 float iso, int tl, int tt, uint32_t processed,
+                                                 const uint32_t* s_before, W& w) {
   const int lane = lane_id();
   if (lane < VR_NBR) {
     w.tleaf[lane] = tl;
@@ -634,7 +631,17 @@ __device__ __forceinline__ int mc_tile_setup(const VR_GLOBAL NanoLeaf* leaves, c
   const uint32_t r01 = w.rows[x * 9 + y + 1], r11 = w.rows[(x + 1) * 9 + y + 1];
   const uint32_t all = r00 & r10 & r01 & r11, any = r00 | r10 | r01 | r11;
   const uint32_t one = all & (all >> 1), zero = ~any & ~(any >> 1);
-  uint32_t mixed = ~(one | zero) & 0xFFu;  // bit z: cell (x, y, z)
+  return ~(one | zero) & 0xFFu;
+}
+
+// mc_tile_rows, then the tile's mixed cells in visiting order in w.mlist.
+// Returns the mixed-cell count.
+template <bool MASKS>
+__device__ __forceinline__ int mc_tile_setup(const VR_GLOBAL NanoLeaf* leaves, const MCMaskQ& q, float bg, float iso,
+                                             int tl, int tt, uint32_t processed, const uint32_t* s_before,
+                                             MCTileLDS& w) {
+  const int lane = lane_id();
+  uint32_t mixed = mc_tile_rows<MASKS>(leaves, q, bg, iso, tl, tt, processed, s_before, w);
   const int cnt = __popc(mixed);
   const int incl = wave_incl_scan(cnt);
   int k = incl - cnt;
@@ -659,7 +666,6 @@ __device__ __forceinline__ int mc_edge_slot(int x, int y, int z, uint32_t es) {
   return (int)(es & 0x7FFu) + x * (int)((es >> 11) & 0x7Fu) + y * (int)(es >> 18) + z;
 }
 #define MC_EDGES 1944
-#define MC_FOREIGN 432  // lattice edges on a tile's three low planes: 3 x 144
 
 // Index (relative to the grid) of the vertex on local edge e of cell
 // (x, y, z) whose owner lies in an earlier tile: the first processed
@@ -857,31 +863,43 @@ __device__ __forceinline__ void mc_ranges(const int* tile_base, int G, int nt, c
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
 // Mixed cells go in rounds of 64, one per lane: a wave scan places the
 // round's vertices and triangles, and lists of the round's vertices
-// (cell, edge) and triangles (cell, row entry, case) spread the writes one
+// (cell, edge) and triangles (round cell, row entry) spread the writes one
 // item per lane.  Vertex indices go through the tile's lattice-edge table
 // in LDS (16 bits per edge): an owned vertex stores its index relative to
 // the tile's first vertex at its edge's slot; an edge owned in an earlier
-// tile (it lies on one of the tile's low planes) is resolved once, by the
-// first cell of this tile that contains it, from that tile's face meta of
-// pass 1 into a side table, and its slot holds 0x8000 | the side-table
-// entry.  Owners precede the cells that share their edges in visiting
-// order, so every slot a triangle reads was written in this or an earlier
-// round, before the round's triangles.  A round with more items than the
-// lists hold is written cell by cell.  Vertex indices are relative to the
-// tile's grid.
+// tile (it lies on one of the tile's three low planes, at most 3 x 144) is
+// resolved once, by the first cell of this tile that contains it, from that
+// tile's face meta of pass 1: into a side table of MC_FSIDE entries, its
+// slot holding 0x8000 | the entry, or, past the side table, its slot holding
+// 0xFFFF and every triangle corner on it resolving it again.  Owners
+// precede the cells that share their edges in visiting order, so every
+// slot a triangle reads was written in this or an earlier round, before the
+// round's triangles.  A round with more items than the lists hold is
+// written cell by cell.  Vertex indices are relative to the tile's grid.
+// The per-wave LDS is kept under 5.8 KB (the round's cell list is built per
+// round, the triangle list holds round-local cell indices, the triangle
+// table is nibble-packed), so three 8-wave blocks fit a CU: 6 waves per SIMD.
 #define MC_EWAVES 8
 #define MC_ETHREADS (64 * MC_EWAVES)
-#define MC_RV 192
-#define MC_RT 256
+#define MC_RV 128   // vertices listed per round
+#define MC_RT 192   // triangles listed per round
+#define MC_FL 192   // earlier-tile edges listed per resolution pass
+#define MC_FSIDE 96 // side-table entries per tile
 struct MCEmitLDS {
+  uint32_t rows[81];             // sign rows: bit z of rows[x * 9 + y] = value at (x, y, z) < iso
+  int tleaf[VR_NBR];             // neighbouring leaves (-1: none)
+  int ttile[VR_NBR];             // neighbouring tiles (-1: not processed)
+  uint16_t self[28];             // per cell class: edges the cell owns itself
+  uint16_t fitem[28];            // per cell class: edges owned in an earlier tile that this cell resolves
+  uint16_t rlist[64];            // the round's mixed cells
+  uint8_t rcube[64];             // their cases
   uint16_t edge[MC_EDGES];       // lattice-edge table of the tile
-  int foreign[MC_FOREIGN];       // vertex index (relative to the grid) of edges owned in an earlier tile
+  int foreign[MC_FSIDE];         // vertex index (relative to the grid) of edges owned in an earlier tile
   union {
-    uint32_t tlist[MC_RT];       // cell | triangle of the cell << 9 | case << 12
-    uint16_t flist[MC_FOREIGN];  // cell << 4 | edge, edges owned in an earlier tile to resolve
+    uint16_t tlist[MC_RT];       // round cell | triangle of the cell << 6
+    uint16_t flist[MC_FL];       // cell << 4 | edge, earlier-tile edges to resolve
   };
   uint16_t vlist[MC_RV];         // cell << 4 | edge
-  uint16_t fitem[28];            // per cell class: edges owned in an earlier tile that this cell resolves
 };
 
 // Tile t's lists and header in one vector load per array: lanes < 27 get
@@ -901,33 +919,55 @@ __device__ __forceinline__ void mc_emit_lists(const int* tleaf, const int* ttile
   if (lane < VR_NBR) *m = ttile[(size_t)t * VR_NBR + lane];
 }
 
+// The round's cells (visiting positions base .. base + 63) into rlist, from
+// each lane's mixed cells and their exclusive prefix.
+__device__ __forceinline__ void mc_round_list(uint32_t mixed, int mexcl, int base, int lane, uint16_t* rlist) {
+  int k = mexcl;
+  for (uint32_t m = mixed; m; m &= m - 1u, ++k)
+    if (k >= base && k < base + 64) rlist[k - base] = (uint16_t)(lane * 8 + (__ffs(m) - 1));
+}
+
+// What resolving an earlier tile's edge again needs (mc_foreign_index).
+struct MCForeign {
+  uint32_t processed;
+  const uint32_t* s_ownsort;
+  const uint32_t* s_cand;
+  const int* ttile;
+  const uint32_t* meta;
+  const int* vbase;
+  int vrel;
+};
+
 // Vertex index (relative to the grid) on local edge e of cell (x, y, z).
 __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const MCEmitLDS& el,
-                                               const uint32_t* s_eslot, int vbrel) {
+                                               const uint32_t* s_eslot, int vbrel, const MCForeign& fc) {
   const uint32_t v = el.edge[mc_edge_slot(x, y, z, s_eslot[e])];
-  const int f = el.foreign[(v & 0x1FFu) < MC_FOREIGN ? (v & 0x1FFu) : 0];
+  if (v == 0xFFFFu)
+    return mc_foreign_index(x, y, z, e, fc.processed, fc.s_ownsort, fc.s_cand, fc.ttile, fc.meta, fc.vbase, fc.vrel);
+  const int f = el.foreign[(v & 0xFFu) < MC_FSIDE ? (v & 0xFFu) : 0];
   return (v & 0x8000u) ? f : vbrel + (int)v;
 }
 
 template <bool MASKS>
-__global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mc_emit(
+__global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_mc_emit(
     const GridView* views, const int4* tiles, const int* ntiles, const int* tleaf, const int* ttile,
     const int* tile_base, float iso, const uint32_t* meta, const int* tv, const int* tt, const int* vbase,
     const int* tbase, const int* totals, double* verts, uint32_t* tris, long long vcap, long long tricap, int G,
     int* ranges, int* counts_out) {
   __shared__ uint16_t s_emn[256];
-  __shared__ __align__(16) int8_t s_tri[256 * 16];  // TRI_TABLE rows
+  __shared__ uint64_t s_trin[256];  // TRI_TABLE rows, one nibble per entry
   __shared__ uint32_t s_ownsort[27 * 12];
   __shared__ uint32_t s_before[27 * 12];
   __shared__ uint32_t s_fbefore[27 * 12];
   __shared__ uint32_t s_cand[48];
   __shared__ uint32_t s_einfo[12];
   __shared__ uint32_t s_eslot[12];
-  __shared__ MCTileLDS s_w[MC_EWAVES];
   __shared__ MCEmitLDS s_e[MC_EWAVES];
   const int tid = threadIdx.x, lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (int i = tid; i < 256; i += MC_ETHREADS) s_emn[i] = c_mc_emn[i];
-  for (int i = tid; i < 256 * 4; i += MC_ETHREADS) reinterpret_cast<uint32_t*>(s_tri)[i] = c_mc_tri4[i];
+  for (int i = tid; i < 256; i += MC_ETHREADS) {
+    s_emn[i] = c_mc_emn[i];
+    s_trin[i] = c_mc_trin[i];
+  }
   for (int i = tid; i < 27 * 12; i += MC_ETHREADS) {
     s_ownsort[i] = c_mc_ownsort[i];
     s_before[i] = c_mc_before[i];
@@ -939,7 +979,6 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
     s_eslot[tid] = c_mc_eslot[tid];
   }
   __syncthreads();
-  MCTileLDS& w = s_w[wv];
   MCEmitLDS& el = s_e[wv];
   const int nt = *ntiles;
   if (blockIdx.x == 0) mc_ranges(tile_base, G, nt, vbase, tbase, totals, ranges, counts_out);
@@ -981,7 +1020,11 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
     const int vbrel = vb - vrel;
-    const int n = mc_tile_setup<MASKS>(leaves, cq, bg, iso, tl, ttv, processed, s_before, w);
+    const uint32_t mixed = mc_tile_rows<MASKS>(leaves, cq, bg, iso, tl, ttv, processed, s_before, el);
+    const int mcnt = __popc(mixed);
+    const int mincl = wave_incl_scan(mcnt);
+    const int mexcl = mincl - mcnt;
+    const int n = __builtin_amdgcn_readlane(mincl, 63);
     if (MASKS && tn < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
     // Per class: the edges owned in an earlier tile that this cell resolves
     // (it is the first cell of this tile containing them: no candidate of
@@ -994,18 +1037,21 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
              << e;
       el.fitem[lane] = (uint16_t)m;
     }
-    wave_lds_sync();
+    const MCForeign fc{processed, s_ownsort, s_cand, el.ttile, meta, vbase, vrel};
     int vcarry = 0, tcarry = 0, fcarry = 0;
     for (int base = 0; base < n; base += 64) {
+      mc_round_list(mixed, mexcl, base, lane, el.rlist);
+      wave_lds_sync();
       const int i = base + lane;
       int c = 0, packed = 0, ntr = 0;
       uint32_t owned = 0u, cube = 0u, fm = 0u;
       if (i < n) {
-        c = w.mlist[i];
+        c = el.rlist[lane];
         const int ccls = mc_cell_class(c >> 6, (c >> 3) & 7, c & 7);
-        cube = mc_case_rows(w.rows, c >> 6, (c >> 3) & 7, c & 7);
+        cube = mc_case_rows(el.rows, c >> 6, (c >> 3) & 7, c & 7);
+        el.rcube[lane] = (uint8_t)cube;
         const uint32_t emn = s_emn[cube];
-        owned = (emn & 0xFFFu) & w.self[ccls];
+        owned = (emn & 0xFFFu) & el.self[ccls];
         fm = (emn & 0xFFFu) & el.fitem[ccls];
         ntr = (int)(emn >> 12);
         packed = __popc(owned) | (ntr << 16);
@@ -1016,68 +1062,61 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
       const int vr = excl & 0xFFFF, tr = excl >> 16;
       const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
       const bool listed = rv <= MC_RV && rt <= MC_RT;  // uniform
-      MCCorner cv[MC_RV / 64] = {};
-      if (listed) {
-        // The round's vertices are listed and their corner values requested
-        // first, so that the loads overlap the resolution of foreign edges.
-        if (i < n) {
-          int k = vr;
-          for (uint32_t m = owned; m; m &= m - 1u, ++k) el.vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
-        }
-        wave_lds_sync();
-#pragma unroll
-        for (int j = 0; j < MC_RV / 64; ++j) {
-          const int k = lane + 64 * j;
-          if (k < rv) {
-            const int it = el.vlist[k], cc = it >> 4;
-            cv[j] = mc_corner_load(leaves, w.tleaf, cc >> 6, (cc >> 3) & 7, cc & 7, it & 15, s_einfo);
-          }
-        }
+      if (listed && i < n) {
+        int k = vr;
+        for (uint32_t m = owned; m; m &= m - 1u, ++k) el.vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
       }
-      // Edges owned in an earlier tile, one resolved per lane (each lattice
-      // edge once per tile, so at most MC_FOREIGN per tile).
+      // Edges owned in an earlier tile, one resolved per lane, MC_FL per
+      // pass (each lattice edge once per tile).
       const int nf = __popc(fm);
       const int fincl = wave_incl_scan(nf);
       const int ftot = __builtin_amdgcn_readlane(fincl, 63);
-      if (ftot > 0) {
+      for (int fb = 0; fb < ftot; fb += MC_FL) {
         int k = fincl - nf;
-        for (uint32_t m = fm; m; m &= m - 1u, ++k) el.flist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
+        for (uint32_t m = fm; m; m &= m - 1u, ++k)
+          if (k >= fb && k < fb + MC_FL) el.flist[k - fb] = (uint16_t)((c << 4) | (__ffs(m) - 1));
         wave_lds_sync();
-        for (int k2 = lane; k2 < ftot; k2 += 64) {
+        const int fn = ftot - fb < MC_FL ? ftot - fb : MC_FL;
+        for (int k2 = lane; k2 < fn; k2 += 64) {
           const int it = el.flist[k2], cc = it >> 4, fe = it & 15;
           const int fx = cc >> 6, fy = (cc >> 3) & 7, fz = cc & 7;
-          const int slot = fcarry + k2;
-          el.foreign[slot] = mc_foreign_index(fx, fy, fz, fe, processed, s_ownsort, s_cand, w.ttile, meta, vbase, vrel);
-          el.edge[mc_edge_slot(fx, fy, fz, s_eslot[fe])] = (uint16_t)(0x8000u | (uint32_t)slot);
+          const int slot = fcarry + fb + k2;
+          const int es = mc_edge_slot(fx, fy, fz, s_eslot[fe]);
+          if (slot < MC_FSIDE) {
+            el.foreign[slot] =
+                mc_foreign_index(fx, fy, fz, fe, processed, s_ownsort, s_cand, el.ttile, meta, vbase, vrel);
+            el.edge[es] = (uint16_t)(0x8000u | (uint32_t)slot);
+          } else {
+            el.edge[es] = 0xFFFFu;  // resolved again where a triangle needs it
+          }
         }
-        fcarry += ftot;
         wave_lds_sync();
       }
+      fcarry += ftot;
       if (listed) {
         // (the triangle list shares its LDS with the foreign list above)
         if (i < n)
-          for (int q2 = 0; q2 < ntr; ++q2) el.tlist[tr + q2] = (uint32_t)c | ((uint32_t)q2 << 9) | (cube << 12);
-#pragma unroll
-        for (int j = 0; j < MC_RV / 64; ++j) {
-          const int k = lane + 64 * j;
-          if (k < rv) {
-            const int vi = vb + vcarry + k;
-            const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
-            const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
-            el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
-            if (vi < vmax) mc_vertex_store(cv[j], bg, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
-          }
+          for (int q2 = 0; q2 < ntr; ++q2) el.tlist[tr + q2] = (uint16_t)(lane | (q2 << 6));
+        wave_lds_sync();
+        for (int k = lane; k < rv; k += 64) {
+          const int vi = vb + vcarry + k;
+          const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
+          const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
+          el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
+          if (vi < vmax)
+            mc_write_vertex(leaves, bg, el.tleaf, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
         }
         wave_lds_sync();
         for (int k = lane; k < rt; k += 64) {
           const int ti = tb + tcarry + k;
           const uint32_t it = el.tlist[k];
-          const int cc = (int)(it & 511u), q2 = (int)((it >> 9) & 7u);
+          const int rc = (int)(it & 63u), q2 = (int)(it >> 6);
+          const int cc = el.rlist[rc];
           const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
-          const int8_t* row = s_tri + (it >> 12) * 16 + 3 * q2;
-          const int i0 = mc_corner_index(ux, uy, uz, row[0], el, s_eslot, vbrel);
-          const int i1 = mc_corner_index(ux, uy, uz, row[1], el, s_eslot, vbrel);
-          const int i2 = mc_corner_index(ux, uy, uz, row[2], el, s_eslot, vbrel);
+          const uint64_t row = s_trin[el.rcube[rc]] >> (12 * q2);
+          const int i0 = mc_corner_index(ux, uy, uz, (int)(row & 15u), el, s_eslot, vbrel, fc);
+          const int i1 = mc_corner_index(ux, uy, uz, (int)((row >> 4) & 15u), el, s_eslot, vbrel, fc);
+          const int i2 = mc_corner_index(ux, uy, uz, (int)((row >> 8) & 15u), el, s_eslot, vbrel, fc);
           if (ti < tmax) {
             uint32_t* o = tris + 3 * (size_t)ti;
             o[0] = (uint32_t)i0;
@@ -1095,16 +1134,18 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(4, 
             const int ve = __ffs(m) - 1;
             el.edge[mc_edge_slot(cx, cy, cz, s_eslot[ve])] = (uint16_t)vk;
             if (vb + vk < vmax)
-              mc_write_vertex(leaves, bg, w.tleaf, cx, cy, cz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)(vb + vk));
+              mc_write_vertex(leaves, bg, el.tleaf, cx, cy, cz, ve, to, vs, iso, s_einfo,
+                              verts + 3 * (size_t)(vb + vk));
           }
         }
         wave_lds_sync();
         if (i < n) {
+          const uint64_t row = s_trin[cube];
           int ti = tb + tcarry + tr;
           for (int q2 = 0; q2 < ntr && ti < tmax; ++q2, ++ti)
             for (int jj = 0; jj < 3; ++jj)
-              tris[3 * (size_t)ti + jj] =
-                  (uint32_t)mc_corner_index(cx, cy, cz, s_tri[cube * 16 + 3 * q2 + jj], el, s_eslot, vbrel);
+              tris[3 * (size_t)ti + jj] = (uint32_t)mc_corner_index(
+                  cx, cy, cz, (int)((row >> (4 * (3 * q2 + jj))) & 15u), el, s_eslot, vbrel, fc);
         }
       }
       wave_lds_sync();

@@ -3,7 +3,7 @@ and vs the oracle, in all three modes (loop, eager, graph replay)."""
 import numpy as np
 import pytest
 
-from util import assert_nano_equal
+from util import assert_nano_equal, white_noise_block
 
 pytestmark = pytest.mark.gpu
 
@@ -281,3 +281,20 @@ def test_batch_inputs_changed_after_create(orc, ctx, vr):
     for mode in [vr.BATCH_EAGER, vr.BATCH_GRAPH, vr.BATCH_LOOP]:
         b.step(dt, 0.0, vr.RK3, 0.0, mode)
         check_against_oracle(orc, b, inputs, spec, [dt] * len(spec), 0.0, mode, f"mode {mode}")
+
+
+@pytest.mark.parametrize("mode", ["eager", "graph"])
+def test_batch_white_noise_meshes(orc, ctx, vr, mode):
+    """The batch's sign-mask marching cubes on random values (zero velocity:
+    the advected grid keeps them): dense rounds and tiles whose earlier-tile
+    edges overflow the side table, bit-exact against the oracle."""
+    announce(f"batch white noise {mode}")
+    og = white_noise_block(orc, 11)
+    inputs = [og.to_nano()]
+    spec = [(None, (0.0, 0.0, 0.0))]
+    g = ctx.upload(inputs[0])
+    b = vr.Batch(ctx, [g], [ctx.velocity_analytic(vr.VEL_UNIFORM, (0.0, 0.0, 0.0))])
+    m = vr.BATCH_EAGER if mode == "eager" else vr.BATCH_GRAPH
+    for _ in range(2):
+        b.step(0.25, 0.0, vr.RK3, 0.0, m)
+    check_against_oracle(orc, b, inputs, spec, [0.25], 0.0, m, f"white noise {mode}")

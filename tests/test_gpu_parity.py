@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from util import (NANO_HEADER, active_voxels, assert_nano_equal, sphere128_rigid_dt, sphere128, header, leaf_array,
-                  mesh_equal, rigid_velocity_grids, slotted_sphere)
+                  mesh_equal, rigid_velocity_grids, slotted_sphere, white_noise_block)
 
 pytestmark = pytest.mark.gpu
 
@@ -251,6 +251,19 @@ def test_marching_cubes_matches(orc, ctx, case):
     vg, tg = ctx.upload(og.to_nano()).mesh(0.0)
     mesh_equal(vg, tg, vw, tw, vs, case)
     assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), f"{case}: vertices not bit-exact"
+
+
+def test_marching_cubes_white_noise(orc, ctx):
+    """Random values instead of a distance field: rounds too full for the
+    per-round lists (written cell by cell) and tiles with more edges owned by
+    earlier tiles than the side table holds (resolved again per corner)."""
+    announce("marching cubes white noise")
+    og = white_noise_block(orc, 7)
+    vw, tw, _ = orc.mesh(og, 0.0)
+    vg, tg = ctx.upload(og.to_nano()).mesh(0.0)
+    assert len(tw) > 40 * 512  # dense: ~3 triangles per cell
+    mesh_equal(vg, tg, vw, tw, 1.0, "white noise")
+    assert np.array_equal(vg.view(np.uint64), vw.view(np.uint64)), "white noise: vertices not bit-exact"
 
 
 def test_marching_cubes_empty(orc, ctx):

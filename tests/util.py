@@ -138,3 +138,18 @@ def config4_fd64(args):
     lu = orc.rollout_loss_f64(g0, up, n, 1.0 / n, dt, 1, steps, tg)
     ld = orc.rollout_loss_f64(g0, dn, n, 1.0 / n, dt, 1, steps, tg)
     return (lu - ld) / (2 * h)
+
+
+def white_noise_block(orc, seed, n=24, origin=(-8, -8, -8), bg=3.0):
+    """Seeded uniform [-1, 1) values at every voxel of an n^3 block (all
+    active): nearly every lattice edge crosses 0, so marching-cubes tiles hold
+    far more vertices, triangles and edges owned by earlier tiles than a
+    level set does."""
+    vals = np.random.default_rng(seed).uniform(-1.0, 1.0, size=(n, n, n)).astype(np.float32)
+    g = orc.new_grid(bg, 1.0)
+    ox, oy, oz = origin
+    for x in range(n):
+        for y in range(n):
+            for z in range(n):
+                g.set(ox + x, oy + y, oz + z, float(vals[x, y, z]))
+    return g
