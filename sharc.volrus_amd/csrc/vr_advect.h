@@ -67,80 +67,121 @@ __device__ __forceinline__ void stage_interior(const GridView& in, int li, float
   }
 }
 
-// Halo: the 1000 - 512 brick positions with some local coordinate -1 or 8,
-// as a table built at compile time: brick index | neighbour slot (0..26) <<
-// 10 | voxel offset in that neighbour << 15.
-struct AdvectHalo {
-  uint32_t e[488];
+// The halo as load tasks derived from the task number, with no table: 72
+// four-value tasks (the +-x and +-y faces and the four z-parallel edges,
+// four values contiguous along z in the neighbour leaf, one 16-byte load
+// each) and 200 one-value tasks (the +-z faces, the x- and y-parallel edges
+// and the corners).  A task names its neighbour slot (dx + 1) * 9 + (dy + 1)
+// * 3 + (dz + 1), the voxel offset in that neighbour (local coordinate 7 on
+// a - side, 0 on a + side) and the brick position.
+struct HaloTask {
+  int slot, src, dst;
 };
-constexpr AdvectHalo make_advect_halo() {
-  AdvectHalo t{};
-  int h = 0;
-  for (int bx = -1; bx <= 8; ++bx)
-    for (int by = -1; by <= 8; ++by)
-      for (int bz = -1; bz <= 8; ++bz) {
-        if (bx >= 0 && bx <= 7 && by >= 0 && by <= 7 && bz >= 0 && bz <= 7) continue;
-        const int dx = bx < 0 ? -1 : (bx > 7 ? 1 : 0), dy = by < 0 ? -1 : (by > 7 ? 1 : 0),
-                  dz = bz < 0 ? -1 : (bz > 7 ? 1 : 0);
-        const uint32_t b = (uint32_t)((bx + 1) * 100 + (by + 1) * 10 + (bz + 1));
-        const uint32_t d = (uint32_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
-        const uint32_t off = (uint32_t)(((bx & 7) << 6) | ((by & 7) << 3) | (bz & 7));
-        t.e[h++] = b | (d << 10) | (off << 15);
-      }
+__device__ __forceinline__ HaloTask halo_vec_task(int v) {  // v in [0, 72)
+  HaloTask t;
+  const int h4 = (v & 1) * 4;
+  if (v < 64) {
+    const int face = v >> 4, r = (v >> 1) & 7;
+    if (face < 2) {  // -x (slot 4, its x = 7 slab), +x (slot 22, x = 0): row y = r
+      t.slot = face ? 22 : 4;
+      t.src = (face ? 0 : 448) + r * 8 + h4;
+      t.dst = (face ? 900 : 0) + (r + 1) * 10 + 1 + h4;
+    } else {  // -y (slot 10, y = 7), +y (slot 16, y = 0): row x = r
+      t.slot = face == 3 ? 16 : 10;
+      t.src = r * 64 + (face == 3 ? 0 : 56) + h4;
+      t.dst = (r + 1) * 100 + (face == 3 ? 90 : 0) + 1 + h4;
+    }
+  } else {  // z-parallel edges: x and y at -1 / 8
+    const int e = (v - 64) >> 1, px = e >> 1, py = e & 1;
+    t.slot = px * 18 + py * 6 + 1;
+    t.src = (px ? 0 : 448) + (py ? 0 : 56) + h4;
+    t.dst = (px ? 900 : 0) + (py ? 90 : 0) + 1 + h4;
+  }
   return t;
 }
-static __constant__ AdvectHalo c_advect_halo = make_advect_halo();
-
-// Absent neighbours read the background (leaf 0 is read and discarded, so
-// the load does not wait on the branch).  Each thread's table entries, then
-// its values, are all loaded before any is used (every thread loads, past
-// the end of the table at entry 0), so the round trips overlap instead of
-// following one another.
-template <int NT>
-__device__ __forceinline__ void stage_halo(const GridView& in, const int* nb, float* brick, int tid) {
-  constexpr int HQ = (488 + NT - 1) / NT;
-  const float bg = in.background;
-  uint32_t e[HQ];
-#pragma unroll
-  for (int q = 0; q < HQ; ++q) {
-    const int h = tid + q * NT;
-    e[q] = c_advect_halo.e[h < 488 ? h : 0];
+__device__ __forceinline__ HaloTask halo_one_task(int s) {  // s in [0, 200)
+  HaloTask t;
+  if (s < 128) {  // -z (slot 12, z = 7), +z (slot 14, z = 0)
+    const int f = s >> 6, x = (s >> 3) & 7, y = s & 7;
+    t.slot = f ? 14 : 12;
+    t.src = x * 64 + y * 8 + (f ? 0 : 7);
+    t.dst = (x + 1) * 100 + (y + 1) * 10 + (f ? 9 : 0);
+  } else if (s < 192) {
+    const int e = ((s - 128) >> 3) & 3, k = s & 7, p1 = e >> 1, p2 = e & 1;
+    if (s < 160) {  // y-parallel edges: x and z at -1 / 8
+      t.slot = p1 * 18 + 3 + p2 * 2;
+      t.src = (p1 ? 0 : 448) + k * 8 + (p2 ? 0 : 7);
+      t.dst = (p1 ? 900 : 0) + (k + 1) * 10 + (p2 ? 9 : 0);
+    } else {  // x-parallel edges: y and z at -1 / 8
+      t.slot = 9 + p1 * 6 + p2 * 2;
+      t.src = k * 64 + (p1 ? 0 : 56) + (p2 ? 0 : 7);
+      t.dst = (k + 1) * 100 + (p1 ? 90 : 0) + (p2 ? 9 : 0);
+    }
+  } else {  // corners
+    const int c = s - 192, px = (c >> 2) & 1, py = (c >> 1) & 1, pz = c & 1;
+    t.slot = px * 18 + py * 6 + pz * 2;
+    t.src = (px ? 0 : 448) + (py ? 0 : 56) + (pz ? 0 : 7);
+    t.dst = (px ? 900 : 0) + (py ? 90 : 0) + (pz ? 9 : 0);
   }
-  int src[HQ];
-  float v[HQ];
-#pragma unroll
-  for (int q = 0; q < HQ; ++q) {
-    src[q] = nb[(e[q] >> 10) & 31u];
-    v[q] = in.leaves[src[q] < 0 ? 0 : src[q]].values[e[q] >> 15];
-  }
-#pragma unroll
-  for (int q = 0; q < HQ; ++q)
-    if (tid + q * NT < 488) brick[e[q] & 1023u] = src[q] < 0 ? bg : v[q];
+  return t;
 }
 
-// The halo as stage_halo, with the neighbour list in a register: lane d of
-// every wave holds entry d (d < 27), read with a lane permute, so the list
-// needs no LDS and no barrier before the halo loads.
+// Stages the 488 halo values of the 10^3 brick (the positions with a local
+// coordinate -1 or 8) by tasks with NT threads; nbget(slot) gives the
+// neighbouring leaf (-1: none).  Every load is issued before any LDS write;
+// an absent neighbour reads leaf 0 and is replaced by the background.
+template <int NT, class NB>
+__device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, float* brick, int tid) {
+  constexpr int VQ = (72 + NT - 1) / NT, SQ = (200 + NT - 1) / NT;
+  __builtin_assume(tid >= 0 && tid < NT);
+  const float bg = in.background;
+  float4 vv[VQ];
+  int vsrc[VQ], vdst[VQ];
+#pragma unroll
+  for (int q = 0; q < VQ; ++q) {
+    const int v = tid + q * NT;
+    const HaloTask t = halo_vec_task(v < 72 ? v : 0);
+    vsrc[q] = nbget(t.slot);
+    vdst[q] = v < 72 ? t.dst : -1;
+    vv[q] = *reinterpret_cast<const VR_GLOBAL float4*>(&in.leaves[vsrc[q] < 0 ? 0 : vsrc[q]].values[t.src]);
+  }
+  float sv[SQ];
+  int ssrc[SQ], sdst[SQ];
+#pragma unroll
+  for (int q = 0; q < SQ; ++q) {
+    const int s = tid + q * NT;
+    const HaloTask t = halo_one_task(s < 200 ? s : 0);
+    ssrc[q] = nbget(t.slot);
+    sdst[q] = s < 200 ? t.dst : -1;
+    sv[q] = in.leaves[ssrc[q] < 0 ? 0 : ssrc[q]].values[t.src];
+  }
+#pragma unroll
+  for (int q = 0; q < VQ; ++q)
+    if (vdst[q] >= 0) {
+      const float4 a = vsrc[q] < 0 ? make_float4(bg, bg, bg, bg) : vv[q];
+      float* b = brick + vdst[q];
+      b[0] = a.x;
+      b[1] = a.y;
+      b[2] = a.z;
+      b[3] = a.w;
+    }
+#pragma unroll
+  for (int q = 0; q < SQ; ++q)
+    if (sdst[q] >= 0) brick[sdst[q]] = ssrc[q] < 0 ? bg : sv[q];
+}
+
+// The halo with the neighbour list in LDS (nb[d], d < 27), after a barrier.
+template <int NT>
+__device__ __forceinline__ void stage_halo(const GridView& in, const int* nb, float* brick, int tid) {
+  stage_halo_tasks<NT>(in, [nb](int d) { return nb[d]; }, brick, tid);
+}
+
+// The halo with the neighbour list in a register: lane d of every wave
+// holds entry d (d < 27), read with a lane permute, so the list needs no
+// LDS and no barrier before the halo loads.
 template <int NT>
 __device__ __forceinline__ void stage_halo_reg(const GridView& in, int nbv, float* brick, int tid) {
-  constexpr int HQ = (488 + NT - 1) / NT;
-  const float bg = in.background;
-  uint32_t e[HQ];
-#pragma unroll
-  for (int q = 0; q < HQ; ++q) {
-    const int h = tid + q * NT;
-    e[q] = c_advect_halo.e[h < 488 ? h : 0];
-  }
-  int src[HQ];
-  float v[HQ];
-#pragma unroll
-  for (int q = 0; q < HQ; ++q) {
-    src[q] = __shfl(nbv, (int)((e[q] >> 10) & 31u));
-    v[q] = in.leaves[src[q] < 0 ? 0 : src[q]].values[e[q] >> 15];
-  }
-#pragma unroll
-  for (int q = 0; q < HQ; ++q)
-    if (tid + q * NT < 488) brick[e[q] & 1023u] = src[q] < 0 ? bg : v[q];
+  stage_halo_tasks<NT>(in, [nbv](int d) { return __shfl(nbv, d); }, brick, tid);
 }
 
 // LDS of one advecting block (5088 bytes without the vortex tables, so 32

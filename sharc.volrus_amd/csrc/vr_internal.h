@@ -64,8 +64,8 @@ enum ScratchSlot {
 
 // Device scratch of the marching-cubes passes (reused across calls).
 struct MCMulti {
-  DevBuf views, leaf_base, keep, pos, sums, sums2, totals, tile_base, tiles, tanchor, tleaf, ttile, meta, tv, tt, vb, tb,
-      ranges;
+  DevBuf views, leaf_base, keep, pos, sums, sums2, totals, tile_base, tiles, tanchor, tleaf, ttile, cmask, meta, tv, tt,
+      vb, tb, ranges;
 };
 
 // Kernel classes timed with HIP events when timing is on.

@@ -424,6 +424,33 @@ __global__ __launch_bounds__(256) void k_mc_tile_nbr(const GridView* views, cons
   }
 }
 
+// Per tile and cell class (27: per axis local 0 / 1..6 / 7), the edges a
+// cell of that class owns itself (no candidate ordered before it lies in a
+// processed tile) in bits 0..11, and in bits 16..27 the edges owned in an
+// earlier tile that such a cell resolves for its tile (a candidate ordered
+// before the tile's first one is processed, and no candidate of the tile
+// precedes the cell).  They depend only on which neighbour tiles are
+// processed, so they are built once with the tile lists; candidates ordered
+// before a cell lie in tiles d <= 13, the tile itself and those before it.
+__global__ void k_mc_class_masks(const int* ttile, const int* ntiles, uint32_t* cmask) {
+  const size_t total = (size_t)*ntiles * 27;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = q / 27;
+    const int c = (int)(q % 27);
+    uint32_t processed = 0u;
+#pragma unroll
+    for (int d = 0; d <= 13; ++d) processed |= (uint32_t)(ttile[t * VR_NBR + d] >= 0) << d;
+    uint32_t self = 0u, fitem = 0u;
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+      const uint32_t b = c_mc_before[c * 12 + e], fb = c_mc_fbefore[c * 12 + e];
+      self |= (uint32_t)((b & processed) == 0u) << e;
+      fitem |= (uint32_t)((fb & processed) != 0u && ((b >> 13) & 1u) == 0u) << e;
+    }
+    cmask[q] = self | (fitem << 16);
+  }
+}
+
 // ------------------------------------------------------------------ per-cell rules
 // Owner of a lattice edge: among the four cells that contain it, the one of a
 // processed tile with the smallest visiting key (tile origin, lx, ly, lz).
@@ -590,21 +617,21 @@ __device__ __forceinline__ void mc_mask_rows(const MCMaskQ& q, uint32_t bgs, int
   }
 }
 
-// Loads the tile's neighbour lists (fetched into lanes < 27), stages its
-// sign rows (MASKS: from the mask words q, else from the values) and builds
-// the per-class self masks (no candidate ordered before the cell lies in a
-// processed tile; s_before is c_mc_before in LDS).  Returns the lane's mixed
-// cells (corners on both sides of iso: a case other than 0 or 255): bit z of
-// lane x * 8 + y is cell (x, y, z).  W: per-wave LDS with rows, tleaf,
-// ttile and self.  Whole wave, uniform control flow.
-template <bool MASKS, class W>
+// Loads the tile's neighbour lists and class masks (k_mc_class_masks;
+// fetched into lanes < 27) into LDS, stages its sign rows (MASKS: from the
+// mask words q, else from the values).  Returns the lane's mixed cells
+// (corners on both sides of iso: a case other than 0 or 255): bit z of lane
+// x * 8 + y is cell (x, y, z).  W: per-wave LDS with rows, tleaf, ttile and
+// self (and fitem when FITEM).  Whole wave, uniform control flow.
+template <bool MASKS, bool FITEM = false, class W>
 __device__ __forceinline__ uint32_t mc_tile_rows(const VR_GLOBAL NanoLeaf* leaves, const MCMaskQ& q, float bg,
-                                                 float iso, int tl, int tt, uint32_t processed,
-                                                 const uint32_t* s_before, W& w) {
+                                                 float iso, int tl, int tt, uint32_t cm, W& w) {
   const int lane = lane_id();
   if (lane < VR_NBR) {
     w.tleaf[lane] = tl;
     w.ttile[lane] = tt;
+    w.self[lane] = (uint16_t)(cm & 0xFFFFu);
+    if constexpr (FITEM) w.fitem[lane] = (uint16_t)(cm >> 16);
   }
   wave_lds_sync();
   if constexpr (MASKS) {
@@ -616,12 +643,6 @@ __device__ __forceinline__ uint32_t mc_tile_rows(const VR_GLOBAL NanoLeaf* leave
     mc_half_bits(q0, bg, iso, lane, w.rows);
     mc_half_bits(q1, bg, iso, lane + 64, w.rows);
     if (lane < 162 - 128) mc_half_bits(q2, bg, iso, lane + 128, w.rows);
-  }
-  if (lane < 27) {
-    uint32_t m = 0;
-#pragma unroll
-    for (int e = 0; e < 12; ++e) m |= (uint32_t)((s_before[lane * 12 + e] & processed) == 0u) << e;
-    w.self[lane] = (uint16_t)m;
   }
   wave_lds_sync();
   // Lane x * 8 + y tests its 8 cells along z with bit operations on the
@@ -638,10 +659,9 @@ __device__ __forceinline__ uint32_t mc_tile_rows(const VR_GLOBAL NanoLeaf* leave
 // Returns the mixed-cell count.
 template <bool MASKS>
 __device__ __forceinline__ int mc_tile_setup(const VR_GLOBAL NanoLeaf* leaves, const MCMaskQ& q, float bg, float iso,
-                                             int tl, int tt, uint32_t processed, const uint32_t* s_before,
-                                             MCTileLDS& w) {
+                                             int tl, int tt, uint32_t cm, MCTileLDS& w) {
   const int lane = lane_id();
-  uint32_t mixed = mc_tile_rows<MASKS>(leaves, q, bg, iso, tl, tt, processed, s_before, w);
+  uint32_t mixed = mc_tile_rows<MASKS>(leaves, q, bg, iso, tl, tt, cm, w);
   const int cnt = __popc(mixed);
   const int incl = wave_incl_scan(cnt);
   int k = incl - cnt;
@@ -753,12 +773,16 @@ __device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves
 
 // ------------------------------------------------------------------ pass 1
 // Tile t's lists in one vector load per array: lanes < 27 get its neighbour
-// leaves (*l) and tiles (*m), lane 27 of *l its grid (tiles[t].w).
-__device__ __forceinline__ void mc_count_lists(const int* tleaf, const int* ttile, const int4* tiles, int t, int lane,
-                                               int* l, int* m) {
+// leaves (*l), tiles (*m) and class masks (*k), lane 27 of *l its grid
+// (tiles[t].w).
+__device__ __forceinline__ void mc_count_lists(const int* tleaf, const int* ttile, const uint32_t* cmask,
+                                               const int4* tiles, int t, int lane, int* l, int* m, uint32_t* k) {
   const int* p = lane < VR_NBR ? tleaf + (size_t)t * VR_NBR + lane : reinterpret_cast<const int*>(tiles) + (size_t)t * 4 + 3;
   if (lane <= VR_NBR) *l = p[0];
-  if (lane < VR_NBR) *m = ttile[(size_t)t * VR_NBR + lane];
+  if (lane < VR_NBR) {
+    *m = ttile[(size_t)t * VR_NBR + lane];
+    *k = cmask[(size_t)t * VR_NBR + lane];
+  }
 }
 
 // Per mixed cell (in visiting order): case, owned crossing edges, triangle
@@ -769,14 +793,14 @@ __device__ __forceinline__ void mc_count_lists(const int* tleaf, const int* ttil
 // neighbour lists load while the current one is processed.
 template <bool MASKS>
 __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
-                                                         const int* tleaf, const int* ttile, float iso,
-                                                         uint32_t* meta, int* tv, int* tt, int tcap) {
+                                                         const int* tleaf, const int* ttile, const uint32_t* cmask,
+                                                         float iso, uint32_t* meta, int* tv, int* tt, int tcap) {
   __shared__ uint16_t s_emn[256];
-  __shared__ uint32_t s_before[27 * 12];
+  __shared__ uint8_t s_ccls[512];  // class of cell c = x * 64 + y * 8 + z (mc_cell_class)
   __shared__ MCTileLDS s_w[MC_WAVES];
   const int tid = threadIdx.x, lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int i = tid; i < 256; i += MC_THREADS) s_emn[i] = c_mc_emn[i];
-  for (int i = tid; i < 27 * 12; i += MC_THREADS) s_before[i] = c_mc_before[i];
+  for (int i = tid; i < 512; i += MC_THREADS) s_ccls[i] = (uint8_t)mc_cell_class(i >> 6, (i >> 3) & 7, i & 7);
   __syncthreads();
   MCTileLDS& w = s_w[wv];
   const int nt = *ntiles;
@@ -792,20 +816,22 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   // (q).  Vector loads throughout: a scalar load of a later tile would hold
   // up every LDS wait of this one.
   int ntl = -1, ntt = -1, ftl = -1, ftt = -1;
-  if (t < t_end) mc_count_lists(tleaf, ttile, tiles, t, lane, &ntl, &ntt);
-  if (t + t_step < t_end) mc_count_lists(tleaf, ttile, tiles, t + t_step, lane, &ftl, &ftt);
+  uint32_t ncm = 0u, fcm = 0u;
+  if (t < t_end) mc_count_lists(tleaf, ttile, cmask, tiles, t, lane, &ntl, &ntt, &ncm);
+  if (t + t_step < t_end) mc_count_lists(tleaf, ttile, cmask, tiles, t + t_step, lane, &ftl, &ftt, &fcm);
   MCMaskQ q{};
   if (MASKS && t < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(ntl, VR_NBR)].signs, ntl, lane);
   for (; t < t_end; t += t_step) {
     const int tl = ntl, ttv = ntt;
-    const uint32_t processed = (uint32_t)__ballot(lane < VR_NBR && ttv >= 0);
+    const uint32_t cm = ncm;
     const int tn = t + t_step, tf = tn + t_step;
     ntl = ftl;
     ntt = ftt;
-    if (tf < t_end) mc_count_lists(tleaf, ttile, tiles, tf, lane, &ftl, &ftt);
+    ncm = fcm;
+    if (tf < t_end) mc_count_lists(tleaf, ttile, cmask, tiles, tf, lane, &ftl, &ftt, &fcm);
     const GridView& g = views[__builtin_amdgcn_readlane(tl, VR_NBR)];
     const MCMaskQ cq = q;
-    const int n = mc_tile_setup<MASKS>(g.leaves, cq, g.background, iso, tl, ttv, processed, s_before, w);
+    const int n = mc_tile_setup<MASKS>(g.leaves, cq, g.background, iso, tl, ttv, cm, w);
     // the next tile's mask words load while this one is counted
     if (MASKS && tn < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(ntl, VR_NBR)].signs, ntl, lane);
     int vcarry = 0, tcarry = 0;
@@ -817,7 +843,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
         c = w.mlist[i];
         const int x = c >> 6, y = (c >> 3) & 7, z = c & 7;
         const uint32_t emn = s_emn[mc_case_rows(w.rows, x, y, z)];
-        owned = (emn & 0xFFFu) & w.self[mc_cell_class(x, y, z)];
+        owned = (emn & 0xFFFu) & w.self[s_ccls[c]];
         packed = __popc(owned) | (int)((emn >> 12) << 16);
       }
       const int incl = wave_incl_scan(packed);
@@ -903,12 +929,13 @@ struct MCEmitLDS {
 };
 
 // Tile t's lists and header in one vector load per array: lanes < 27 get
-// its neighbour leaves (*l) and tiles (*m); lanes 27..34 of *l its vertex
+// its neighbour leaves (*l), tiles (*m) and class masks (*k); lanes 27..34
+// of *l its vertex
 // and triangle counts, first vertex and triangle, and tiles[t] (x, y, z,
 // grid).
-__device__ __forceinline__ void mc_emit_lists(const int* tleaf, const int* ttile, const int* tv, const int* tt,
-                                              const int* vbase, const int* tbase, const int4* tiles, int t, int lane,
-                                              int* l, int* m) {
+__device__ __forceinline__ void mc_emit_lists(const int* tleaf, const int* ttile, const uint32_t* cmask, const int* tv,
+                                              const int* tt, const int* vbase, const int* tbase, const int4* tiles,
+                                              int t, int lane, int* l, int* m, uint32_t* k) {
   const int* p = lane < VR_NBR ? tleaf + (size_t)t * VR_NBR + lane
                  : lane == VR_NBR     ? tv + t
                  : lane == VR_NBR + 1 ? tt + t
@@ -916,15 +943,22 @@ __device__ __forceinline__ void mc_emit_lists(const int* tleaf, const int* ttile
                  : lane == VR_NBR + 3 ? tbase + t
                                       : reinterpret_cast<const int*>(tiles) + (size_t)t * 4 + (lane - VR_NBR - 4);
   if (lane < VR_NBR + 8) *l = p[0];
-  if (lane < VR_NBR) *m = ttile[(size_t)t * VR_NBR + lane];
+  if (lane < VR_NBR) {
+    *m = ttile[(size_t)t * VR_NBR + lane];
+    *k = cmask[(size_t)t * VR_NBR + lane];
+  }
 }
 
-// The round's cells (visiting positions base .. base + 63) into rlist, from
-// each lane's mixed cells and their exclusive prefix.
-__device__ __forceinline__ void mc_round_list(uint32_t mixed, int mexcl, int base, int lane, uint16_t* rlist) {
-  int k = mexcl;
-  for (uint32_t m = mixed; m; m &= m - 1u, ++k)
-    if (k >= base && k < base + 64) rlist[k - base] = (uint16_t)(lane * 8 + (__ffs(m) - 1));
+// The round's cells (visiting positions base .. base + 63) into rlist.
+// *rem: the lane's mixed cells not yet listed, *rk: the visiting position
+// of the first of them; rounds run in order, so a lane lists only its cells
+// of this round and consumes them.
+__device__ __forceinline__ void mc_round_list(uint32_t* rem, int* rk, int base, int lane, uint16_t* rlist) {
+  uint32_t m = *rem;
+  int k = *rk;
+  for (; m && k < base + 64; m &= m - 1u, ++k) rlist[k - base] = (uint16_t)(lane * 8 + (__ffs(m) - 1));
+  *rem = m;
+  *rk = k;
 }
 
 // What resolving an earlier tile's edge again needs (mc_foreign_index).
@@ -951,14 +985,13 @@ __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const
 template <bool MASKS>
 __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_mc_emit(
     const GridView* views, const int4* tiles, const int* ntiles, const int* tleaf, const int* ttile,
-    const int* tile_base, float iso, const uint32_t* meta, const int* tv, const int* tt, const int* vbase,
-    const int* tbase, const int* totals, double* verts, uint32_t* tris, long long vcap, long long tricap, int G,
-    int* ranges, int* counts_out) {
+    const int* tile_base, const uint32_t* cmask, float iso, const uint32_t* meta, const int* tv, const int* tt,
+    const int* vbase, const int* tbase, const int* totals, double* verts, uint32_t* tris, long long vcap,
+    long long tricap, int G, int* ranges, int* counts_out) {
   __shared__ uint16_t s_emn[256];
   __shared__ uint64_t s_trin[256];  // TRI_TABLE rows, one nibble per entry
   __shared__ uint32_t s_ownsort[27 * 12];
-  __shared__ uint32_t s_before[27 * 12];
-  __shared__ uint32_t s_fbefore[27 * 12];
+  __shared__ uint8_t s_ccls[512];  // class of cell c = x * 64 + y * 8 + z (mc_cell_class)
   __shared__ uint32_t s_cand[48];
   __shared__ uint32_t s_einfo[12];
   __shared__ uint32_t s_eslot[12];
@@ -968,11 +1001,8 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     s_emn[i] = c_mc_emn[i];
     s_trin[i] = c_mc_trin[i];
   }
-  for (int i = tid; i < 27 * 12; i += MC_ETHREADS) {
-    s_ownsort[i] = c_mc_ownsort[i];
-    s_before[i] = c_mc_before[i];
-    s_fbefore[i] = c_mc_fbefore[i];
-  }
+  for (int i = tid; i < 27 * 12; i += MC_ETHREADS) s_ownsort[i] = c_mc_ownsort[i];
+  for (int i = tid; i < 512; i += MC_ETHREADS) s_ccls[i] = (uint8_t)mc_cell_class(i >> 6, (i >> 3) & 7, i & 7);
   if (tid < 48) s_cand[tid] = c_mc_cand4[tid];
   if (tid < 12) {
     s_einfo[tid] = c_mc_edgeinfo[tid];
@@ -992,12 +1022,15 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
   // current tile.  Vector loads throughout: a scalar load of a later tile
   // would hold up every LDS wait of this one.
   int h_tl = -1, h_tt = -1, ftl = -1, ftt = -1;
-  if (t < t_end) mc_emit_lists(tleaf, ttile, tv, tt, vbase, tbase, tiles, t, lane, &h_tl, &h_tt);
-  if (t + t_step < t_end) mc_emit_lists(tleaf, ttile, tv, tt, vbase, tbase, tiles, t + t_step, lane, &ftl, &ftt);
+  uint32_t h_cm = 0u, fcm = 0u;
+  if (t < t_end) mc_emit_lists(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, t, lane, &h_tl, &h_tt, &h_cm);
+  if (t + t_step < t_end)
+    mc_emit_lists(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, t + t_step, lane, &ftl, &ftt, &fcm);
   MCMaskQ q{};
   if (MASKS && t < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
   for (; t < t_end; t += t_step) {
     const int tl = h_tl, ttv = h_tt;
+    const uint32_t cm = h_cm;
     const int c_nv = __builtin_amdgcn_readlane(tl, VR_NBR), c_nt = __builtin_amdgcn_readlane(tl, VR_NBR + 1);
     const int vb = __builtin_amdgcn_readlane(tl, VR_NBR + 2), tb = __builtin_amdgcn_readlane(tl, VR_NBR + 3);
     const int4 to = make_int4(__builtin_amdgcn_readlane(tl, VR_NBR + 4), __builtin_amdgcn_readlane(tl, VR_NBR + 5),
@@ -1005,7 +1038,8 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     const int tn = t + t_step, tf = tn + t_step;
     h_tl = ftl;
     h_tt = ftt;
-    if (tf < t_end) mc_emit_lists(tleaf, ttile, tv, tt, vbase, tbase, tiles, tf, lane, &ftl, &ftt);
+    h_cm = fcm;
+    if (tf < t_end) mc_emit_lists(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, tf, lane, &ftl, &ftt, &fcm);
     const MCMaskQ cq = q;
     if (c_nv == 0 && c_nt == 0) {  // nothing to write (uniform over the wave)
       if (MASKS && tn < t_end)
@@ -1020,34 +1054,24 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
     const int vbrel = vb - vrel;
-    const uint32_t mixed = mc_tile_rows<MASKS>(leaves, cq, bg, iso, tl, ttv, processed, s_before, el);
+    const uint32_t mixed = mc_tile_rows<MASKS, true>(leaves, cq, bg, iso, tl, ttv, cm, el);
     const int mcnt = __popc(mixed);
     const int mincl = wave_incl_scan(mcnt);
-    const int mexcl = mincl - mcnt;
+    uint32_t rem = mixed;  // cells not yet listed (mc_round_list)
+    int rk = mincl - mcnt;
     const int n = __builtin_amdgcn_readlane(mincl, 63);
     if (MASKS && tn < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
-    // Per class: the edges owned in an earlier tile that this cell resolves
-    // (it is the first cell of this tile containing them: no candidate of
-    // this tile precedes it).
-    if (lane < 27) {
-      uint32_t m = 0;
-#pragma unroll
-      for (int e = 0; e < 12; ++e)
-        m |= (uint32_t)((s_fbefore[lane * 12 + e] & processed) != 0u && ((s_before[lane * 12 + e] >> 13) & 1u) == 0u)
-             << e;
-      el.fitem[lane] = (uint16_t)m;
-    }
     const MCForeign fc{processed, s_ownsort, s_cand, el.ttile, meta, vbase, vrel};
     int vcarry = 0, tcarry = 0, fcarry = 0;
     for (int base = 0; base < n; base += 64) {
-      mc_round_list(mixed, mexcl, base, lane, el.rlist);
+      mc_round_list(&rem, &rk, base, lane, el.rlist);
       wave_lds_sync();
       const int i = base + lane;
       int c = 0, packed = 0, ntr = 0;
       uint32_t owned = 0u, cube = 0u, fm = 0u;
       if (i < n) {
         c = el.rlist[lane];
-        const int ccls = mc_cell_class(c >> 6, (c >> 3) & 7, c & 7);
+        const int ccls = s_ccls[c];
         cube = mc_case_rows(el.rows, c >> 6, (c >> 3) & 7, c & 7);
         el.rcube[lane] = (uint8_t)cube;
         const uint32_t emn = s_emn[cube];
@@ -1211,6 +1235,7 @@ int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int*
   if ((e = s.tiles.ensure(sizeof(int4) * tc)) != hipSuccess || (e = s.tanchor.ensure(sizeof(int) * tc)) != hipSuccess ||
       (e = s.tleaf.ensure(sizeof(int) * VR_NBR * tc)) != hipSuccess ||
       (e = s.ttile.ensure(sizeof(int) * VR_NBR * tc)) != hipSuccess ||
+      (e = s.cmask.ensure(sizeof(uint32_t) * VR_NBR * tc)) != hipSuccess ||
       (e = s.meta.ensure(sizeof(uint32_t) * MC_FACE * tc)) != hipSuccess || (e = s.tv.ensure(sizeof(int) * tc)) != hipSuccess ||
       (e = s.tt.ensure(sizeof(int) * tc)) != hipSuccess || (e = s.vb.ensure(sizeof(int) * tc)) != hipSuccess ||
       (e = s.tb.ensure(sizeof(int) * tc)) != hipSuccess ||
@@ -1234,6 +1259,8 @@ int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int*
     hipLaunchKernelGGL(k_mc_tile_nbr_slots, dim3(blocks_for((size_t)tcap * VR_NBR, 256)), dim3(256), 0, ctx->stream,
                        d_views, s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(),
                        s.tleaf.as<int>(), s.ttile.as<int>());
+  hipLaunchKernelGGL(k_mc_class_masks, dim3(blocks_for((size_t)tcap * VR_NBR, 256)), dim3(256), 0, ctx->stream,
+                     s.ttile.as<int>(), totals, s.cmask.as<uint32_t>());
   return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc tiles");
 }
 
@@ -1246,7 +1273,8 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long l
     const void* k = masks ? (const void*)k_mc_count<true> : (const void*)k_mc_count<false>;
     hipLaunchKernelGGL(masks ? k_mc_count<true> : k_mc_count<false>, dim3(mc_blocks(ctx, k, MC_WAVES, tcap)),
                        dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(), totals, s.tleaf.as<int>(),
-                       s.ttile.as<int>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), tcap);
+                       s.ttile.as<int>(), s.cmask.as<uint32_t>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(),
+                       s.tt.as<int>(), tcap);
   }
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = vr_exclusive_scan2(s.tv.as<int>(), s.vb.as<int>(), totals + 1, s.tt.as<int>(), s.tb.as<int>(),
@@ -1269,8 +1297,8 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int t
     const void* k = masks ? (const void*)k_mc_emit<true> : (const void*)k_mc_emit<false>;
     hipLaunchKernelGGL(masks ? k_mc_emit<true> : k_mc_emit<false>, dim3(mc_blocks(ctx, k, MC_EWAVES, tcap)),
                        dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
-                       s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(), iso,
-                       s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
+                       s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(),
+                       s.cmask.as<uint32_t>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
                        s.totals.as<int>(), verts, tris, vcap, tricap, G, s.ranges.as<int>(), counts_out);
   }
   return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc emit");
@@ -1358,8 +1386,8 @@ int mc_run(vr_ctx* ctx, const vr_grid* g, float iso, MCScratch* unused, vr_mesh*
 // Byte offsets of the MCMulti ranges for n leaves (tile capacity 7 n); the
 // same function sizes the array and carves it.
 struct MCNoAllocLayout {
-  size_t off[18];
-  size_t len[18];
+  size_t off[19];
+  size_t len[19];
   size_t total;
 };
 static constexpr size_t vr_align_up(size_t x) { return (x + VR_NOALLOC_ALIGN - 1) / VR_NOALLOC_ALIGN * VR_NOALLOC_ALIGN; }
@@ -1367,13 +1395,13 @@ static constexpr size_t vr_scan_ints(size_t n) { return (n + 1023) / 1024 + 1; }
 static constexpr MCNoAllocLayout mc_noalloc_layout(size_t n) {
   MCNoAllocLayout L{};
   const size_t n7 = 7 * n, tc = 7 * (n < 1 ? 1 : n) + 1;
-  const size_t len[18] = {sizeof(GridView), 2 * sizeof(int), sizeof(int) * (n7 + 1), sizeof(int) * (n7 + 1),
+  const size_t len[19] = {sizeof(GridView), 2 * sizeof(int), sizeof(int) * (n7 + 1), sizeof(int) * (n7 + 1),
                           sizeof(int) * (vr_scan_ints(n7) + 1), sizeof(int) * (vr_scan_ints(tc) + 1), 4 * sizeof(int),
                           2 * sizeof(int), 16 * tc, sizeof(int) * tc, sizeof(int) * VR_NBR * tc,
-                          sizeof(int) * VR_NBR * tc, sizeof(uint32_t) * MC_FACE * tc, sizeof(int) * tc,
-                          sizeof(int) * tc, sizeof(int) * tc, sizeof(int) * tc, 8 * sizeof(int)};
+                          sizeof(int) * VR_NBR * tc, sizeof(uint32_t) * VR_NBR * tc, sizeof(uint32_t) * MC_FACE * tc,
+                          sizeof(int) * tc, sizeof(int) * tc, sizeof(int) * tc, sizeof(int) * tc, 8 * sizeof(int)};
   size_t at = 0;
-  for (int i = 0; i < 18; ++i) {
+  for (int i = 0; i < 19; ++i) {
     L.off[i] = at;
     L.len[i] = vr_align_up(len[i]);
     at += L.len[i];
@@ -1425,11 +1453,12 @@ extern "C" int vr_marching_cubes_noalloc(vr_ctx* ctx, const vr_grid* g, float is
   }
   // The passes' buffers as borrowed ranges of this device's array, laid out
   // for this grid's leaf count (every range within the array: n <= max).
-  constexpr int kSlots = 18;
+  constexpr int kSlots = 19;
   const MCNoAllocLayout L = mc_noalloc_layout((size_t)n);
   MCMulti s;
   DevBuf* slot[kSlots] = {&s.views, &s.leaf_base, &s.keep, &s.pos, &s.sums, &s.sums2, &s.totals, &s.tile_base,
-                          &s.tiles, &s.tanchor, &s.tleaf, &s.ttile, &s.meta, &s.tv, &s.tt, &s.vb, &s.tb, &s.ranges};
+                          &s.tiles, &s.tanchor, &s.tleaf, &s.ttile, &s.cmask, &s.meta, &s.tv, &s.tt, &s.vb, &s.tb,
+                          &s.ranges};
   for (int i = 0; i < kSlots; ++i) slot[i]->borrow(base + L.off[i], L.len[i]);
   // No event records (they would create events) inside this call.
   const bool timing = ctx->timing;
