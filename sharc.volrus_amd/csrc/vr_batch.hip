@@ -391,10 +391,27 @@ static int batch_mc(vr_ctx* ctx, vr_batch* b, float iso, bool sizing) {
     if (rc != VR_OK) return rc;
     b->tiles_ready = true;
   }
-  rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->ltot, b->tcap, iso, true);
+  long long waves = 1;
+  rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->ltot, b->tcap, iso, true, &waves);
   if (rc != VR_OK) return rc;
   double mesh_bytes = 0.0;  // known to the host after a sizing run only (timing is off in graphs)
   if (sizing) {
+    // The count pass's mixed-cell lists get room for this step's demand (a
+    // step of the same parameters has the same cells, so a graph replay
+    // fits too); when they did not fit, the count runs again.
+    unsigned long long share = 0;
+    if ((e = hipMemcpyAsync(&share, b->mc.mctr.p, sizeof(share), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+      return vr_hip_fail(ctx, e, "readback");
+    const unsigned long long want = share * (unsigned long long)waves;
+    if (want > 0x7fffffffULL)
+      return vr_fail(ctx, VR_ERR_CAPACITY, "batch needs %llu mixed-cell list entries (at most 2^31 - 1)", want);
+    if (share > b->mc.mcell.bytes / sizeof(uint32_t) / (unsigned long long)waves) {
+      batch_drop_graph(b);  // it reads the old lists
+      if ((e = b->mc.mcell.ensure(sizeof(uint32_t) * (size_t)want)) != hipSuccess) return vr_hip_fail(ctx, e, "hipMalloc");
+      rc = mc_count_multi(ctx, b->mc, b->d_out_views.as<GridView>(), b->ltot, b->tcap, iso, true);
+      if (rc != VR_OK) return rc;
+    }
     int th[3];
     if ((e = hipMemcpyAsync(th, b->mc.totals.p, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)

@@ -66,6 +66,10 @@ enum ScratchSlot {
 struct MCMulti {
   DevBuf views, leaf_base, keep, pos, sums, sums2, totals, tile_base, tiles, tanchor, tleaf, ttile, cmask, meta, tv, tt,
       vb, tb, ranges;
+  // Mixed-cell lists handed from the count pass to the emit pass (masks):
+  // 4 bytes per mixed cell (mcell, sized by the caller), offset and count
+  // per tile (mtile), the 64-bit allocation counter (mctr).
+  DevBuf mcell, mtile, mctr;
 };
 
 // Kernel classes timed with HIP events when timing is on.
@@ -196,9 +200,12 @@ int mc_tiles_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* 
 int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int* d_leaf_base, int G, long long ltot,
                     int tcap);
 // masks: the views carry sign masks for `iso` (GridView::signs), which the
-// cells are classified from instead of the values.
+// cells are classified from instead of the values; the count pass then also
+// writes each tile's mixed cells to s.mcell for the emit pass, each of its
+// waves into an equal share (*waves_out: the count of waves), and the
+// largest share a wave needed, in cells, to s.mctr.
 int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso,
-                   bool masks = false);
+                   bool masks = false, long long* waves_out = nullptr);
 // Timed (class TC_MC_EMIT) with algorithmic bytes per leaf (2128, the leaf,
 // or with masks its 2048 value bytes and 64-byte sign mask) * leaf_count +
 // mesh_bytes (24 per vertex + 12 per triangle written).

@@ -791,10 +791,19 @@ __device__ __forceinline__ void mc_count_lists(const int* tleaf, const int* ttil
 // per-tile vertex and triangle totals.  Cells with case 0 or 255 add nothing
 // and are never visited.  Each wave loops over tiles; the next tile's
 // neighbour lists load while the current one is processed.
-template <bool MASKS>
+// LIST: the tile's mixed cells in visiting order also go to mcell as cell |
+// case << 9.  Each wave writes its tiles' lists one after another into its
+// own equal share of mcap cells (the tiles of a wave are fixed by the grid
+// size), so no offset is shared; *mctr (64-bit, cleared before the launch)
+// ends at the largest share any wave needed.  mtile[2 t] = the list's
+// offset (-1 past the wave's share, and for tiles without mixed cells),
+// mtile[2 t + 1] = the mixed-cell count.
+template <bool MASKS, bool LIST>
 __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
                                                          const int* tleaf, const int* ttile, const uint32_t* cmask,
-                                                         float iso, uint32_t* meta, int* tv, int* tt, int tcap) {
+                                                         float iso, uint32_t* meta, int* tv, int* tt, int tcap,
+                                                         uint32_t* mcell, int* mtile, unsigned long long* mctr,
+                                                         long long mcap) {
   __shared__ uint16_t s_emn[256];
   __shared__ uint8_t s_ccls[512];  // class of cell c = x * 64 + y * 8 + z (mc_cell_class)
   __shared__ MCTileLDS s_w[MC_WAVES];
@@ -811,6 +820,10 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   }
   int t_end, t_step;
   int t = xcd_first_sub(nt, wv, MC_WAVES, &t_end, &t_step);
+  // LIST: this wave's share of mcell and the cells its tiles used so far
+  const long long share = LIST ? mcap / ((long long)gridDim.x * MC_WAVES) : 0;
+  const long long share0 = share * ((long long)blockIdx.x * MC_WAVES + wv);
+  long long used = 0;
   // Lists of tile t (ntl, ntt) and of the tile after it (ftl, ftt) are in
   // flight, loaded by mc_count_lists; MASKS: so are tile t's sign-mask words
   // (q).  Vector loads throughout: a scalar load of a later tile would hold
@@ -834,6 +847,17 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
     const int n = mc_tile_setup<MASKS>(g.leaves, cq, g.background, iso, tl, ttv, cm, w);
     // the next tile's mask words load while this one is counted
     if (MASKS && tn < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(ntl, VR_NBR)].signs, ntl, lane);
+    long long moff = -1;
+    if (LIST) {
+      if (n > 0) {
+        moff = used + n <= share ? share0 + used : -1;
+        used += n;
+      }
+      if (lane == 0) {
+        mtile[2 * (size_t)t] = (int)moff;
+        mtile[2 * (size_t)t + 1] = n;
+      }
+    }
     int vcarry = 0, tcarry = 0;
     for (int base = 0; base < n; base += 64) {
       const int i = base + lane;
@@ -842,9 +866,11 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
       if (i < n) {
         c = w.mlist[i];
         const int x = c >> 6, y = (c >> 3) & 7, z = c & 7;
-        const uint32_t emn = s_emn[mc_case_rows(w.rows, x, y, z)];
+        const uint32_t cube = mc_case_rows(w.rows, x, y, z);
+        const uint32_t emn = s_emn[cube];
         owned = (emn & 0xFFFu) & w.self[s_ccls[c]];
         packed = __popc(owned) | (int)((emn >> 12) << 16);
+        if (LIST && moff >= 0) mcell[moff + i] = (uint32_t)c | (cube << 9);
       }
       const int incl = wave_incl_scan(packed);
       if (i < n) {
@@ -861,6 +887,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
     }
     wave_lds_sync();
   }
+  if (LIST && lane == 0 && used > 0) atomicMax(mctr, (unsigned long long)used);
 }
 
 // Per-grid vertex and triangle ranges: out[4 g] = first vertex, [4 g + 1] =
@@ -930,19 +957,21 @@ struct MCEmitLDS {
 
 // Tile t's lists and header in one vector load per array: lanes < 27 get
 // its neighbour leaves (*l), tiles (*m) and class masks (*k); lanes 27..34
-// of *l its vertex
-// and triangle counts, first vertex and triangle, and tiles[t] (x, y, z,
-// grid).
+// of *l its vertex and triangle counts, first vertex and triangle, and
+// tiles[t] (x, y, z, grid); LIST: lanes 35 and 36 its mixed-cell list
+// offset and count (mtile).
+template <bool LIST>
 __device__ __forceinline__ void mc_emit_lists(const int* tleaf, const int* ttile, const uint32_t* cmask, const int* tv,
                                               const int* tt, const int* vbase, const int* tbase, const int4* tiles,
-                                              int t, int lane, int* l, int* m, uint32_t* k) {
+                                              const int* mtile, int t, int lane, int* l, int* m, uint32_t* k) {
   const int* p = lane < VR_NBR ? tleaf + (size_t)t * VR_NBR + lane
                  : lane == VR_NBR     ? tv + t
                  : lane == VR_NBR + 1 ? tt + t
                  : lane == VR_NBR + 2 ? vbase + t
                  : lane == VR_NBR + 3 ? tbase + t
-                                      : reinterpret_cast<const int*>(tiles) + (size_t)t * 4 + (lane - VR_NBR - 4);
-  if (lane < VR_NBR + 8) *l = p[0];
+                 : lane < VR_NBR + 8  ? reinterpret_cast<const int*>(tiles) + (size_t)t * 4 + (lane - VR_NBR - 4)
+                                      : mtile + 2 * (size_t)t + (lane - VR_NBR - 8);
+  if (lane < VR_NBR + (LIST ? 10 : 8)) *l = p[0];
   if (lane < VR_NBR) {
     *m = ttile[(size_t)t * VR_NBR + lane];
     *k = cmask[(size_t)t * VR_NBR + lane];
@@ -959,6 +988,17 @@ __device__ __forceinline__ void mc_round_list(uint32_t* rem, int* rk, int base, 
   for (; m && k < base + 64; m &= m - 1u, ++k) rlist[k - base] = (uint16_t)(lane * 8 + (__ffs(m) - 1));
   *rem = m;
   *rk = k;
+}
+
+// LIST: requests the entries of the first two rounds of the tile whose
+// header hl holds (mc_emit_lists): *a = entry lane, *b = entry 64 + lane
+// (0 past the count).
+__device__ __forceinline__ void mc_list_issue(const uint32_t* mcell, int hl, int lane, uint32_t* a, uint32_t* b) {
+  const int off = __builtin_amdgcn_readlane(hl, VR_NBR + 8), cnt = __builtin_amdgcn_readlane(hl, VR_NBR + 9);
+  const int nv = __builtin_amdgcn_readlane(hl, VR_NBR), ntr = __builtin_amdgcn_readlane(hl, VR_NBR + 1);
+  if (off < 0 || (nv == 0 && ntr == 0)) return;
+  if (lane < cnt) *a = mcell[(size_t)off + lane];
+  if (64 + lane < cnt) *b = mcell[(size_t)off + 64 + lane];
 }
 
 // What resolving an earlier tile's edge again needs (mc_foreign_index).
@@ -982,12 +1022,17 @@ __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const
   return (v & 0x8000u) ? f : vbrel + (int)v;
 }
 
-template <bool MASKS>
+// LIST: the tiles' mixed cells and cases come from the count pass's lists
+// (mcell, mtile) instead of the sign rows; the first two rounds' entries
+// of the next tile load while the current one is processed.  A tile whose
+// list did not fit (offset -1) writes nothing: that happens only when the
+// mesh is larger than the capacity the lists were sized for.
+template <bool MASKS, bool LIST>
 __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_mc_emit(
     const GridView* views, const int4* tiles, const int* ntiles, const int* tleaf, const int* ttile,
     const int* tile_base, const uint32_t* cmask, float iso, const uint32_t* meta, const int* tv, const int* tt,
     const int* vbase, const int* tbase, const int* totals, double* verts, uint32_t* tris, long long vcap,
-    long long tricap, int G, int* ranges, int* counts_out) {
+    long long tricap, int G, int* ranges, int* counts_out, const uint32_t* mcell, const int* mtile) {
   __shared__ uint16_t s_emn[256];
   __shared__ uint64_t s_trin[256];  // TRI_TABLE rows, one nibble per entry
   __shared__ uint32_t s_ownsort[27 * 12];
@@ -1023,11 +1068,17 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
   // would hold up every LDS wait of this one.
   int h_tl = -1, h_tt = -1, ftl = -1, ftt = -1;
   uint32_t h_cm = 0u, fcm = 0u;
-  if (t < t_end) mc_emit_lists(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, t, lane, &h_tl, &h_tt, &h_cm);
+  if (t < t_end)
+    mc_emit_lists<LIST>(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, mtile, t, lane, &h_tl, &h_tt, &h_cm);
   if (t + t_step < t_end)
-    mc_emit_lists(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, t + t_step, lane, &ftl, &ftt, &fcm);
+    mc_emit_lists<LIST>(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, mtile, t + t_step, lane, &ftl, &ftt, &fcm);
   MCMaskQ q{};
-  if (MASKS && t < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
+  if (MASKS && !LIST && t < t_end)
+    q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
+  // LIST: entries of the first two rounds of tile t (e0, e1); lanes past
+  // the count hold 0.
+  uint32_t e0 = 0u, e1 = 0u;
+  if (LIST && t < t_end) mc_list_issue(mcell, h_tl, lane, &e0, &e1);
   for (; t < t_end; t += t_step) {
     const int tl = h_tl, ttv = h_tt;
     const uint32_t cm = h_cm;
@@ -1036,16 +1087,25 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     const int4 to = make_int4(__builtin_amdgcn_readlane(tl, VR_NBR + 4), __builtin_amdgcn_readlane(tl, VR_NBR + 5),
                               __builtin_amdgcn_readlane(tl, VR_NBR + 6), __builtin_amdgcn_readlane(tl, VR_NBR + 7));
     const int tn = t + t_step, tf = tn + t_step;
+    const int moff = LIST ? __builtin_amdgcn_readlane(tl, VR_NBR + 8) : 0;
+    const int mn = LIST ? __builtin_amdgcn_readlane(tl, VR_NBR + 9) : 0;
     h_tl = ftl;
     h_tt = ftt;
     h_cm = fcm;
-    if (tf < t_end) mc_emit_lists(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, tf, lane, &ftl, &ftt, &fcm);
+    if (tf < t_end)
+      mc_emit_lists<LIST>(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, mtile, tf, lane, &ftl, &ftt, &fcm);
     const MCMaskQ cq = q;
+    uint32_t c0 = e0, c1 = e1;  // LIST: this tile's first two rounds
+    if (LIST) {
+      e0 = e1 = 0u;
+      if (tn < t_end) mc_list_issue(mcell, h_tl, lane, &e0, &e1);
+    }
     if (c_nv == 0 && c_nt == 0) {  // nothing to write (uniform over the wave)
-      if (MASKS && tn < t_end)
+      if (MASKS && !LIST && tn < t_end)
         q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
       continue;
     }
+    if (LIST && moff < 0) continue;  // the list did not fit: the mesh exceeds its capacity
     const uint32_t processed = (uint32_t)__ballot(lane < VR_NBR && ttv >= 0);
     const GridView& g = views[to.w];
     const VR_GLOBAL NanoLeaf* leaves = g.leaves;
@@ -1054,25 +1114,48 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     const int g0 = tile_base[to.w];
     const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
     const int vbrel = vb - vrel;
-    const uint32_t mixed = mc_tile_rows<MASKS, true>(leaves, cq, bg, iso, tl, ttv, cm, el);
-    const int mcnt = __popc(mixed);
-    const int mincl = wave_incl_scan(mcnt);
-    uint32_t rem = mixed;  // cells not yet listed (mc_round_list)
-    int rk = mincl - mcnt;
-    const int n = __builtin_amdgcn_readlane(mincl, 63);
-    if (MASKS && tn < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
+    uint32_t rem = 0u;  // !LIST: cells not yet listed (mc_round_list)
+    int rk = 0, n;
+    if constexpr (LIST) {
+      if (lane < VR_NBR) {
+        el.tleaf[lane] = tl;
+        el.ttile[lane] = ttv;
+        el.self[lane] = (uint16_t)(cm & 0xFFFFu);
+        el.fitem[lane] = (uint16_t)(cm >> 16);
+      }
+      n = mn;
+    } else {
+      const uint32_t mixed = mc_tile_rows<MASKS, true>(leaves, cq, bg, iso, tl, ttv, cm, el);
+      const int mcnt = __popc(mixed);
+      const int mincl = wave_incl_scan(mcnt);
+      rem = mixed;
+      rk = mincl - mcnt;
+      n = __builtin_amdgcn_readlane(mincl, 63);
+      if (MASKS && tn < t_end)
+        q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
+    }
     const MCForeign fc{processed, s_ownsort, s_cand, el.ttile, meta, vbase, vrel};
     int vcarry = 0, tcarry = 0, fcarry = 0;
     for (int base = 0; base < n; base += 64) {
-      mc_round_list(&rem, &rk, base, lane, el.rlist);
-      wave_lds_sync();
       const int i = base + lane;
+      uint32_t ent = 0u;
+      if constexpr (LIST) {
+        // this round's entry, and the load of the round after next
+        ent = c0;
+        c0 = c1;
+        c1 = 0u;
+        if (base + 128 + lane < n) c1 = mcell[(size_t)moff + base + 128 + lane];
+        if (i < n) el.rlist[lane] = (uint16_t)(ent & 511u);
+      } else {
+        mc_round_list(&rem, &rk, base, lane, el.rlist);
+      }
+      wave_lds_sync();
       int c = 0, packed = 0, ntr = 0;
       uint32_t owned = 0u, cube = 0u, fm = 0u;
       if (i < n) {
-        c = el.rlist[lane];
+        c = LIST ? (int)(ent & 511u) : el.rlist[lane];
         const int ccls = s_ccls[c];
-        cube = mc_case_rows(el.rows, c >> 6, (c >> 3) & 7, c & 7);
+        cube = LIST ? (ent >> 9) : mc_case_rows(el.rows, c >> 6, (c >> 3) & 7, c & 7);
         el.rcube[lane] = (uint8_t)cube;
         const uint32_t emn = s_emn[cube];
         owned = (emn & 0xFFFu) & el.self[ccls];
@@ -1265,16 +1348,29 @@ int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int*
 }
 
 // Phase C: count, scans (the tile lists of phases A and B in s).
-int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso, bool masks) {
+int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long ltot, int tcap, float iso, bool masks,
+                   long long* waves_out) {
   hipError_t e;
   int* totals = s.totals.as<int>();
+  long long mcap = 0;
+  if (masks) {
+    if ((e = s.mtile.ensure(sizeof(int) * 2 * (size_t)(tcap + 1))) != hipSuccess ||
+        (e = s.mctr.ensure(sizeof(unsigned long long))) != hipSuccess ||
+        (e = vr_fill_int(s.mctr.as<int>(), 0, 2, ctx->stream)) != hipSuccess)
+      return vr_hip_fail(ctx, e, "scratch");
+    mcap = (long long)(s.mcell.bytes / sizeof(uint32_t));
+    if (mcap > 0x7fffffffLL) mcap = 0x7fffffffLL;  // offsets are stored as int
+  }
   {
     TimedLaunch tl(ctx, TC_MC, masks ? 64.0 : (double)VR_LEAF_BYTES, 0.0, (double)ltot);
-    const void* k = masks ? (const void*)k_mc_count<true> : (const void*)k_mc_count<false>;
-    hipLaunchKernelGGL(masks ? k_mc_count<true> : k_mc_count<false>, dim3(mc_blocks(ctx, k, MC_WAVES, tcap)),
-                       dim3(MC_THREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(), totals, s.tleaf.as<int>(),
-                       s.ttile.as<int>(), s.cmask.as<uint32_t>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(),
-                       s.tt.as<int>(), tcap);
+    const void* k = masks ? (const void*)k_mc_count<true, true> : (const void*)k_mc_count<false, false>;
+    const int blocks = mc_blocks(ctx, k, MC_WAVES, tcap);
+    if (waves_out) *waves_out = (long long)blocks * MC_WAVES;
+    hipLaunchKernelGGL((masks ? k_mc_count<true, true> : k_mc_count<false, false>),
+                       dim3(blocks), dim3(MC_THREADS), 0, ctx->stream, d_views,
+                       s.tiles.as<int4>(), totals, s.tleaf.as<int>(), s.ttile.as<int>(), s.cmask.as<uint32_t>(), iso,
+                       s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), tcap, s.mcell.as<uint32_t>(),
+                       s.mtile.as<int>(), s.mctr.as<unsigned long long>(), mcap);
   }
   if ((e = hipGetLastError()) != hipSuccess ||
       (e = vr_exclusive_scan2(s.tv.as<int>(), s.vb.as<int>(), totals + 1, s.tt.as<int>(), s.tb.as<int>(),
@@ -1294,12 +1390,13 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int t
   {
     // with masks: each leaf's sign mask and its 2048 value bytes (vertices interpolate values throughout the leaf)
     TimedLaunch tl(ctx, TC_MC_EMIT, masks ? 2048.0 + 64.0 : (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
-    const void* k = masks ? (const void*)k_mc_emit<true> : (const void*)k_mc_emit<false>;
-    hipLaunchKernelGGL(masks ? k_mc_emit<true> : k_mc_emit<false>, dim3(mc_blocks(ctx, k, MC_EWAVES, tcap)),
-                       dim3(MC_ETHREADS), 0, ctx->stream, d_views, s.tiles.as<int4>(),
-                       s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(), s.tile_base.as<int>(),
-                       s.cmask.as<uint32_t>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(),
-                       s.totals.as<int>(), verts, tris, vcap, tricap, G, s.ranges.as<int>(), counts_out);
+    const void* k = masks ? (const void*)k_mc_emit<true, true> : (const void*)k_mc_emit<false, false>;
+    hipLaunchKernelGGL((masks ? k_mc_emit<true, true> : k_mc_emit<false, false>),
+                       dim3(mc_blocks(ctx, k, MC_EWAVES, tcap)), dim3(MC_ETHREADS), 0, ctx->stream, d_views,
+                       s.tiles.as<int4>(), s.totals.as<int>(), s.tleaf.as<int>(), s.ttile.as<int>(),
+                       s.tile_base.as<int>(), s.cmask.as<uint32_t>(), iso, s.meta.as<uint32_t>(), s.tv.as<int>(),
+                       s.tt.as<int>(), s.vb.as<int>(), s.tb.as<int>(), s.totals.as<int>(), verts, tris, vcap, tricap,
+                       G, s.ranges.as<int>(), counts_out, s.mcell.as<uint32_t>(), s.mtile.as<int>());
   }
   return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc emit");
 }
