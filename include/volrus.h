@@ -158,7 +158,7 @@ int vr_mesh_create(vr_ctx* ctx, vr_mesh** out);
 /* No counterpart in the reference: volume_to_mesh for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops).  It calls no
  * hipMalloc or hipFree and synchronises nothing: its scratch is a fixed
- * array in device memory (about 27 MB per device, held until the process
+ * array in device memory (about 30 MB per device, held until the process
  * ends) for grids of at most VR_NOALLOC_MAX_LEAVES leaves (VR_ERR_CAPACITY
  * above that).  The caller passes device buffers: xyz (3 * vcap doubles),
  * tri (3 * tcap uint32) and counts (2 int32: vertices, triangles, written on

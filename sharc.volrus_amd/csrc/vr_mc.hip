@@ -916,8 +916,9 @@ __device__ __forceinline__ void mc_ranges(const int* tile_base, int G, int nt, c
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
 // Mixed cells go in rounds of 64, one per lane: a wave scan places the
 // round's vertices and triangles, and lists of the round's vertices
-// (cell, edge) and triangles (round cell, row entry) spread the writes one
-// item per lane.  Vertex indices go through the tile's lattice-edge table
+// (cell, edge) and triangles (round cell and row entry; with the count
+// pass's lists, the cell and the triangle's three edges) spread the writes
+// one item per lane.  Vertex indices go through the tile's lattice-edge table
 // in LDS (16 bits per edge): an owned vertex stores its index relative to
 // the tile's first vertex at its edge's slot; an edge owned in an earlier
 // tile (it lies on one of the tile's three low planes, at most 3 x 144) is
@@ -930,8 +931,9 @@ __device__ __forceinline__ void mc_ranges(const int* tile_base, int G, int nt, c
 // round's triangles.  A round with more items than the lists hold is
 // written cell by cell.  Vertex indices are relative to the tile's grid.
 // The per-wave LDS is kept under 5.8 KB (the round's cell list is built per
-// round, the triangle list holds round-local cell indices, the triangle
-// table is nibble-packed), so three 8-wave blocks fit a CU: 6 waves per SIMD.
+// round, the 32-bit triangle entries of the list mode share the space of
+// the sign rows it does not use, the triangle table is nibble-packed), so
+// three 8-wave blocks fit a CU: 6 waves per SIMD.
 #define MC_EWAVES 8
 #define MC_ETHREADS (64 * MC_EWAVES)
 #define MC_RV 128   // vertices listed per round
@@ -939,7 +941,19 @@ __device__ __forceinline__ void mc_ranges(const int* tile_base, int G, int nt, c
 #define MC_FL 192   // earlier-tile edges listed per resolution pass
 #define MC_FSIDE 96 // side-table entries per tile
 struct MCEmitLDS {
-  uint32_t rows[81];             // sign rows: bit z of rows[x * 9 + y] = value at (x, y, z) < iso
+  union {
+    struct {                     // classification from sign rows (no lists)
+      uint32_t rows[81];         // sign rows: bit z of rows[x * 9 + y] = value at (x, y, z) < iso
+      union {
+        uint16_t tlist[MC_RT];   // round cell | triangle of the cell << 6
+        uint16_t flist[MC_FL];   // cell << 4 | edge, earlier-tile edges to resolve
+      };
+    };
+    union {                      // lists from the count pass
+      uint32_t tlist32[MC_RT];   // cell | the triangle's edges << 9 (4 bits each)
+      uint16_t flist32[MC_FL];   // as flist
+    };
+  };
   int tleaf[VR_NBR];             // neighbouring leaves (-1: none)
   int ttile[VR_NBR];             // neighbouring tiles (-1: not processed)
   uint16_t self[28];             // per cell class: edges the cell owns itself
@@ -948,10 +962,6 @@ struct MCEmitLDS {
   uint8_t rcube[64];             // their cases
   uint16_t edge[MC_EDGES];       // lattice-edge table of the tile
   int foreign[MC_FSIDE];         // vertex index (relative to the grid) of edges owned in an earlier tile
-  union {
-    uint16_t tlist[MC_RT];       // round cell | triangle of the cell << 6
-    uint16_t flist[MC_FL];       // cell << 4 | edge, earlier-tile edges to resolve
-  };
   uint16_t vlist[MC_RV];         // cell << 4 | edge
 };
 
@@ -1173,19 +1183,35 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
         int k = vr;
         for (uint32_t m = owned; m; m &= m - 1u, ++k) el.vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
       }
+      // LIST: the corner values of the round's vertices (at most MC_RV = 2 x
+      // 64) are requested now and consumed after the foreign edges and the
+      // triangle list are done.
+      MCCorner pc0{}, pc1{};
+      if (LIST && listed) {
+        wave_lds_sync();
+        if (lane < rv) {
+          const int it = el.vlist[lane];
+          pc0 = mc_corner_load(leaves, el.tleaf, (it >> 10) & 7, (it >> 7) & 7, (it >> 4) & 7, it & 15, s_einfo);
+        }
+        if (64 + lane < rv) {
+          const int it = el.vlist[64 + lane];
+          pc1 = mc_corner_load(leaves, el.tleaf, (it >> 10) & 7, (it >> 7) & 7, (it >> 4) & 7, it & 15, s_einfo);
+        }
+      }
       // Edges owned in an earlier tile, one resolved per lane, MC_FL per
       // pass (each lattice edge once per tile).
+      uint16_t* const flist = LIST ? el.flist32 : el.flist;
       const int nf = __popc(fm);
       const int fincl = wave_incl_scan(nf);
       const int ftot = __builtin_amdgcn_readlane(fincl, 63);
       for (int fb = 0; fb < ftot; fb += MC_FL) {
         int k = fincl - nf;
         for (uint32_t m = fm; m; m &= m - 1u, ++k)
-          if (k >= fb && k < fb + MC_FL) el.flist[k - fb] = (uint16_t)((c << 4) | (__ffs(m) - 1));
+          if (k >= fb && k < fb + MC_FL) flist[k - fb] = (uint16_t)((c << 4) | (__ffs(m) - 1));
         wave_lds_sync();
         const int fn = ftot - fb < MC_FL ? ftot - fb : MC_FL;
         for (int k2 = lane; k2 < fn; k2 += 64) {
-          const int it = el.flist[k2], cc = it >> 4, fe = it & 15;
+          const int it = flist[k2], cc = it >> 4, fe = it & 15;
           const int fx = cc >> 6, fy = (cc >> 3) & 7, fz = cc & 7;
           const int slot = fcarry + fb + k2;
           const int es = mc_edge_slot(fx, fy, fz, s_eslot[fe]);
@@ -1202,28 +1228,63 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
       fcarry += ftot;
       if (listed) {
         // (the triangle list shares its LDS with the foreign list above)
-        if (i < n)
-          for (int q2 = 0; q2 < ntr; ++q2) el.tlist[tr + q2] = (uint16_t)(lane | (q2 << 6));
+        if (i < n) {
+          if constexpr (LIST) {
+            const uint64_t row = s_trin[cube];
+            for (int q2 = 0; q2 < ntr; ++q2)
+              el.tlist32[tr + q2] = (uint32_t)c | (((uint32_t)(row >> (12 * q2)) & 0xFFFu) << 9);
+          } else {
+            for (int q2 = 0; q2 < ntr; ++q2) el.tlist[tr + q2] = (uint16_t)(lane | (q2 << 6));
+          }
+        }
         wave_lds_sync();
-        for (int k = lane; k < rv; k += 64) {
-          const int vi = vb + vcarry + k;
-          const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
-          const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
-          el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
-          if (vi < vmax)
-            mc_write_vertex(leaves, bg, el.tleaf, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
+        if constexpr (LIST) {
+          // at most two vertices per lane, with their corner values in pc0, pc1
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const int k = lane + 64 * j;
+            if (k < rv) {
+              const int vi = vb + vcarry + k;
+              const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
+              const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
+              el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
+              if (vi < vmax)
+                mc_vertex_store(j == 0 ? pc0 : pc1, bg, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
+            }
+          }
+        } else {
+          for (int k = lane; k < rv; k += 64) {
+            const int vi = vb + vcarry + k;
+            const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
+            const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
+            el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
+            if (vi < vmax)
+              mc_write_vertex(leaves, bg, el.tleaf, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
+          }
         }
         wave_lds_sync();
         for (int k = lane; k < rt; k += 64) {
           const int ti = tb + tcarry + k;
-          const uint32_t it = el.tlist[k];
-          const int rc = (int)(it & 63u), q2 = (int)(it >> 6);
-          const int cc = el.rlist[rc];
-          const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
-          const uint64_t row = s_trin[el.rcube[rc]] >> (12 * q2);
-          const int i0 = mc_corner_index(ux, uy, uz, (int)(row & 15u), el, s_eslot, vbrel, fc);
-          const int i1 = mc_corner_index(ux, uy, uz, (int)((row >> 4) & 15u), el, s_eslot, vbrel, fc);
-          const int i2 = mc_corner_index(ux, uy, uz, (int)((row >> 8) & 15u), el, s_eslot, vbrel, fc);
+          int ux, uy, uz;
+          uint32_t tri_e;  // the triangle's three edges, 4 bits each
+          if constexpr (LIST) {
+            const uint32_t it = el.tlist32[k];
+            ux = (int)((it >> 6) & 7u);
+            uy = (int)((it >> 3) & 7u);
+            uz = (int)(it & 7u);
+            tri_e = it >> 9;
+          } else {
+            const uint32_t it = el.tlist[k];
+            const int rc = (int)(it & 63u), q2 = (int)(it >> 6);
+            const int cc = el.rlist[rc];
+            ux = cc >> 6;
+            uy = (cc >> 3) & 7;
+            uz = cc & 7;
+            tri_e = (uint32_t)(s_trin[el.rcube[rc]] >> (12 * q2));
+          }
+          const int i0 = mc_corner_index(ux, uy, uz, (int)(tri_e & 15u), el, s_eslot, vbrel, fc);
+          const int i1 = mc_corner_index(ux, uy, uz, (int)((tri_e >> 4) & 15u), el, s_eslot, vbrel, fc);
+          const int i2 = mc_corner_index(ux, uy, uz, (int)((tri_e >> 8) & 15u), el, s_eslot, vbrel, fc);
           if (ti < tmax) {
             uint32_t* o = tris + 3 * (size_t)ti;
             o[0] = (uint32_t)i0;
@@ -1388,8 +1449,9 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int t
   if (vcap < 0) vcap = 0x7fffffffLL;
   if (tricap < 0) tricap = 0x7fffffffLL;
   {
-    // with masks: each leaf's sign mask and its 2048 value bytes (vertices interpolate values throughout the leaf)
-    TimedLaunch tl(ctx, TC_MC_EMIT, masks ? 2048.0 + 64.0 : (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
+    // with masks (cells from the count pass's lists): each leaf's 2048 value bytes (vertices interpolate values
+    // throughout the leaf); the count pass reads the 64-byte sign masks
+    TimedLaunch tl(ctx, TC_MC_EMIT, masks ? 2048.0 : (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
     const void* k = masks ? (const void*)k_mc_emit<true, true> : (const void*)k_mc_emit<false, false>;
     hipLaunchKernelGGL((masks ? k_mc_emit<true, true> : k_mc_emit<false, false>),
                        dim3(mc_blocks(ctx, k, MC_EWAVES, tcap)), dim3(MC_ETHREADS), 0, ctx->stream, d_views,
