@@ -571,48 +571,57 @@ __device__ __forceinline__ void mc_half_bits(const MCHalf& q, float bg, float is
 }
 
 // Sign rows from the grid's sign masks (16 words per leaf), requested one
-// tile ahead: lane l holds the mask words of rows l and l + 64 (< 81).  Row
-// r = x * 9 + y has z = 0..7 in byte y & 3 of word (x & 7) * 2 + (y & 7) / 4
-// of its leaf and z = 8 in the lowest bit of that byte in the +z neighbour;
-// an absent leaf has the background's sign (mc_mask_rows).  tlr: the tile's
-// neighbour leaves, lane d holding entry d (d < 27).  Whole wave.
+// tile ahead in three load instructions: lane l holds the leaf and +z
+// neighbour words of row l, and word w1 holds, in lanes 0..16, the leaf
+// word of row 64 + l and, in lanes 32..48, the +z neighbour word of row
+// 64 + (l - 32) (rows < 81).  Row r = x * 9 + y has z = 0..7 in byte y & 3
+// of word (x & 7) * 2 + (y & 7) / 4 of its leaf and z = 8 in the lowest bit
+// of that byte in the +z neighbour; an absent leaf has the background's
+// sign (mc_mask_rows).  tlr: the tile's neighbour leaves, lane d holding
+// entry d (d < 27).  Whole wave.
 struct MCMaskQ {
-  uint32_t a0, b0, a1, b1;  // row leaf and +z neighbour words of the two rows
-  uint32_t ok;              // bit k: the leaf of word k (a0, b0, a1, b1) is present
+  uint32_t a0, b0, w1;  // row l's leaf and +z words; the words of rows 64.. as above
+  uint32_t ok;          // bit 0: a0's leaf present, 1: b0's, 2: w1's
 };
 __device__ __forceinline__ MCMaskQ mc_mask_issue(const VR_GLOBAL uint32_t* sg, int tlr, int lane) {
   MCMaskQ q;
-  q.ok = 0u;
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int r = k == 0 ? lane : (lane < 81 - 64 ? lane + 64 : 80);
+  {
+    const int r = lane;
     const int x = (r * 57) >> 9, y = r - 9 * x;  // r / 9 for r < 81
     const int d = ((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + 1;
     const int la = __shfl(tlr, d), lb = __shfl(tlr, d + 1);
     const int wi = (x & 7) * 2 + ((y & 7) >> 2);
-    const uint32_t wa = sg[(size_t)(la < 0 ? 0 : la) * 16 + wi];
-    const uint32_t wb = sg[(size_t)(lb < 0 ? 0 : lb) * 16 + wi];
-    if (k == 0) {
-      q.a0 = wa;
-      q.b0 = wb;
-    } else {
-      q.a1 = wa;
-      q.b1 = wb;
-    }
-    q.ok |= ((uint32_t)(la >= 0) | ((uint32_t)(lb >= 0) << 1)) << (2 * k);
+    q.a0 = sg[(size_t)(la < 0 ? 0 : la) * 16 + wi];
+    q.b0 = sg[(size_t)(lb < 0 ? 0 : lb) * 16 + wi];
+    q.ok = (uint32_t)(la >= 0) | ((uint32_t)(lb >= 0) << 1);
+  }
+  {
+    const int hi = lane >= 32, j = (lane & 31) < 81 - 64 ? (lane & 31) : 80 - 64;
+    const int r = 64 + j;
+    const int x = (r * 57) >> 9, y = r - 9 * x;
+    const int d = ((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + 1 + hi;
+    const int l = __shfl(tlr, d);
+    const int wi = (x & 7) * 2 + ((y & 7) >> 2);
+    q.w1 = sg[(size_t)(l < 0 ? 0 : l) * 16 + wi];
+    q.ok |= (uint32_t)(l >= 0) << 2;
   }
   return q;
 }
 // bgs: 0x1FF when the background is below iso, else 0.
 __device__ __forceinline__ void mc_mask_rows(const MCMaskQ& q, uint32_t bgs, int lane, uint32_t* rows) {
+  // row 64 + l's +z word and its presence from lane 32 + l
+  const uint32_t b1 = (uint32_t)__shfl((int)q.w1, (lane + 32) & 63);
+  const uint32_t ok1 = (uint32_t)__shfl((int)q.ok, (lane + 32) & 63);
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int r = lane + 64 * k;
     const int x = (r * 57) >> 9, y = r - 9 * x;
     const int sh = (y & 3) * 8;
-    const uint32_t wa = k == 0 ? q.a0 : q.a1, wb = k == 0 ? q.b0 : q.b1;
-    const uint32_t lo = ((q.ok >> (2 * k)) & 1u) ? (wa >> sh) & 0xFFu : bgs & 0xFFu;
-    const uint32_t hi = ((q.ok >> (2 * k + 1)) & 1u) ? (wb >> sh) & 1u : bgs >> 8;
+    const uint32_t wa = k == 0 ? q.a0 : q.w1, wb = k == 0 ? q.b0 : b1;
+    const bool pa = k == 0 ? (q.ok & 1u) != 0u : (q.ok & 4u) != 0u;
+    const bool pb = k == 0 ? (q.ok & 2u) != 0u : (ok1 & 4u) != 0u;
+    const uint32_t lo = pa ? (wa >> sh) & 0xFFu : bgs & 0xFFu;
+    const uint32_t hi = pb ? (wb >> sh) & 1u : bgs >> 8;
     if (k == 0 || lane < 81 - 64) rows[r] = lo | (hi << 8);
   }
 }
@@ -773,16 +782,14 @@ __device__ __forceinline__ void mc_write_vertex(const VR_GLOBAL NanoLeaf* leaves
 
 // ------------------------------------------------------------------ pass 1
 // Tile t's lists in one vector load per array: lanes < 27 get its neighbour
-// leaves (*l), tiles (*m) and class masks (*k), lane 27 of *l its grid
-// (tiles[t].w).
-__device__ __forceinline__ void mc_count_lists(const int* tleaf, const int* ttile, const uint32_t* cmask,
-                                               const int4* tiles, int t, int lane, int* l, int* m, uint32_t* k) {
+// leaves (*l) and class masks (*k), lane 27 of *l its grid (tiles[t].w).
+// (The count pass needs no neighbour tiles: the class masks carry what it
+// uses of them.)
+__device__ __forceinline__ void mc_count_lists(const int* tleaf, const uint32_t* cmask, const int4* tiles, int t,
+                                               int lane, int* l, uint32_t* k) {
   const int* p = lane < VR_NBR ? tleaf + (size_t)t * VR_NBR + lane : reinterpret_cast<const int*>(tiles) + (size_t)t * 4 + 3;
   if (lane <= VR_NBR) *l = p[0];
-  if (lane < VR_NBR) {
-    *m = ttile[(size_t)t * VR_NBR + lane];
-    *k = cmask[(size_t)t * VR_NBR + lane];
-  }
+  if (lane < VR_NBR) *k = cmask[(size_t)t * VR_NBR + lane];
 }
 
 // Per mixed cell (in visiting order): case, owned crossing edges, triangle
@@ -800,7 +807,7 @@ __device__ __forceinline__ void mc_count_lists(const int* tleaf, const int* ttil
 // mtile[2 t + 1] = the mixed-cell count.
 template <bool MASKS, bool LIST>
 __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, const int4* tiles, const int* ntiles,
-                                                         const int* tleaf, const int* ttile, const uint32_t* cmask,
+                                                         const int* tleaf, const uint32_t* cmask,
                                                          float iso, uint32_t* meta, int* tv, int* tt, int tcap,
                                                          uint32_t* mcell, int* mtile, unsigned long long* mctr,
                                                          long long mcap) {
@@ -828,35 +835,28 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
   // flight, loaded by mc_count_lists; MASKS: so are tile t's sign-mask words
   // (q).  Vector loads throughout: a scalar load of a later tile would hold
   // up every LDS wait of this one.
-  int ntl = -1, ntt = -1, ftl = -1, ftt = -1;
+  int ntl = -1, ftl = -1;
   uint32_t ncm = 0u, fcm = 0u;
-  if (t < t_end) mc_count_lists(tleaf, ttile, cmask, tiles, t, lane, &ntl, &ntt, &ncm);
-  if (t + t_step < t_end) mc_count_lists(tleaf, ttile, cmask, tiles, t + t_step, lane, &ftl, &ftt, &fcm);
+  if (t < t_end) mc_count_lists(tleaf, cmask, tiles, t, lane, &ntl, &ncm);
+  if (t + t_step < t_end) mc_count_lists(tleaf, cmask, tiles, t + t_step, lane, &ftl, &fcm);
   MCMaskQ q{};
   if (MASKS && t < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(ntl, VR_NBR)].signs, ntl, lane);
   for (; t < t_end; t += t_step) {
-    const int tl = ntl, ttv = ntt;
+    const int tl = ntl;
     const uint32_t cm = ncm;
     const int tn = t + t_step, tf = tn + t_step;
     ntl = ftl;
-    ntt = ftt;
     ncm = fcm;
-    if (tf < t_end) mc_count_lists(tleaf, ttile, cmask, tiles, tf, lane, &ftl, &ftt, &fcm);
+    if (tf < t_end) mc_count_lists(tleaf, cmask, tiles, tf, lane, &ftl, &fcm);
     const GridView& g = views[__builtin_amdgcn_readlane(tl, VR_NBR)];
     const MCMaskQ cq = q;
-    const int n = mc_tile_setup<MASKS>(g.leaves, cq, g.background, iso, tl, ttv, cm, w);
+    const int n = mc_tile_setup<MASKS>(g.leaves, cq, g.background, iso, tl, -1, cm, w);
     // the next tile's mask words load while this one is counted
     if (MASKS && tn < t_end) q = mc_mask_issue(views[__builtin_amdgcn_readlane(ntl, VR_NBR)].signs, ntl, lane);
     long long moff = -1;
-    if (LIST) {
-      if (n > 0) {
-        moff = used + n <= share ? share0 + used : -1;
-        used += n;
-      }
-      if (lane == 0) {
-        mtile[2 * (size_t)t] = (int)moff;
-        mtile[2 * (size_t)t + 1] = n;
-      }
+    if (LIST && n > 0) {
+      moff = used + n <= share ? share0 + used : -1;
+      used += n;
     }
     int vcarry = 0, tcarry = 0;
     for (int base = 0; base < n; base += 64) {
@@ -881,9 +881,11 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_count(const GridView* views, 
       vcarry += tot & 0xFFFF;
       tcarry += tot >> 16;
     }
-    if (lane == 0) {
-      tv[t] = vcarry;
-      tt[t] = tcarry;
+    // the tile's totals (and LIST: its list offset and count) in one store
+    // instruction, one lane per array
+    if (lane < (LIST ? 4 : 2)) {
+      int* const dst = lane == 0 ? tv + t : (lane == 1 ? tt + t : mtile + 2 * (size_t)t + (lane - 2));
+      *dst = lane == 0 ? vcarry : (lane == 1 ? tcarry : (lane == 2 ? (int)moff : n));
     }
     wave_lds_sync();
   }
@@ -1429,7 +1431,7 @@ int mc_count_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, long long l
     if (waves_out) *waves_out = (long long)blocks * MC_WAVES;
     hipLaunchKernelGGL((masks ? k_mc_count<true, true> : k_mc_count<false, false>),
                        dim3(blocks), dim3(MC_THREADS), 0, ctx->stream, d_views,
-                       s.tiles.as<int4>(), totals, s.tleaf.as<int>(), s.ttile.as<int>(), s.cmask.as<uint32_t>(), iso,
+                       s.tiles.as<int4>(), totals, s.tleaf.as<int>(), s.cmask.as<uint32_t>(), iso,
                        s.meta.as<uint32_t>(), s.tv.as<int>(), s.tt.as<int>(), tcap, s.mcell.as<uint32_t>(),
                        s.mtile.as<int>(), s.mctr.as<unsigned long long>(), mcap);
   }
