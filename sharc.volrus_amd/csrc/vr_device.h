@@ -101,12 +101,15 @@ __device__ __forceinline__ bool grid_is_active(const GridView& g, int x, int y, 
   return (g.leaves[li].mask[off >> 6] >> (off & 63)) & 1ull;
 }
 
-// Rust `f64 as i32`: saturating, NaN -> 0.
-// Branch-free: the clamp keeps the conversion in range (a NaN clamps to
-// -2^31 and is then replaced by 0).
+// Rust `f64 as i32`: saturating, NaN -> 0.  The hardware conversion
+// v_cvt_i32_f64 does exactly that (out-of-range values and infinities
+// saturate, NaN gives 0), in one instruction instead of a clamp, a
+// conversion and a NaN select; C++'s (int) cast leaves out-of-range values
+// undefined, so the instruction is named here.
 __device__ __forceinline__ int f64_to_i32(double v) {
-  const int t = (int)fmin(fmax(v, -2147483648.0), 2147483647.0);
-  return v != v ? 0 : t;
+  int r;
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(v));
+  return r;
 }
 
 // The 7-lerp blend of src/tools/interpolation.rs:39-47, in f32, in that order.

@@ -122,6 +122,22 @@ def test_advect_dense_velocity_large_dt(orc, ctx):
             assert_nano_equal(got, want, f"dense r={r} dt={dt} scheme={scheme}")
 
 
+def test_advect_departure_saturates_to_int_min(orc, ctx):
+    """Departure points far below the i32 range: Rust's saturating `f64 as i32`
+    (interpolation.rs:33-35) puts the cell at x = i32::MIN, where values were planted."""
+    announce("advect saturating departure")
+    g = orc.sphere(3.0, (0.0, 0.0, 0.0), 1.0, 3.0)  # voxel size 1: index = world
+    for y in range(-8, 9):
+        for z in range(-8, 9):
+            g.set(-2 ** 31, y, z, 0.25 * y - 0.125 * z + 0.5)
+    params = (1e14, 0.0, 0.0)
+    want = orc.advect_analytic(g, 2, params, 1.0).to_nano()
+    got = ctx.upload(g.to_nano()).advect(ctx.velocity_analytic(2, params), 1.0).download()
+    assert_nano_equal(got, want, "saturating departure")
+    # voxel (0, 1, 0) of the band sampled the planted plane
+    assert orc.from_nano(got).get(0, 1, 0) == 0.75
+
+
 def test_advect_input_with_empty_leaf(orc, ctx):
     announce("advect empty leaf")
     g = orc.sphere(5.0, (0, 0, 0), 1.0, 3.0)
