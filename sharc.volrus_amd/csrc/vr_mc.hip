@@ -302,20 +302,42 @@ __global__ __launch_bounds__(256) void k_mc_scatter_tiles(const GridView* views,
 // and looks up two of the eight leaf slots; d = 12 (0, 0, -1) can only be
 // the tile just before.  The slots are gathered in LDS and written out
 // tile by tile.
-// One thread per slot: the lower-latency form for small tile lists (no
-// block barriers; each slot's lookup runs on its own).
-__global__ void k_mc_tile_nbr_slots(const GridView* views, const int4* tiles, const int* tanchor, const int* ntiles,
-                                    const int* tile_base, int* tleaf, int* ttile) {
-  const int nt = *ntiles;
-  const size_t total = (size_t)nt * VR_NBR;
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-    const int t = (int)(q / VR_NBR), d = (int)(q % VR_NBR);
+// Both forms also write, per tile and cell class (27: per axis local 0 /
+// 1..6 / 7), the edges a cell of that class owns itself (no candidate
+// ordered before it lies in a processed tile) in bits 0..11, and in bits
+// 16..27 the edges owned in an earlier tile that such a cell resolves for
+// its tile (a candidate ordered before the tile's first one is processed,
+// and no candidate of the tile precedes the cell).  They depend only on
+// which neighbour tiles are processed, so the count and emit passes load
+// them instead of deriving them per tile; candidates ordered before a cell
+// lie in tiles d <= 13, the tile itself and those before it.  tt(d): the
+// tile's neighbour tile at slot d (-1: not processed).
+template <class TT>
+__device__ __forceinline__ uint32_t mc_class_mask(int c, TT tt) {
+  uint32_t processed = 0u;
+#pragma unroll
+  for (int d = 0; d <= 13; ++d) processed |= (uint32_t)(tt(d) >= 0) << d;
+  uint32_t self = 0u, fitem = 0u;
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const uint32_t b = c_mc_before[c * 12 + e], fb = c_mc_fbefore[c * 12 + e];
+    self |= (uint32_t)((b & processed) == 0u) << e;
+    fitem |= (uint32_t)((fb & processed) != 0u && ((b >> 13) & 1u) == 0u) << e;
+  }
+  return self | (fitem << 16);
+}
+
+// Slot d of tile t (k_mc_tile_nbr_slots): writes tleaf and ttile, returns the ttile entry.
+__device__ __forceinline__ int mc_tile_slot(const GridView* views, const int4* tiles, const int* tanchor, const int* tile_base, int t,
+                            int d, int* tleaf, int* ttile) {
+  const size_t q = (size_t)t * VR_NBR + d;
+  {
     const int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
     const bool want_leaf = dx >= 0 && dy >= 0 && dz >= 0, want_tile = d <= 13;
     if (!want_leaf && !want_tile) {
       tleaf[q] = -1;
       ttile[q] = -1;
-      continue;
+      return -1;
     }
     const int4 o = tiles[t];
     const int g = o.w;
@@ -340,6 +362,27 @@ __global__ void k_mc_tile_nbr_slots(const GridView* views, const int4* tiles, co
       tt = (lb < hi && tiles[lb].x == x && tiles[lb].y == y && tiles[lb].z == z) ? lb : -1;
     }
     ttile[q] = tt;
+    return tt;
+  }
+}
+
+// One thread per slot: the lower-latency form for small tile lists (no
+// search barriers; each slot's lookup runs on its own).  A block of 9 x 27
+// threads takes 9 tiles at a time, so a tile's slots meet in LDS for its
+// class masks.
+#define MC_TNS_TILES 9
+__global__ __launch_bounds__(MC_TNS_TILES * VR_NBR) void k_mc_tile_nbr_slots(
+    const GridView* views, const int4* tiles, const int* tanchor, const int* ntiles, const int* tile_base, int* tleaf,
+    int* ttile, uint32_t* cmask) {
+  __shared__ int s_tt[MC_TNS_TILES * VR_NBR];
+  const int nt = *ntiles;
+  const int lt = (int)threadIdx.x / VR_NBR, d = (int)threadIdx.x % VR_NBR;
+  for (int t0 = blockIdx.x * MC_TNS_TILES; t0 < nt; t0 += gridDim.x * MC_TNS_TILES) {
+    const int t = t0 + lt;
+    if (t < nt) s_tt[threadIdx.x] = mc_tile_slot(views, tiles, tanchor, tile_base, t, d, tleaf, ttile);
+    __syncthreads();
+    if (t < nt) cmask[(size_t)t * VR_NBR + d] = mc_class_mask(d, [&](int dd) { return s_tt[lt * VR_NBR + dd]; });
+    __syncthreads();
   }
 }
 
@@ -347,7 +390,8 @@ __global__ void k_mc_tile_nbr_slots(const GridView* views, const int4* tiles, co
 #define MC_TN_BIG 32768  // tile capacity from which the tile-per-lane form is used
 __constant__ int c_tn_leaf[8] = {13, 14, 16, 17, 22, 23, 25, 26};
 __global__ __launch_bounds__(256) void k_mc_tile_nbr(const GridView* views, const int4* tiles, const int* tanchor,
-                                                     const int* ntiles, const int* tile_base, int* tleaf, int* ttile) {
+                                                     const int* ntiles, const int* tile_base, int* tleaf, int* ttile,
+                                                     uint32_t* cmask) {
   __shared__ int s_tl[MC_TN_TILES * VR_NBR], s_tt[MC_TN_TILES * VR_NBR];
   const int nt = *ntiles;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -419,35 +463,10 @@ __global__ __launch_bounds__(256) void k_mc_tile_nbr(const GridView* views, cons
     for (int i = threadIdx.x; i < cnt; i += 256) {
       tleaf[(size_t)t0 * VR_NBR + i] = s_tl[i];
       ttile[(size_t)t0 * VR_NBR + i] = s_tt[i];
+      const int* tt = s_tt + (i / VR_NBR) * VR_NBR;
+      cmask[(size_t)t0 * VR_NBR + i] = mc_class_mask(i % VR_NBR, [tt](int dd) { return tt[dd]; });
     }
     __syncthreads();
-  }
-}
-
-// Per tile and cell class (27: per axis local 0 / 1..6 / 7), the edges a
-// cell of that class owns itself (no candidate ordered before it lies in a
-// processed tile) in bits 0..11, and in bits 16..27 the edges owned in an
-// earlier tile that such a cell resolves for its tile (a candidate ordered
-// before the tile's first one is processed, and no candidate of the tile
-// precedes the cell).  They depend only on which neighbour tiles are
-// processed, so they are built once with the tile lists; candidates ordered
-// before a cell lie in tiles d <= 13, the tile itself and those before it.
-__global__ void k_mc_class_masks(const int* ttile, const int* ntiles, uint32_t* cmask) {
-  const size_t total = (size_t)*ntiles * 27;
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-    const size_t t = q / 27;
-    const int c = (int)(q % 27);
-    uint32_t processed = 0u;
-#pragma unroll
-    for (int d = 0; d <= 13; ++d) processed |= (uint32_t)(ttile[t * VR_NBR + d] >= 0) << d;
-    uint32_t self = 0u, fitem = 0u;
-#pragma unroll
-    for (int e = 0; e < 12; ++e) {
-      const uint32_t b = c_mc_before[c * 12 + e], fb = c_mc_fbefore[c * 12 + e];
-      self |= (uint32_t)((b & processed) == 0u) << e;
-      fitem |= (uint32_t)((fb & processed) != 0u && ((b >> 13) & 1u) == 0u) << e;
-    }
-    cmask[q] = self | (fitem << 16);
   }
 }
 
@@ -626,7 +645,7 @@ __device__ __forceinline__ void mc_mask_rows(const MCMaskQ& q, uint32_t bgs, int
   }
 }
 
-// Loads the tile's neighbour lists and class masks (k_mc_class_masks;
+// Loads the tile's neighbour lists and class masks (mc_class_mask;
 // fetched into lanes < 27) into LDS, stages its sign rows (MASKS: from the
 // mask words q, else from the values).  Returns the lane's mixed cells
 // (corners on both sides of iso: a case other than 0 or 255): bit z of lane
@@ -1400,13 +1419,11 @@ int mc_tiles_finish(vr_ctx* ctx, MCMulti& s, const GridView* d_views, const int*
   if (tcap >= MC_TN_BIG)
     hipLaunchKernelGGL(k_mc_tile_nbr, dim3(blocks_for((size_t)tcap, MC_TN_TILES)), dim3(256), 0, ctx->stream, d_views,
                        s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(), s.tleaf.as<int>(),
-                       s.ttile.as<int>());
+                       s.ttile.as<int>(), s.cmask.as<uint32_t>());
   else
-    hipLaunchKernelGGL(k_mc_tile_nbr_slots, dim3(blocks_for((size_t)tcap * VR_NBR, 256)), dim3(256), 0, ctx->stream,
-                       d_views, s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(),
-                       s.tleaf.as<int>(), s.ttile.as<int>());
-  hipLaunchKernelGGL(k_mc_class_masks, dim3(blocks_for((size_t)tcap * VR_NBR, 256)), dim3(256), 0, ctx->stream,
-                     s.ttile.as<int>(), totals, s.cmask.as<uint32_t>());
+    hipLaunchKernelGGL(k_mc_tile_nbr_slots, dim3(blocks_for((size_t)tcap, MC_TNS_TILES)), dim3(MC_TNS_TILES * VR_NBR), 0,
+                       ctx->stream, d_views, s.tiles.as<int4>(), s.tanchor.as<int>(), totals, s.tile_base.as<int>(),
+                       s.tleaf.as<int>(), s.ttile.as<int>(), s.cmask.as<uint32_t>());
   return (e = hipGetLastError()) == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "mc tiles");
 }
 
