@@ -115,22 +115,38 @@ __device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* 
 // persistent != 0: `gridDim.x` blocks loop, taking chunks from their XCD's
 // counter.  persistent == 0: one chunk per block, block b taking chunk
 // b / 8 of XCD b % 8's range (the grid covers 8 x the longest range).  ONE:
-// one chunk per block and chunks of one leaf.
+// one chunk per block and chunks of one leaf, so chunk c is global leaf c:
+// its grid is found by a search of leaf_base (G + 1 entries, in the scalar
+// cache) instead of a load from the chunk table, which would start each
+// block's chain of dependent loads with a miss.
 template <int VK, int SCHEME, int NW, bool ONE>
 __global__ __launch_bounds__(64 * NW) void k_batch_advect(const GridView* ins, const VelView* vels,
                                                           NanoLeaf* const* outs, const int4* chunks, int* counters,
                                                           const int* ranges, int persistent, const double* dts,
                                                           double dt, double tf0, double tf1, double tf2,
-                                                          uint32_t* signs, const int* leaf_base, float iso) {
+                                                          uint32_t* signs, const int* leaf_base, int G, float iso) {
   __shared__ AdvectLDS<VK> s;
   __shared__ int s_item;
   const int tid = threadIdx.x;
   const int xcd = blockIdx.x & 7;
   const int lo = ranges[xcd], hi = ranges[xcd + 1];
   const double tf[3] = {tf0, tf1, tf2};
-  if (ONE || !persistent) {
+  if (ONE) {
     const int c = lo + (int)(blockIdx.x >> 3);
-    if (c < hi) batch_chunk<VK, SCHEME, NW, ONE>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf, signs, leaf_base, iso);
+    if (c < hi) {
+      int g0 = 0, g1 = G;  // leaf_base[g0] <= c < leaf_base[g1]
+      while (g1 - g0 > 1) {
+        const int mid = (g0 + g1) >> 1;
+        if (leaf_base[mid] <= c) g0 = mid; else g1 = mid;
+      }
+      batch_chunk<VK, SCHEME, NW, true>(ins, vels, outs, make_int4(g0, c - leaf_base[g0], 1, 0), s, tid, dts, dt, tf,
+                                        signs, leaf_base, iso);
+    }
+    return;
+  }
+  if (!persistent) {
+    const int c = lo + (int)(blockIdx.x >> 3);
+    if (c < hi) batch_chunk<VK, SCHEME, NW, false>(ins, vels, outs, chunks[c], s, tid, dts, dt, tf, signs, leaf_base, iso);
     return;
   }
   // Chunk indices read from LDS are block-uniform; readfirstlane tells the
@@ -155,28 +171,28 @@ template <int VK, int SCHEME, int NW, bool ONE>
 static void launch_batch_k(int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
                            NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges, int persistent,
                            const double* dts, double dt, const double tf[3], uint32_t* signs,
-                           const int* leaf_base, float iso) {
+                           const int* leaf_base, int G, float iso) {
   hipLaunchKernelGGL((k_batch_advect<VK, SCHEME, NW, ONE>), dim3(blocks), dim3(64 * NW), 0, st, ins, vels, outs,
-                     chunks, counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2], signs, leaf_base, iso);
+                     chunks, counters, ranges, persistent, dts, dt, tf[0], tf[1], tf[2], signs, leaf_base, G, iso);
 }
 
 template <int VK, int SCHEME>
 static void launch_batch_t(int nw, bool one, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
                            NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges, int persistent,
                            const double* dts, double dt, const double tf[3], uint32_t* signs,
-                           const int* leaf_base, float iso) {
+                           const int* leaf_base, int G, float iso) {
   if (nw == 1) {
     if (one)
-      launch_batch_k<VK, SCHEME, 1, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, iso);
+      launch_batch_k<VK, SCHEME, 1, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, G, iso);
     else
       launch_batch_k<VK, SCHEME, 1, false>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt,
-                                           tf, signs, leaf_base, iso);
+                                           tf, signs, leaf_base, G, iso);
   } else {
     if (one)
-      launch_batch_k<VK, SCHEME, 4, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, iso);
+      launch_batch_k<VK, SCHEME, 4, true>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, G, iso);
     else
       launch_batch_k<VK, SCHEME, 4, false>(blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt,
-                                           tf, signs, leaf_base, iso);
+                                           tf, signs, leaf_base, G, iso);
   }
 }
 
@@ -184,9 +200,9 @@ template <int SCHEME>
 static void launch_batch_s(int vk, int nw, bool one, int blocks, hipStream_t st, const GridView* ins, const VelView* vels,
                            NanoLeaf* const* outs, const int4* chunks, int* counters, const int* ranges,
                            int persistent, const double* dts, double dt, const double tf[3], uint32_t* signs,
-                           const int* leaf_base, float iso) {
+                           const int* leaf_base, int G, float iso) {
 #define VR_BK(K) \
-  launch_batch_t<K, SCHEME>(nw, one, blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, iso)
+  launch_batch_t<K, SCHEME>(nw, one, blocks, st, ins, vels, outs, chunks, counters, ranges, persistent, dts, dt, tf, signs, leaf_base, G, iso)
   switch (vk) {
     case VK_UNIFORM: VR_BK(VK_UNIFORM); break;
     case VK_RIGID: VR_BK(VK_RIGID); break;
@@ -361,11 +377,11 @@ static int batch_advect(vr_ctx* ctx, vr_batch* b, double dt, double t, int schem
     if (scheme == VR_SCHEME_RK3)
       launch_batch_s<3>(b->vk, nw, one, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
                         b->d_outs.as<NanoLeaf*>(), b->d_chunks.as<int4>(), b->d_counters.as<int>(),
-                        b->d_ranges.as<int>(), persistent, dts, dt, tf, signs, leaf_base, iso);
+                        b->d_ranges.as<int>(), persistent, dts, dt, tf, signs, leaf_base, b->G, iso);
     else
       launch_batch_s<1>(b->vk, nw, one, blocks, ctx->stream, b->d_in.as<GridView>(), b->d_vel.as<VelView>(),
                         b->d_outs.as<NanoLeaf*>(), b->d_chunks.as<int4>(), b->d_counters.as<int>(),
-                        b->d_ranges.as<int>(), persistent, dts, dt, tf, signs, leaf_base, iso);
+                        b->d_ranges.as<int>(), persistent, dts, dt, tf, signs, leaf_base, b->G, iso);
   }
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? VR_OK : vr_hip_fail(ctx, e, "batch advect");
