@@ -34,8 +34,10 @@ __constant__ uint64_t c_mc_trin[256];
 // contains the same lattice edge and that edge's local number there, packed
 // as 4 int8 (dx, dy, dz, e').
 __constant__ uint32_t c_mc_cand4[12 * 4];
-// Per local edge: corner a | corner b << 3 | a's offset bits << 6 | b's << 9
-// (offset bits x | y << 1 | z << 2), the edge's reference orientation.
+// Per local edge, the edge's reference orientation: corner a's offset from
+// the cell as packed fields (x << 8 | y << 4 | z, each 0 or 1) | corner b's
+// << 16.  Added to a packed cell (mc_pack) they give the corners' packed
+// lattice points (each coordinate 0..8).
 __constant__ uint32_t c_mc_edgeinfo[12];
 // Candidate of each local edge that is the cell itself.
 __constant__ int c_mc_self[12];
@@ -79,7 +81,8 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
     const int ca = edge[2 * e], cb = edge[2 * e + 1];
     const uint32_t ab = (uint32_t)(corner[3 * ca] | (corner[3 * ca + 1] << 1) | (corner[3 * ca + 2] << 2));
     const uint32_t bb = (uint32_t)(corner[3 * cb] | (corner[3 * cb + 1] << 1) | (corner[3 * cb + 2] << 2));
-    einfo[e] = (uint32_t)ca | ((uint32_t)cb << 3) | (ab << 6) | (bb << 9);
+    const auto pack = [](uint32_t o) { return ((o & 1u) << 8) | (((o >> 1) & 1u) << 4) | ((o >> 2) & 1u); };
+    einfo[e] = pack(ab) | (pack(bb) << 16);
     const uint32_t lo = ab & bb, ax = ab ^ bb;  // lower endpoint offset bits, axis bit
     const uint32_t lx = lo & 1u, ly = (lo >> 1) & 1u, lz = lo >> 2;
     uint32_t k0, mx, my;
@@ -746,17 +749,24 @@ struct MCCorner {
   float va, vb;
   int ok;  // bit 0: corner a's leaf present, bit 1: corner b's
 };
+// A tile lattice point (x, y, z in 0..8) as packed fields x << 8 | y << 4 | z.
+__device__ __forceinline__ uint32_t mc_pack(int x, int y, int z) { return ((uint32_t)x << 8) | ((uint32_t)y << 4) | (uint32_t)z; }
+// Its leaf: the tile's own (slot 13) or the +x / +y / +z neighbour once that
+// coordinate is 8 (bit 3 of the field).
+__device__ __forceinline__ int mc_slot9(uint32_t s) {
+  return 13 + 9 * (int)((s >> 11) & 1u) + 3 * (int)((s >> 7) & 1u) + (int)((s >> 3) & 1u);
+}
+// Its offset in that leaf (x & 7) * 64 + (y & 7) * 8 + (z & 7).
+__device__ __forceinline__ int mc_off9(uint32_t s) { return (int)(((s >> 2) & 0x1C0u) | ((s >> 1) & 0x38u) | (s & 7u)); }
+
 __device__ __forceinline__ MCCorner mc_corner_load(const VR_GLOBAL NanoLeaf* leaves, const int* tl, int x, int y, int z,
                                                    int e, const uint32_t* s_einfo) {
-  const uint32_t ei = s_einfo[e];
-  const int ao = (int)((ei >> 6) & 7u), bo = (int)((ei >> 9) & 7u);
-  const int ax = x + (ao & 1), ay = y + ((ao >> 1) & 1), az = z + (ao >> 2);
-  const int bx = x + (bo & 1), by = y + ((bo >> 1) & 1), bz = z + (bo >> 2);
-  const int la = tl[((ax >> 3) + 1) * 9 + ((ay >> 3) + 1) * 3 + ((az >> 3) + 1)];
-  const int lb = tl[((bx >> 3) + 1) * 9 + ((by >> 3) + 1) * 3 + ((bz >> 3) + 1)];
+  const uint32_t ei = s_einfo[e], p = mc_pack(x, y, z);
+  const uint32_t sa = p + (ei & 0xFFFFu), sb = p + (ei >> 16);
+  const int la = tl[mc_slot9(sa)], lb = tl[mc_slot9(sb)];
   MCCorner c;
-  c.va = leaves[la < 0 ? 0 : la].values[voxel_offset(ax, ay, az)];
-  c.vb = leaves[lb < 0 ? 0 : lb].values[voxel_offset(bx, by, bz)];
+  c.va = leaves[la < 0 ? 0 : la].values[mc_off9(sa)];
+  c.vb = leaves[lb < 0 ? 0 : lb].values[mc_off9(sb)];
   c.ok = (int)(la >= 0) | ((int)(lb >= 0) << 1);
   return c;
 }
@@ -770,10 +780,9 @@ __device__ __forceinline__ MCCorner mc_corner_load(const VR_GLOBAL NanoLeaf* lea
 __device__ __forceinline__ void mc_vertex_store(const MCCorner& cv, float bg, int x, int y, int z, int e,
                                                 const int4& to, double vs, float iso, const uint32_t* s_einfo,
                                                 double* out) {
-  const uint32_t ei = s_einfo[e];
-  const int ao = (int)((ei >> 6) & 7u), bo = (int)((ei >> 9) & 7u);
-  const int ax = x + (ao & 1), ay = y + ((ao >> 1) & 1), az = z + (ao >> 2);
-  const int bx = x + (bo & 1), by = y + ((bo >> 1) & 1), bz = z + (bo >> 2);
+  const uint32_t ei = s_einfo[e], pk = mc_pack(x, y, z);
+  const uint32_t sa = pk + (ei & 0xFFFFu), sb = pk + (ei >> 16);
+  const int ax = (int)((sa >> 8) & 15u), ay = (int)((sa >> 4) & 15u), az = (int)(sa & 15u);
   const float va = (cv.ok & 1) ? cv.va : bg;
   const float vbv = (cv.ok & 2) ? cv.vb : bg;
   double tpar;
@@ -781,11 +790,11 @@ __device__ __forceinline__ void mc_vertex_store(const MCCorner& cv, float bg, in
     tpar = (double)((iso - va) / (vbv - va));
   else
     tpar = 0.5;
-  const int span = ao ^ bo;  // 1, 2 or 4: the axis of the edge
+  const uint32_t span = (ei & 0xFFFFu) ^ (ei >> 16);  // 0x100, 0x10 or 1: the axis of the edge
   const double pax = (double)(to.x + ax) * vs, pay = (double)(to.y + ay) * vs, paz = (double)(to.z + az) * vs;
   // pa on the spanned axis is that axis's corner-a coordinate
-  const bool sx = span == 1, sy = span == 2, sz = span == 4;
-  const int ib = (sx ? to.x + bx : 0) + (sy ? to.y + by : 0) + (sz ? to.z + bz : 0);
+  const bool sx = span == 0x100u, sy = span == 0x10u, sz = span == 1u;
+  const int ib = sx ? to.x + (int)((sb >> 8) & 15u) : (sy ? to.y + (int)((sb >> 4) & 15u) : to.z + (int)(sb & 15u));
   const double pa = sx ? pax : (sy ? pay : paz), pb = (double)ib * vs;
   const double p = pa + tpar * (pb - pa);
   out[0] = sx ? p : pax;
