@@ -1067,6 +1067,44 @@ __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const
 // of the next tile load while the current one is processed.  A tile whose
 // list did not fit (offset -1) writes nothing: that happens only when the
 // mesh is larger than the capacity the lists were sized for.
+// Edges owned in an earlier tile (fm0 of cell c0, then fm1 of cell c1, per
+// lane), one resolved per lane, MC_FL per pass: into the side table from
+// slot *fcarry on (its edge slot holding 0x8000 | the entry), or past the
+// side table marked 0xFFFF and resolved again where a triangle needs it.
+// Each lattice edge is listed once per tile (by the first cell of the tile
+// containing it).  Whole wave.
+__device__ __forceinline__ void mc_foreign_pass(uint32_t fm0, int c0, uint32_t fm1, int c1, int lane, uint16_t* flist,
+                                                int* fcarry, MCEmitLDS& el, const uint32_t* s_eslot,
+                                                const MCForeign& fc) {
+  const int nf = __popc(fm0) + __popc(fm1);
+  const int fincl = wave_incl_scan(nf);
+  const int ftot = __builtin_amdgcn_readlane(fincl, 63);
+  for (int fb = 0; fb < ftot; fb += MC_FL) {
+    int k = fincl - nf;
+    for (uint32_t m = fm0; m; m &= m - 1u, ++k)
+      if (k >= fb && k < fb + MC_FL) flist[k - fb] = (uint16_t)((c0 << 4) | (__ffs(m) - 1));
+    for (uint32_t m = fm1; m; m &= m - 1u, ++k)
+      if (k >= fb && k < fb + MC_FL) flist[k - fb] = (uint16_t)((c1 << 4) | (__ffs(m) - 1));
+    wave_lds_sync();
+    const int fn = ftot - fb < MC_FL ? ftot - fb : MC_FL;
+    for (int k2 = lane; k2 < fn; k2 += 64) {
+      const int it = flist[k2], cc = it >> 4, fe = it & 15;
+      const int fx = cc >> 6, fy = (cc >> 3) & 7, fz = cc & 7;
+      const int slot = *fcarry + fb + k2;
+      const int es = mc_edge_slot(fx, fy, fz, s_eslot[fe]);
+      if (slot < MC_FSIDE) {
+        el.foreign[slot] =
+            mc_foreign_index(fx, fy, fz, fe, fc.processed, fc.s_ownsort, fc.s_cand, fc.ttile, fc.meta, fc.vbase, fc.vrel);
+        el.edge[es] = (uint16_t)(0x8000u | (uint32_t)slot);
+      } else {
+        el.edge[es] = 0xFFFFu;
+      }
+    }
+    wave_lds_sync();
+  }
+  *fcarry += ftot;
+}
+
 template <bool MASKS, bool LIST>
 __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_mc_emit(
     const GridView* views, const int4* tiles, const int* ntiles, const int* tleaf, const int* ttile,
@@ -1176,6 +1214,17 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     }
     const MCForeign fc{processed, s_ownsort, s_cand, el.ttile, meta, vbase, vrel};
     int vcarry = 0, tcarry = 0, fcarry = 0;
+    if constexpr (LIST) {
+      // The earlier-tile edges of the first two rounds' cells (entries c0,
+      // c1) in one pass at the start of the tile, so that its loads of the
+      // earlier tiles' face meta wait once per tile; later rounds resolve
+      // their own.
+      wave_lds_sync();
+      uint32_t fm0 = 0u, fm1 = 0u;
+      if (lane < n) fm0 = (uint32_t)(s_emn[c0 >> 9] & 0xFFFu) & el.fitem[s_ccls[c0 & 511u]];
+      if (64 + lane < n) fm1 = (uint32_t)(s_emn[c1 >> 9] & 0xFFFu) & el.fitem[s_ccls[c1 & 511u]];
+      mc_foreign_pass(fm0, (int)(c0 & 511u), fm1, (int)(c1 & 511u), lane, el.flist32, &fcarry, el, s_eslot, fc);
+    }
     for (int base = 0; base < n; base += 64) {
       const int i = base + lane;
       uint32_t ent = 0u;
@@ -1228,34 +1277,9 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
           pc1 = mc_corner_load(leaves, el.tleaf, (it >> 10) & 7, (it >> 7) & 7, (it >> 4) & 7, it & 15, s_einfo);
         }
       }
-      // Edges owned in an earlier tile, one resolved per lane, MC_FL per
-      // pass (each lattice edge once per tile).
-      uint16_t* const flist = LIST ? el.flist32 : el.flist;
-      const int nf = __popc(fm);
-      const int fincl = wave_incl_scan(nf);
-      const int ftot = __builtin_amdgcn_readlane(fincl, 63);
-      for (int fb = 0; fb < ftot; fb += MC_FL) {
-        int k = fincl - nf;
-        for (uint32_t m = fm; m; m &= m - 1u, ++k)
-          if (k >= fb && k < fb + MC_FL) flist[k - fb] = (uint16_t)((c << 4) | (__ffs(m) - 1));
-        wave_lds_sync();
-        const int fn = ftot - fb < MC_FL ? ftot - fb : MC_FL;
-        for (int k2 = lane; k2 < fn; k2 += 64) {
-          const int it = flist[k2], cc = it >> 4, fe = it & 15;
-          const int fx = cc >> 6, fy = (cc >> 3) & 7, fz = cc & 7;
-          const int slot = fcarry + fb + k2;
-          const int es = mc_edge_slot(fx, fy, fz, s_eslot[fe]);
-          if (slot < MC_FSIDE) {
-            el.foreign[slot] =
-                mc_foreign_index(fx, fy, fz, fe, processed, s_ownsort, s_cand, el.ttile, meta, vbase, vrel);
-            el.edge[es] = (uint16_t)(0x8000u | (uint32_t)slot);
-          } else {
-            el.edge[es] = 0xFFFFu;  // resolved again where a triangle needs it
-          }
-        }
-        wave_lds_sync();
-      }
-      fcarry += ftot;
+      // Edges owned in an earlier tile (LIST: those of the first 128 cells
+      // were resolved at the start of the tile).
+      if (!LIST || base >= 128) mc_foreign_pass(fm, c, 0u, 0, lane, LIST ? el.flist32 : el.flist, &fcarry, el, s_eslot, fc);
       if (listed) {
         // (the triangle list shares its LDS with the foreign list above)
         if (i < n) {
