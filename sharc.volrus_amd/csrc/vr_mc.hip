@@ -941,6 +941,11 @@ __device__ __forceinline__ void mc_ranges(const int* tile_base, int G, int nt, c
   }
 }
 
+__global__ void k_mc_ranges(const int* tile_base, int G, const int* ntiles, const int* vbase, const int* tbase,
+                            const int* totals, int* out, int* counts_out) {
+  mc_ranges(tile_base, G, *ntiles, vbase, tbase, totals, out, counts_out);
+}
+
 // ------------------------------------------------------------------ pass 2
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
@@ -1134,7 +1139,9 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
   __syncthreads();
   MCEmitLDS& el = s_e[wv];
   const int nt = *ntiles;
-  if (blockIdx.x == 0) mc_ranges(tile_base, G, nt, vbase, tbase, totals, ranges, counts_out);
+  // LIST: the ranges come from k_mc_ranges before this launch, so a tile's
+  // first grid vertex is a scalar-cache read of ranges[4 g]
+  if (!LIST && blockIdx.x == 0) mc_ranges(tile_base, G, nt, vbase, tbase, totals, ranges, counts_out);
   // writes stop at the counts or at the output capacity, whichever is lower
   const int vmax = (int)min((long long)totals[1], vcap), tmax = (int)min((long long)totals[2], tricap);
   int t_end, t_step;
@@ -1189,8 +1196,13 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     const VR_GLOBAL NanoLeaf* leaves = g.leaves;
     const float bg = g.background;
     const double vs = g.vs;
-    const int g0 = tile_base[to.w];
-    const int vrel = g0 < nt ? vbase[g0] : totals[1];  // first vertex of this grid
+    int vrel;  // first vertex of this grid
+    if constexpr (LIST) {
+      vrel = ranges[4 * to.w];
+    } else {
+      const int g0 = tile_base[to.w];
+      vrel = g0 < nt ? vbase[g0] : totals[1];
+    }
     const int vbrel = vb - vrel;
     uint32_t rem = 0u;  // !LIST: cells not yet listed (mc_round_list)
     int rk = 0, n;
@@ -1504,6 +1516,9 @@ int mc_emit_multi(vr_ctx* ctx, MCMulti& s, const GridView* d_views, int G, int t
     // with masks (cells from the count pass's lists): each leaf's 2048 value bytes (vertices interpolate values
     // throughout the leaf); the count pass reads the 64-byte sign masks
     TimedLaunch tl(ctx, TC_MC_EMIT, masks ? 2048.0 : (double)VR_LEAF_BYTES, mesh_bytes, leaf_count, true);
+    if (masks)
+      hipLaunchKernelGGL(k_mc_ranges, dim3(1), dim3(256), 0, ctx->stream, s.tile_base.as<int>(), G, s.totals.as<int>(),
+                         s.vb.as<int>(), s.tb.as<int>(), s.totals.as<int>(), s.ranges.as<int>(), counts_out);
     const void* k = masks ? (const void*)k_mc_emit<true, true> : (const void*)k_mc_emit<false, false>;
     hipLaunchKernelGGL((masks ? k_mc_emit<true, true> : k_mc_emit<false, false>),
                        dim3(mc_blocks(ctx, k, MC_EWAVES, tcap)), dim3(MC_ETHREADS), 0, ctx->stream, d_views,
