@@ -203,12 +203,11 @@ using AdvectLDS = AdvectLDST<advect_tab(VK)>;
 // Advects leaf li of `in` into out[li] with NW waves (the whole block,
 // tid in [0, 64 NW)): the active voxels are listed by a ballot/popcount
 // prefix over the 8 mask words and spread densely over the block's lanes
-// (one pass for a band leaf with 4 waves); each result is stored straight
-// into the output leaf, whose inactive positions get the background.
+// (one pass for a band leaf with 4 waves); the results and, at the
+// inactive positions, the background are gathered in the brick's interior
+// and leave as the whole output leaf in 16-byte stores.
 // Output origin and mask are the input's (the topology does not change,
-// src/tools/advection.rs:34-53).  The background stores follow the
-// results, so each output cache line is written within a short window and
-// leaves the L2 once.  SIGNS: also writes the leaf's sign mask for `iso`
+// src/tools/advection.rs:34-53).  SIGNS: also writes the leaf's sign mask for `iso`
 // to sg[0..15] (bit (o & 31) of word o >> 5 = value at offset o < iso,
 // background at the inactive offsets), gathered with LDS atomics as the
 // results come, so marching cubes over the output can classify cells
@@ -277,16 +276,47 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   if (tid == 3) O.pad = 0;
   stage_halo_reg<NT>(in, nbv, s.brick, tid);
   __syncthreads();
-  for (int i = tid; i < total; i += NT) {
-    const int off = s.list[i];
-    const float r = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt,
-                                             tf, tab, s.vtab);
-    O.values[off] = r;
-    if (SIGNS && r < iso) atomicOr(&s.sgn[off >> 5], 1u << (off & 31));
+  // Results stay in registers until every voxel has sampled the brick
+  // (at most 512 / NT per thread), then they and the background go into
+  // the brick's interior, and the output leaf leaves with 16-byte stores:
+  // two per lane of a one-wave block instead of one store per list pass
+  // for the results and one per slab for the background.
+  constexpr int RQ = 512 / NT;
+  float r[RQ];
+#pragma unroll
+  for (int q = 0; q < RQ; ++q) {
+    const int i = tid + q * NT;
+    r[q] = 0.0f;
+    if (i < total) {
+      const int off = s.list[i];
+      r[q] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt, tf, tab,
+                                      s.vtab);
+      if (SIGNS && r[q] < iso) atomicOr(&s.sgn[off >> 5], 1u << (off & 31));
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < RQ; ++q) {
+    const int i = tid + q * NT;
+    if (i < total) {
+      const int off = s.list[i];
+      s.brick[((off >> 6) + 1) * 100 + (((off >> 3) & 7) + 1) * 10 + (off & 7) + 1] = r[q];
+    }
   }
 #pragma unroll
   for (int x = 0; x < 8; ++x)
-    if ((x % NW) == wave && !((mk[x] >> lane) & 1ull)) O.values[x * 64 + lane] = bg;
+    if ((x % NW) == wave && !((mk[x] >> lane) & 1ull)) s.brick[(x + 1) * 100 + ((lane >> 3) + 1) * 10 + (lane & 7) + 1] = bg;
   __syncthreads();
+#pragma unroll
+  for (int q = 0; q < OQ; ++q) {
+    const int e = tid + q * NT;
+    if (e < 128) {
+      const int x = e >> 4, y = (e >> 1) & 7, z = (e & 1) * 4;
+      const float* b = s.brick + (x + 1) * 100 + (y + 1) * 10 + (z + 1);
+      reinterpret_cast<float4*>(O.values)[e] = make_float4(b[0], b[1], b[2], b[3]);
+    }
+  }
   if (SIGNS && tid < 16) sg[tid] = s.sgn[tid];
+  // the next leaf of a block that loops over several restages the brick
+  __syncthreads();
 }
