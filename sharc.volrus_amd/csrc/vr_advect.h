@@ -4,6 +4,39 @@
 
 #include "vr_device.h"
 
+// Trilinear sample of the input at the departure point whose cell is
+// (i, j, k) and fractions (fx, fy, fz) (src/tools/interpolation.rs:11-48):
+// the cell's corners from the LDS brick of the leaf at (ox, oy, oz) when
+// they all lie in it, else one at a time through the tree.
+__device__ __forceinline__ float sample_departure(const GridView& in, const float* brick, int ox, int oy, int oz, int i,
+                                                  int j, int k, float fx, float fy, float fz) {
+  const int bi = i - ox, bj = j - oy, bk = k - oz;
+  float c[8];
+  if (bi >= -1 && bi <= 7 && bj >= -1 && bj <= 7 && bk >= -1 && bk <= 7) {
+    const int b0 = (bi + 1) * 100 + (bj + 1) * 10 + (bk + 1);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = brick[b0 + (q & 1) * 100 + ((q >> 1) & 1) * 10 + ((q >> 2) & 1)];
+  } else {
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) c[q] = grid_get(in, i + (q & 1), j + ((q >> 1) & 1), k + ((q >> 2) & 1));
+  }
+  return trilinear_blend(c, fx, fy, fz);
+}
+
+// Component of the departure point of a uniform velocity u: departure_from's
+// arithmetic on one axis (every stage samples the same u), the same
+// operations in the same order.
+template <int SCHEME>
+__device__ __forceinline__ double departure_1d(double x, double u, double dt, const double tf[3]) {
+  if constexpr (SCHEME == 1) {
+    return x - dt * (u * tf[0]);
+  } else {
+    const double y1 = x - dt * (u * tf[0]);
+    const double y2 = 0.75 * x + 0.25 * (y1 - dt * (u * tf[1]));
+    return (1.0 / 3.0) * x + (2.0 / 3.0) * (y2 - dt * (u * tf[2]));
+  }
+}
+
 // New value of the active voxel (ox + x, oy + y, oz + z): back-trace, then
 // trilinear sampling of the input (src/tools/advection.rs:39-53).  The
 // departure cell's corners come from the LDS brick (leaf plus one-voxel halo,
@@ -34,18 +67,7 @@ __device__ __forceinline__ float advect_voxel(const GridView& in, const VelView&
   const double ix = p[0] * in.inv_vs, iy = p[1] * in.inv_vs, iz = p[2] * in.inv_vs;
   const double flx = floor(ix), fly = floor(iy), flz = floor(iz);
   const float fx = (float)(ix - flx), fy = (float)(iy - fly), fz = (float)(iz - flz);
-  const int i = f64_to_i32(flx), j = f64_to_i32(fly), k = f64_to_i32(flz);
-  const int bi = i - ox, bj = j - oy, bk = k - oz;
-  float c[8];
-  if (bi >= -1 && bi <= 7 && bj >= -1 && bj <= 7 && bk >= -1 && bk <= 7) {
-    const int b0 = (bi + 1) * 100 + (bj + 1) * 10 + (bk + 1);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) c[q] = brick[b0 + (q & 1) * 100 + ((q >> 1) & 1) * 10 + ((q >> 2) & 1)];
-  } else {
-#pragma unroll 1
-    for (int q = 0; q < 8; ++q) c[q] = grid_get(in, i + (q & 1), j + ((q >> 1) & 1), k + ((q >> 2) & 1));
-  }
-  return trilinear_blend(c, fx, fy, fz);
+  return sample_departure(in, brick, ox, oy, oz, f64_to_i32(flx), f64_to_i32(fly), f64_to_i32(flz), fx, fy, fz);
 }
 
 // Stages the 10^3 brick of leaf li (leaf plus one-voxel halo) with NT
@@ -130,8 +152,12 @@ __device__ __forceinline__ HaloTask halo_one_task(int s) {  // s in [0, 200)
 // coordinate -1 or 8) by tasks with NT threads; nbget(slot) gives the
 // neighbouring leaf (-1: none).  Every load is issued before any LDS write;
 // an absent neighbour reads leaf 0 and is replaced by the background.
+// Only the tasks of neighbour slots in `slots` (bit per slot) run: the
+// other brick positions are left unwritten for a caller that never reads
+// them.
 template <int NT, class NB>
-__device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, float* brick, int tid) {
+__device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, float* brick, int tid,
+                                                 uint32_t slots = 0x7FFFFFFu) {
   constexpr int VQ = (72 + NT - 1) / NT, SQ = (200 + NT - 1) / NT;
   __builtin_assume(tid >= 0 && tid < NT);
   const float bg = in.background;
@@ -142,8 +168,10 @@ __device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, f
     const int v = tid + q * NT;
     const HaloTask t = halo_vec_task(v < 72 ? v : 0);
     vsrc[q] = nbget(t.slot);
-    vdst[q] = v < 72 ? t.dst : -1;
-    vv[q] = *reinterpret_cast<const VR_GLOBAL float4*>(&in.leaves[vsrc[q] < 0 ? 0 : vsrc[q]].values[t.src]);
+    vdst[q] = v < 72 && ((slots >> t.slot) & 1u) ? t.dst : -1;
+    vv[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (vdst[q] >= 0)
+      vv[q] = *reinterpret_cast<const VR_GLOBAL float4*>(&in.leaves[vsrc[q] < 0 ? 0 : vsrc[q]].values[t.src]);
   }
   float sv[SQ];
   int ssrc[SQ], sdst[SQ];
@@ -152,8 +180,9 @@ __device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, f
     const int s = tid + q * NT;
     const HaloTask t = halo_one_task(s < 200 ? s : 0);
     ssrc[q] = nbget(t.slot);
-    sdst[q] = s < 200 ? t.dst : -1;
-    sv[q] = in.leaves[ssrc[q] < 0 ? 0 : ssrc[q]].values[t.src];
+    sdst[q] = s < 200 && ((slots >> t.slot) & 1u) ? t.dst : -1;
+    sv[q] = 0.0f;
+    if (sdst[q] >= 0) sv[q] = in.leaves[ssrc[q] < 0 ? 0 : ssrc[q]].values[t.src];
   }
 #pragma unroll
   for (int q = 0; q < VQ; ++q)
@@ -180,8 +209,9 @@ __device__ __forceinline__ void stage_halo(const GridView& in, const int* nb, fl
 // holds entry d (d < 27), read with a lane permute, so the list needs no
 // LDS and no barrier before the halo loads.
 template <int NT>
-__device__ __forceinline__ void stage_halo_reg(const GridView& in, int nbv, float* brick, int tid) {
-  stage_halo_tasks<NT>(in, [nbv](int d) { return __shfl(nbv, d); }, brick, tid);
+__device__ __forceinline__ void stage_halo_reg(const GridView& in, int nbv, float* brick, int tid,
+                                               uint32_t slots = 0x7FFFFFFu) {
+  stage_halo_tasks<NT>(in, [nbv](int d) { return __shfl(nbv, d); }, brick, tid, slots);
 }
 
 // LDS of one advecting block (5088 bytes without the vortex tables, so 32
@@ -248,6 +278,39 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   }
   const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
   const bool tab = VK == VK_VORTEX && in.diag;
+  // Uniform velocity on a diagonal transform: component a of a departure
+  // point depends on the voxel's coordinate on axis a alone, so the cell
+  // index and fraction on each axis are computed once per leaf for the 8
+  // local coordinates, lane l (mod 8) holding those of coordinate l, and
+  // each voxel gathers its three with lane permutes.
+  // The tables also tell which sides of the brick the departure cells
+  // reach: the halo on the - side of axis a is read only when some cell
+  // starts at local -1, the + side only when one starts at 7 (a cell
+  // outside [-1, 7] reads through the tree), so only the neighbour slots
+  // (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1) whose sides are reached (bits
+  // of `slots`) are loaded; under a CFL number below one that is at most
+  // one side per axis.
+  const bool sep = VK == VK_UNIFORM && in.diag;
+  int sep_i[3] = {0, 0, 0};
+  float sep_f[3] = {0.0f, 0.0f, 0.0f};
+  uint32_t slots = 0x7FFFFFFu;
+  if (sep) {
+    const int l = lane & 7;
+    const int o[3] = {ox, oy, oz};
+    uint32_t side[3];  // bit d + 1: side d (-1, 0, +1) of the axis is reached
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double w = in.mat[5 * a] * (double)(o[a] + l) + in.mat[4 * a + 3];
+      const double ix = departure_1d<SCHEME>(w, vel.a[a], dt, tf) * in.inv_vs;
+      const double fl = floor(ix);
+      sep_f[a] = (float)(ix - fl);
+      sep_i[a] = f64_to_i32(fl);
+      const int b = sep_i[a] - o[a];
+      side[a] = (__ballot(b == -1) ? 1u : 0u) | 2u | (__ballot(b == 7) ? 4u : 0u);
+    }
+    const uint32_t yz = ((side[1] & 1u) ? side[2] : 0u) | (side[2] << 3) | ((side[1] & 4u) ? side[2] << 6 : 0u);
+    slots = ((side[0] & 1u) ? yz : 0u) | (yz << 9) | ((side[0] & 4u) ? yz << 18 : 0u);
+  }
   if (tab && tid < 16) {
     // row coordinates as view_index_to_world's diagonal form computes them
     const int a = tid >> 3, l = tid & 7;
@@ -274,7 +337,7 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   if (SIGNS && tid < 16) s.sgn[tid] = bg < iso ? ~sword : 0u;
   if (tid < 3) O.origin[tid] = tid == 0 ? ox : (tid == 1 ? oy : oz);
   if (tid == 3) O.pad = 0;
-  stage_halo_reg<NT>(in, nbv, s.brick, tid);
+  stage_halo_reg<NT>(in, nbv, s.brick, tid, slots);
   __syncthreads();
   // Results stay in registers until every voxel has sampled the brick
   // (at most 512 / NT per thread), then they and the background go into
@@ -283,16 +346,23 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   // for the results and one per slab for the background.
   constexpr int RQ = 512 / NT;
   float r[RQ];
+  const int wfirst = __builtin_amdgcn_readfirstlane(wave * 64);
 #pragma unroll
   for (int q = 0; q < RQ; ++q) {
-    const int i = tid + q * NT;
     r[q] = 0.0f;
-    if (i < total) {
-      const int off = s.list[i];
-      r[q] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, off >> 6, (off >> 3) & 7, off & 7, dt, tf, tab,
-                                      s.vtab);
-      if (SIGNS && r[q] < iso) atomicOr(&s.sgn[off >> 5], 1u << (off & 31));
+    if (wfirst + q * NT >= total) continue;  // no voxel of this wave in this pass (wave-uniform)
+    const int i = tid + q * NT;
+    const int off = s.list[i < total ? i : 0];
+    const int vx = off >> 6, vy = (off >> 3) & 7, vz = off & 7;
+    if (sep) {
+      // the whole wave permutes (a lane past the list reads harmlessly)
+      const int ci = __shfl(sep_i[0], vx), cj = __shfl(sep_i[1], vy), ck = __shfl(sep_i[2], vz);
+      const float fx = __shfl(sep_f[0], vx), fy = __shfl(sep_f[1], vy), fz = __shfl(sep_f[2], vz);
+      if (i < total) r[q] = sample_departure(in, s.brick, ox, oy, oz, ci, cj, ck, fx, fy, fz);
+    } else if (i < total) {
+      r[q] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, vx, vy, vz, dt, tf, tab, s.vtab);
     }
+    if (SIGNS && i < total && r[q] < iso) atomicOr(&s.sgn[off >> 5], 1u << (off & 31));
   }
   __syncthreads();
 #pragma unroll

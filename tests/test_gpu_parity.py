@@ -148,6 +148,23 @@ def test_advect_input_with_empty_leaf(orc, ctx):
     assert_nano_equal(got, want, "empty leaf")
 
 
+@pytest.mark.parametrize("scheme", [1, 3])
+@pytest.mark.parametrize("disp", [(0.5, -0.5, 0.0), (-0.3, 0.0, 0.9), (1.5, -1.25, 0.75), (-2.5, 2.5, -0.01)])
+def test_advect_uniform_displacements(orc, ctx, scheme, disp):
+    """Uniform velocity, whose departure cells are tabulated per leaf and axis
+    and whose brick is staged only on the sides they reach: displacements of
+    either sign, zero, and past one voxel (cells outside the brick go through
+    the tree)."""
+    announce(f"advect uniform disp={disp} scheme={scheme}")
+    vs = 1.0 / 64
+    sdf = orc.sphere(0.3, (0.5, 0.5, 0.5), vs, 3.0)
+    params = tuple(d * vs for d in disp)  # voxels per unit time, dt = 1
+    tf = orc.stage_factors(0.0, 0.0, 1.0, scheme)
+    want = orc.advect_analytic(sdf, 2, params, 1.0, scheme, tf).to_nano()
+    got = ctx.upload(sdf.to_nano()).advect(ctx.velocity_analytic(2, params), 1.0, 0.0, scheme).download()
+    assert_nano_equal(got, want, f"uniform disp={disp} scheme={scheme}")
+
+
 # ---------------------------------------------------------------- band tools
 def crossing_set(a):
     return set(map(tuple, np.asarray(a).tolist()))
