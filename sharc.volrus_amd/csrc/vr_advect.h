@@ -230,8 +230,35 @@ constexpr bool advect_tab(int vk) { return vk == VK_VORTEX || vk < 0; }
 template <int VK>
 using AdvectLDS = AdvectLDST<advect_tab(VK)>;
 
+// The loads that start leaf li, all vector loads with no dependence among
+// them: its 512 values (own), neighbour list entry lane (nbv, lanes < 27)
+// and record header dword lane (hdr, lanes < 20: origin, pad and the 8
+// mask words), so that a caller can issue them a leaf ahead.
+template <int NW>
+struct AdvectLoads {
+  float4 own[(128 + 64 * NW - 1) / (64 * NW)];
+  int nbv;
+  uint32_t hdr;
+};
+template <int NW>
+__device__ __forceinline__ AdvectLoads<NW> advect_issue(const VR_GLOBAL NanoLeaf* leaves, const VR_GLOBAL int* nbr,
+                                                        int li, int tid) {
+  constexpr int NT = 64 * NW;
+  const int lane = tid & 63;
+  const VR_GLOBAL NanoLeaf* L = leaves + li;
+  AdvectLoads<NW> a;
+  a.hdr = reinterpret_cast<const VR_GLOBAL uint32_t*>(L)[lane < 20 ? lane : 0];
+  a.nbv = nbr[(size_t)li * VR_NBR + (lane < VR_NBR ? lane : 0)];
+#pragma unroll
+  for (int q = 0; q < (128 + NT - 1) / NT; ++q) {
+    const int e = tid + q * NT;
+    a.own[q] = reinterpret_cast<const VR_GLOBAL float4*>(L->values)[e < 128 ? e : 0];
+  }
+  return a;
+}
+
 // Advects leaf li of `in` into out[li] with NW waves (the whole block,
-// tid in [0, 64 NW)): the active voxels are listed by a ballot/popcount
+// tid in [0, 64 NW)) from its loads `ld` (advect_issue): the active voxels are listed by a ballot/popcount
 // prefix over the 8 mask words and spread densely over the block's lanes
 // (one pass for a band leaf with 4 waves); the results and, at the
 // inactive positions, the background are gathered in the brick's interior
@@ -243,30 +270,20 @@ using AdvectLDS = AdvectLDST<advect_tab(VK)>;
 // results come, so marching cubes over the output can classify cells
 // without reading values.
 template <int VK, int SCHEME, int NW, bool SIGNS = false, class LDS>
-__device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
-                                            LDS& s, int tid, double dt, const double tf[3],
-                                            uint32_t* sg = nullptr, float iso = 0.0f) {
+__device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
+                                                   const AdvectLoads<NW>& ld, LDS& s, int tid, double dt,
+                                                   const double tf[3], uint32_t* sg = nullptr, float iso = 0.0f) {
   constexpr int NT = 64 * NW;
   const int lane = tid & 63, wave = tid >> 6;
-  const NanoLeaf& L = in.leaves[li];
   NanoLeaf& O = out[li];
   const float bg = in.background;
-  // The independent loads go out first (every lane loads, so no branch
-  // waits on them): the neighbour list entry and the leaf's own values.
   constexpr int OQ = (128 + NT - 1) / NT;
-  // every wave holds the neighbour list in lanes 0..26 (stage_halo_reg)
-  const int nbv = in.nbr[(size_t)li * VR_NBR + (lane < VR_NBR ? lane : 0)];
-  const uint64_t mword = L.mask[tid & 7];  // copied to the output by threads 0..7
-  const uint32_t sword = SIGNS ? reinterpret_cast<const uint32_t*>(L.mask)[tid & 15] : 0u;
-  float4 own[OQ];
+  const int nbv = ld.nbv;  // every wave holds the neighbour list in lanes 0..26 (stage_halo_reg)
+  uint64_t mk[8];
 #pragma unroll
-  for (int q = 0; q < OQ; ++q) {
-    const int e = tid + q * NT;
-    own[q] = reinterpret_cast<const float4*>(L.values)[e < 128 ? e : 0];
-  }
-  uint64_t mk[8];  // all eight mask words in one round trip
-#pragma unroll
-  for (int x = 0; x < 8; ++x) mk[x] = L.mask[x];
+  for (int x = 0; x < 8; ++x)
+    mk[x] = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ld.hdr, 4 + 2 * x) |
+            ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ld.hdr, 5 + 2 * x) << 32);
   int total = 0;
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
@@ -276,7 +293,8 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
     if ((x % NW) == wave && ((m >> lane) & 1ull)) s.list[total + rank] = (uint16_t)(x * 64 + lane);
     total += __popcll(m);
   }
-  const int ox = L.origin[0], oy = L.origin[1], oz = L.origin[2];
+  const int ox = __builtin_amdgcn_readlane((int)ld.hdr, 0), oy = __builtin_amdgcn_readlane((int)ld.hdr, 1),
+            oz = __builtin_amdgcn_readlane((int)ld.hdr, 2);
   const bool tab = VK == VK_VORTEX && in.diag;
   // Uniform velocity on a diagonal transform: component a of a departure
   // point depends on the voxel's coordinate on axis a alone, so the cell
@@ -326,17 +344,18 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
     if (e < 128) {
       const int x = e >> 4, y = (e >> 1) & 7, z = (e & 1) * 4;
       float* b = s.brick + (x + 1) * 100 + (y + 1) * 10 + (z + 1);
-      b[0] = own[q].x;
-      b[1] = own[q].y;
-      b[2] = own[q].z;
-      b[3] = own[q].w;
+      b[0] = ld.own[q].x;
+      b[1] = ld.own[q].y;
+      b[2] = ld.own[q].z;
+      b[3] = ld.own[q].w;
     }
   }
-  // Output header: origin and mask are the input's.
-  if (tid < 8) O.mask[tid] = mword;
-  if (SIGNS && tid < 16) s.sgn[tid] = bg < iso ? ~sword : 0u;
-  if (tid < 3) O.origin[tid] = tid == 0 ? ox : (tid == 1 ? oy : oz);
-  if (tid == 3) O.pad = 0;
+  // Output header: origin and mask are the input's, the pad word 0.
+  if (tid < 20) reinterpret_cast<uint32_t*>(&O)[tid] = tid == 3 ? 0u : ld.hdr;
+  if (SIGNS) {
+    const uint32_t sword = (uint32_t)__shfl((int)ld.hdr, 4 + (lane & 15));  // mask dword lane & 15
+    if (tid < 16) s.sgn[tid] = bg < iso ? ~sword : 0u;
+  }
   stage_halo_reg<NT>(in, nbv, s.brick, tid, slots);
   __syncthreads();
   // Results stay in registers until every voxel has sampled the brick
@@ -389,4 +408,13 @@ __device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& v
   if (SIGNS && tid < 16) sg[tid] = s.sgn[tid];
   // the next leaf of a block that loops over several restages the brick
   __syncthreads();
+}
+
+// advect_leaf_loaded with the leaf's loads issued here.
+template <int VK, int SCHEME, int NW, bool SIGNS = false, class LDS>
+__device__ __forceinline__ void advect_leaf(const GridView& in, const VelView& vel, NanoLeaf* out, int li,
+                                            LDS& s, int tid, double dt, const double tf[3],
+                                            uint32_t* sg = nullptr, float iso = 0.0f) {
+  advect_leaf_loaded<VK, SCHEME, NW, SIGNS>(in, vel, out, li, advect_issue<NW>(in.leaves, in.nbr, li, tid), s, tid,
+                                            dt, tf, sg, iso);
 }

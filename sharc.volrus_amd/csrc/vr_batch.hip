@@ -84,6 +84,25 @@ static void batch_drop_graph(vr_batch* b) {
 // specialised for it), or VK_MIXED when kinds differ (a per-leaf switch);
 // NW waves per leaf over its compacted active list.
 
+// Leaf li of a grid with velocity v from its loads ld; VK_MIXED switches on
+// the grid's velocity kind.
+template <int VK, int SCHEME, int NW, class LDS>
+__device__ __forceinline__ void batch_leaf(const GridView& in, const VelView& v, NanoLeaf* out, int li,
+                                           const AdvectLoads<NW>& ld, LDS& s, int tid, double dt, const double tf[3],
+                                           uint32_t* sg, float iso) {
+  if constexpr (VK != VK_MIXED) {
+    advect_leaf_loaded<VK, SCHEME, NW, true>(in, v, out, li, ld, s, tid, dt, tf, sg, iso);
+  } else {
+    switch (v.kind) {
+      case VK_UNIFORM: advect_leaf_loaded<VK_UNIFORM, SCHEME, NW, true>(in, v, out, li, ld, s, tid, dt, tf, sg, iso); break;
+      case VK_RIGID: advect_leaf_loaded<VK_RIGID, SCHEME, NW, true>(in, v, out, li, ld, s, tid, dt, tf, sg, iso); break;
+      case VK_VORTEX: advect_leaf_loaded<VK_VORTEX, SCHEME, NW, true>(in, v, out, li, ld, s, tid, dt, tf, sg, iso); break;
+      case VK_DENSE: advect_leaf_loaded<VK_DENSE, SCHEME, NW, true>(in, v, out, li, ld, s, tid, dt, tf, sg, iso); break;
+      default: advect_leaf_loaded<VK_GRIDS3, SCHEME, NW, true>(in, v, out, li, ld, s, tid, dt, tf, sg, iso); break;
+    }
+  }
+}
+
 template <int VK, int SCHEME, int NW, bool ONE, class LDS>
 __device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* vels, NanoLeaf* const* outs,
                                             const int4& ck, LDS& s, int tid, const double* dts, double dt,
@@ -97,19 +116,9 @@ __device__ __forceinline__ void batch_chunk(const GridView* ins, const VelView* 
   uint32_t* const sg = signs + (size_t)leaf_base[ck.x] * 16;
   // ONE: chunks of one leaf, so no loop (nothing is kept live across leaves).
   const int li_end = ONE ? ck.y + 1 : ck.y + ck.z;
-  for (int li = ck.y; li < li_end; ++li) {
-    if constexpr (VK != VK_MIXED) {
-      advect_leaf<VK, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso);
-    } else {
-      switch (v.kind) {
-        case VK_UNIFORM: advect_leaf<VK_UNIFORM, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
-        case VK_RIGID: advect_leaf<VK_RIGID, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
-        case VK_VORTEX: advect_leaf<VK_VORTEX, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
-        case VK_DENSE: advect_leaf<VK_DENSE, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
-        default: advect_leaf<VK_GRIDS3, SCHEME, NW, true>(in, v, out, li, s, tid, gdt, tf, sg + li * 16, iso); break;
-      }
-    }
-  }
+  for (int li = ck.y; li < li_end; ++li)
+    batch_leaf<VK, SCHEME, NW>(in, v, out, li, advect_issue<NW>(in.leaves, in.nbr, li, tid), s, tid, gdt, tf,
+                               sg + li * 16, iso);
 }
 
 // persistent != 0: `gridDim.x` blocks loop, taking chunks from their XCD's

@@ -130,6 +130,7 @@ def test_batch_graph_new_parameters_resize(orc, ctx, vr):
 
 @pytest.mark.parametrize("kinds,tuning", [
     ("mixed", (1, 0)),      # per-leaf velocity switch, one block per chunk
+    ("mixed", (1, 8)),      # persistent, one-leaf chunks
     ("mixed", (3, 8)),      # persistent, chunk size not dividing the grids
     ("vortex", (16, 4)),    # compacted 4-wave leaves, persistent
     ("rigid", (2, 0)),      # slab leaves, one block per chunk
