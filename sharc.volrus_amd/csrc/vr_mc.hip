@@ -1057,6 +1057,14 @@ struct MCForeign {
   int vrel;
 };
 
+// Vertex index (relative to the grid) from an edge-table entry v other than
+// 0xFFFF: 0x8000 | side-table slot, or the vertex relative to the tile's
+// first (vbrel = that vertex relative to the grid's first).
+__device__ __forceinline__ int mc_corner_known(uint32_t v, const MCEmitLDS& el, int vbrel) {
+  const int f = el.foreign[(v & 0xFFu) < MC_FSIDE ? (v & 0xFFu) : 0];
+  return (v & 0x8000u) ? f : vbrel + (int)v;
+}
+
 // Vertex index (relative to the grid) on local edge e of cell (x, y, z).
 __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const MCEmitLDS& el,
                                                const uint32_t* s_eslot, int vbrel, const MCForeign& fc) {
@@ -1348,9 +1356,22 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
             uz = cc & 7;
             tri_e = (uint32_t)(s_trin[el.rcube[rc]] >> (12 * q2));
           }
-          const int i0 = mc_corner_index(ux, uy, uz, (int)(tri_e & 15u), el, s_eslot, vbrel, fc);
-          const int i1 = mc_corner_index(ux, uy, uz, (int)((tri_e >> 4) & 15u), el, s_eslot, vbrel, fc);
-          const int i2 = mc_corner_index(ux, uy, uz, (int)((tri_e >> 8) & 15u), el, s_eslot, vbrel, fc);
+          // the three corners' table reads go out together; a corner on an
+          // edge past the side table (0xFFFF) is resolved again after them
+          const int e0 = (int)(tri_e & 15u), e1 = (int)((tri_e >> 4) & 15u), e2 = (int)((tri_e >> 8) & 15u);
+          const uint32_t s0 = s_eslot[e0], s1 = s_eslot[e1], s2 = s_eslot[e2];
+          const uint32_t v0 = el.edge[mc_edge_slot(ux, uy, uz, s0)], v1 = el.edge[mc_edge_slot(ux, uy, uz, s1)],
+                         v2 = el.edge[mc_edge_slot(ux, uy, uz, s2)];
+          int i0 = mc_corner_known(v0, el, vbrel), i1 = mc_corner_known(v1, el, vbrel),
+              i2 = mc_corner_known(v2, el, vbrel);
+          if (v0 == 0xFFFFu || v1 == 0xFFFFu || v2 == 0xFFFFu) {
+            if (v0 == 0xFFFFu) i0 = mc_foreign_index(ux, uy, uz, e0, fc.processed, fc.s_ownsort, fc.s_cand, fc.ttile,
+                                                     fc.meta, fc.vbase, fc.vrel);
+            if (v1 == 0xFFFFu) i1 = mc_foreign_index(ux, uy, uz, e1, fc.processed, fc.s_ownsort, fc.s_cand, fc.ttile,
+                                                     fc.meta, fc.vbase, fc.vrel);
+            if (v2 == 0xFFFFu) i2 = mc_foreign_index(ux, uy, uz, e2, fc.processed, fc.s_ownsort, fc.s_cand, fc.ttile,
+                                                     fc.meta, fc.vbase, fc.vrel);
+          }
           if (ti < tmax) {
             uint32_t* o = tris + 3 * (size_t)ti;
             o[0] = (uint32_t)i0;
