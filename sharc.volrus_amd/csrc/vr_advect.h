@@ -148,17 +148,16 @@ __device__ __forceinline__ HaloTask halo_one_task(int s) {  // s in [0, 200)
   return t;
 }
 
-// Stages the 488 halo values of the 10^3 brick (the positions with a local
-// coordinate -1 or 8) by tasks with NT threads; nbget(slot) gives the
-// neighbouring leaf (-1: none).  Every load is issued before any LDS write;
-// an absent neighbour reads leaf 0 and is replaced by the background.
-// Only the tasks of neighbour slots in `slots` (bit per slot) run: the
-// other brick positions are left unwritten for a caller that never reads
-// them.
-template <int NT, class NB>
-__device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, float* brick, int tid,
-                                                 uint32_t slots = 0x7FFFFFFu) {
-  constexpr int VQ = (72 + NT - 1) / NT, SQ = (200 + NT - 1) / NT;
+// Stages halo tasks with NT threads: NVEC four-value tasks vtask(v) and
+// NONE one-value tasks stask(s) (a task with dst < 0 is skipped: its brick
+// positions are left unwritten for a caller that never reads them);
+// nbget(slot) gives the neighbouring leaf (-1: none).  Every load is issued
+// before any LDS write; an absent neighbour reads leaf 0 and is replaced by
+// the background.
+template <int NT, int NVEC, int NONE, class NB, class VT, class ST>
+__device__ __forceinline__ void stage_tasks(const GridView& in, NB nbget, float* brick, int tid, VT vtask,
+                                            ST stask) {
+  constexpr int VQ = (NVEC + NT - 1) / NT, SQ = (NONE + NT - 1) / NT;
   __builtin_assume(tid >= 0 && tid < NT);
   const float bg = in.background;
   float4 vv[VQ];
@@ -166,9 +165,9 @@ __device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, f
 #pragma unroll
   for (int q = 0; q < VQ; ++q) {
     const int v = tid + q * NT;
-    const HaloTask t = halo_vec_task(v < 72 ? v : 0);
+    const HaloTask t = vtask(v < NVEC ? v : 0);
     vsrc[q] = nbget(t.slot);
-    vdst[q] = v < 72 && ((slots >> t.slot) & 1u) ? t.dst : -1;
+    vdst[q] = v < NVEC ? t.dst : -1;
     vv[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (vdst[q] >= 0)
       vv[q] = *reinterpret_cast<const VR_GLOBAL float4*>(&in.leaves[vsrc[q] < 0 ? 0 : vsrc[q]].values[t.src]);
@@ -178,9 +177,9 @@ __device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, f
 #pragma unroll
   for (int q = 0; q < SQ; ++q) {
     const int s = tid + q * NT;
-    const HaloTask t = halo_one_task(s < 200 ? s : 0);
+    const HaloTask t = stask(s < NONE ? s : 0);
     ssrc[q] = nbget(t.slot);
-    sdst[q] = s < 200 && ((slots >> t.slot) & 1u) ? t.dst : -1;
+    sdst[q] = s < NONE ? t.dst : -1;
     sv[q] = 0.0f;
     if (sdst[q] >= 0) sv[q] = in.leaves[ssrc[q] < 0 ? 0 : ssrc[q]].values[t.src];
   }
@@ -197,6 +196,73 @@ __device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, f
 #pragma unroll
   for (int q = 0; q < SQ; ++q)
     if (sdst[q] >= 0) brick[sdst[q]] = ssrc[q] < 0 ? bg : sv[q];
+}
+
+// The 488 halo values of the 10^3 brick (the positions with a local
+// coordinate -1 or 8), of the neighbour slots in `slots` (bit per slot).
+template <int NT, class NB>
+__device__ __forceinline__ void stage_halo_tasks(const GridView& in, NB nbget, float* brick, int tid,
+                                                 uint32_t slots = 0x7FFFFFFu) {
+  stage_tasks<NT, 72, 200>(
+      in, nbget, brick, tid,
+      [slots](int v) {
+        HaloTask t = halo_vec_task(v);
+        if (!((slots >> t.slot) & 1u)) t.dst = -1;
+        return t;
+      },
+      [slots](int s) {
+        HaloTask t = halo_one_task(s);
+        if (!((slots >> t.slot) & 1u)) t.dst = -1;
+        return t;
+      });
+}
+
+// The halo on one side per axis, sx, sy, sz in {-1, 0, +1} (0: that axis
+// needs none): the face of each chosen side, the edges and the corner where
+// two or three chosen sides meet -- at most 217 values, as 34 four-value
+// tasks (the x and y faces by rows along z, the z-parallel edge) and 81
+// one-value tasks (the z face, the y- and x-parallel edges, the corner).
+// Same slots, sources and brick positions as the full task set's.
+template <int NT, class NB>
+__device__ __forceinline__ void stage_halo_sided(const GridView& in, NB nbget, float* brick, int tid, int sx, int sy,
+                                                 int sz) {
+  // per side: source offset of the neighbour's facing slab and brick base
+  const int xs = sx < 0 ? 448 : 0, xd = sx < 0 ? 0 : 900, ys = sy < 0 ? 56 : 0, yd = sy < 0 ? 0 : 90;
+  const int zs = sz < 0 ? 7 : 0, zd = sz < 0 ? 0 : 9;
+  stage_tasks<NT, 34, 81>(
+      in, nbget, brick, tid,
+      [=](int v) {
+        const int h4 = (v & 1) * 4, r = (v >> 1) & 7;
+        HaloTask t{0, 0, -1};
+        if (v < 16) {  // x face, row y = r
+          t = HaloTask{(sx + 1) * 9 + 4, xs + r * 8 + h4, xd + (r + 1) * 10 + 1 + h4};
+          if (sx == 0) t.dst = -1;
+        } else if (v < 32) {  // y face, row x = r
+          t = HaloTask{9 + (sy + 1) * 3 + 1, r * 64 + ys + h4, (r + 1) * 100 + yd + 1 + h4};
+          if (sy == 0) t.dst = -1;
+        } else {  // z-parallel edge
+          t = HaloTask{(sx + 1) * 9 + (sy + 1) * 3 + 1, xs + ys + h4, xd + yd + 1 + h4};
+          if (sx == 0 || sy == 0) t.dst = -1;
+        }
+        return t;
+      },
+      [=](int s) {
+        HaloTask t{0, 0, -1};
+        if (s < 64) {  // z face, row (x, y) = (s >> 3, s & 7)
+          t = HaloTask{12 + sz + 1, (s >> 3) * 64 + (s & 7) * 8 + zs, ((s >> 3) + 1) * 100 + ((s & 7) + 1) * 10 + zd};
+          if (sz == 0) t.dst = -1;
+        } else if (s < 72) {  // y-parallel edge, y = s - 64
+          t = HaloTask{(sx + 1) * 9 + 3 + sz + 1, xs + (s - 64) * 8 + zs, xd + (s - 63) * 10 + zd};
+          if (sx == 0 || sz == 0) t.dst = -1;
+        } else if (s < 80) {  // x-parallel edge, x = s - 72
+          t = HaloTask{9 + (sy + 1) * 3 + sz + 1, (s - 72) * 64 + ys + zs, (s - 71) * 100 + yd + zd};
+          if (sy == 0 || sz == 0) t.dst = -1;
+        } else {  // corner
+          t = HaloTask{(sx + 1) * 9 + (sy + 1) * 3 + sz + 1, xs + ys + zs, xd + yd + zd};
+          if (sx == 0 || sy == 0 || sz == 0) t.dst = -1;
+        }
+        return t;
+      });
 }
 
 // The halo with the neighbour list in LDS (nb[d], d < 27), after a barrier.
@@ -308,10 +374,14 @@ __device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const Vel
   // (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1) whose sides are reached (bits
   // of `slots`) are loaded; under a CFL number below one that is at most
   // one side per axis.
+  // When each axis reaches at most one side, only those sides are staged
+  // (stage_halo_sided: about half the tasks).
   const bool sep = VK == VK_UNIFORM && in.diag;
   int sep_i[3] = {0, 0, 0};
   float sep_f[3] = {0.0f, 0.0f, 0.0f};
   uint32_t slots = 0x7FFFFFFu;
+  bool sided = false;
+  int sd[3] = {0, 0, 0};
   if (sep) {
     const int l = lane & 7;
     const int o[3] = {ox, oy, oz};
@@ -328,6 +398,9 @@ __device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const Vel
     }
     const uint32_t yz = ((side[1] & 1u) ? side[2] : 0u) | (side[2] << 3) | ((side[1] & 4u) ? side[2] << 6 : 0u);
     slots = ((side[0] & 1u) ? yz : 0u) | (yz << 9) | ((side[0] & 4u) ? yz << 18 : 0u);
+    sided = side[0] != 7u && side[1] != 7u && side[2] != 7u;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sd[a] = (side[a] & 1u) ? -1 : ((side[a] & 4u) ? 1 : 0);
   }
   if (tab && tid < 16) {
     // row coordinates as view_index_to_world's diagonal form computes them
@@ -356,7 +429,10 @@ __device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const Vel
     const uint32_t sword = (uint32_t)__shfl((int)ld.hdr, 4 + (lane & 15));  // mask dword lane & 15
     if (tid < 16) s.sgn[tid] = bg < iso ? ~sword : 0u;
   }
-  stage_halo_reg<NT>(in, nbv, s.brick, tid, slots);
+  if (sided)
+    stage_halo_sided<NT>(in, [nbv](int d) { return __shfl(nbv, d); }, s.brick, tid, sd[0], sd[1], sd[2]);
+  else
+    stage_halo_reg<NT>(in, nbv, s.brick, tid, slots);
   __syncthreads();
   // Results stay in registers until every voxel has sampled the brick
   // (at most 512 / NT per thread), then they and the background go into
