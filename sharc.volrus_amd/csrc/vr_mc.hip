@@ -949,7 +949,8 @@ __global__ void k_mc_ranges(const int* tile_base, int G, const int* ntiles, cons
 // ------------------------------------------------------------------ pass 2
 // Owned vertices (owner's edge orientation, f32 t, f64 position;
 // volume_to_mesh.rs:162-186) and triangles in TRI_TABLE order (:191-200).
-// Mixed cells go in rounds of 64, one per lane: a wave scan places the
+// Mixed cells go in rounds of 64, one per lane (with the count pass's
+// lists: of 128, two consecutive cells per lane): a wave scan places the
 // round's vertices and triangles, and lists of the round's vertices
 // (cell, edge) and triangles (round cell and row entry; with the count
 // pass's lists, the cell and the triangle's three edges) spread the writes
@@ -965,14 +966,19 @@ __global__ void k_mc_ranges(const int* tile_base, int G, const int* ntiles, cons
 // slot a triangle reads was written in this or an earlier round, before the
 // round's triangles.  A round with more items than the lists hold is
 // written cell by cell.  Vertex indices are relative to the tile's grid.
-// The per-wave LDS is kept under 5.8 KB (the round's cell list is built per
+// The per-wave LDS is kept to 6.1 KB (the round's cell list is built per
 // round, the 32-bit triangle entries of the list mode share the space of
-// the sign rows it does not use, the triangle table is nibble-packed), so
-// three 8-wave blocks fit a CU: 6 waves per SIMD.
+// the sign rows and cell lists it does not use, the triangle table is
+// nibble-packed) and a block's to 53 744 bytes, so that three 8-wave
+// blocks fit a CU's 160 KB (6 waves per SIMD): LDS is granted in units
+// coarser than the byte count, and at 54 256 bytes per block only two
+// blocks were resident (the emit pass took 10.8 ms instead of 8.0).
 #define MC_EWAVES 8
 #define MC_ETHREADS (64 * MC_EWAVES)
-#define MC_RV 128   // vertices listed per round
-#define MC_RT 192   // triangles listed per round
+#define MC_RV 128   // vertices listed per round (rounds of 64 cells)
+#define MC_RT 192   // triangles listed per round (rounds of 64 cells)
+#define MC_RVL 192  // vertices listed per round of 128 cells (LIST)
+#define MC_RTL 288  // triangles listed per round of 128 cells (LIST)
 #define MC_FL 192   // earlier-tile edges listed per resolution pass
 #define MC_FSIDE 96 // side-table entries per tile
 struct MCEmitLDS {
@@ -983,9 +989,11 @@ struct MCEmitLDS {
         uint16_t tlist[MC_RT];   // round cell | triangle of the cell << 6
         uint16_t flist[MC_FL];   // cell << 4 | edge, earlier-tile edges to resolve
       };
+      uint16_t rlist[64];        // the round's mixed cells
+      uint8_t rcube[64];         // their cases
     };
     union {                      // lists from the count pass
-      uint32_t tlist32[MC_RT];   // cell | the triangle's edges << 9 (4 bits each)
+      uint32_t tlist32[MC_RTL];  // cell | the triangle's edges << 9 (4 bits each)
       uint16_t flist32[MC_FL];   // as flist
     };
   };
@@ -993,11 +1001,9 @@ struct MCEmitLDS {
   int ttile[VR_NBR];             // neighbouring tiles (-1: not processed)
   uint16_t self[28];             // per cell class: edges the cell owns itself
   uint16_t fitem[28];            // per cell class: edges owned in an earlier tile that this cell resolves
-  uint16_t rlist[64];            // the round's mixed cells
-  uint8_t rcube[64];             // their cases
   uint16_t edge[MC_EDGES];       // lattice-edge table of the tile
   int foreign[MC_FSIDE];         // vertex index (relative to the grid) of edges owned in an earlier tile
-  uint16_t vlist[MC_RV];         // cell << 4 | edge
+  uint16_t vlist[MC_RVL];        // cell << 4 | edge
 };
 
 // Tile t's lists and header in one vector load per array: lanes < 27 get
@@ -1035,15 +1041,15 @@ __device__ __forceinline__ void mc_round_list(uint32_t* rem, int* rk, int base, 
   *rk = k;
 }
 
-// LIST: requests the entries of the first two rounds of the tile whose
-// header hl holds (mc_emit_lists): *a = entry lane, *b = entry 64 + lane
-// (0 past the count).
+// LIST: requests the entries of the first round (128 cells) of the tile
+// whose header hl holds (mc_emit_lists): *a = entry 2 lane, *b = entry
+// 2 lane + 1 (0 past the count).
 __device__ __forceinline__ void mc_list_issue(const uint32_t* mcell, int hl, int lane, uint32_t* a, uint32_t* b) {
   const int off = __builtin_amdgcn_readlane(hl, VR_NBR + 8), cnt = __builtin_amdgcn_readlane(hl, VR_NBR + 9);
   const int nv = __builtin_amdgcn_readlane(hl, VR_NBR), ntr = __builtin_amdgcn_readlane(hl, VR_NBR + 1);
   if (off < 0 || (nv == 0 && ntr == 0)) return;
-  if (lane < cnt) *a = mcell[(size_t)off + lane];
-  if (64 + lane < cnt) *b = mcell[(size_t)off + 64 + lane];
+  if (2 * lane < cnt) *a = mcell[(size_t)off + 2 * lane];
+  if (2 * lane + 1 < cnt) *b = mcell[(size_t)off + 2 * lane + 1];
 }
 
 // What resolving an earlier tile's edge again needs (mc_foreign_index).
@@ -1076,8 +1082,8 @@ __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const
 }
 
 // LIST: the tiles' mixed cells and cases come from the count pass's lists
-// (mcell, mtile) instead of the sign rows; the first two rounds' entries
-// of the next tile load while the current one is processed.  A tile whose
+// (mcell, mtile) instead of the sign rows; the first round's entries of
+// the next tile load while the current one is processed.  A tile whose
 // list did not fit (offset -1) writes nothing: that happens only when the
 // mesh is larger than the capacity the lists were sized for.
 // Edges owned in an earlier tile (fm0 of cell c0, then fm1 of cell c1, per
@@ -1168,8 +1174,8 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
   MCMaskQ q{};
   if (MASKS && !LIST && t < t_end)
     q = mc_mask_issue(views[__builtin_amdgcn_readlane(h_tl, VR_NBR + 7)].signs, h_tl, lane);
-  // LIST: entries of the first two rounds of tile t (e0, e1); lanes past
-  // the count hold 0.
+  // LIST: entries 2 lane and 2 lane + 1 of tile t (e0, e1: its first
+  // round); lanes past the count hold 0.
   uint32_t e0 = 0u, e1 = 0u;
   if (LIST && t < t_end) mc_list_issue(mcell, h_tl, lane, &e0, &e1);
   for (; t < t_end; t += t_step) {
@@ -1188,7 +1194,7 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     if (tf < t_end)
       mc_emit_lists<LIST>(tleaf, ttile, cmask, tv, tt, vbase, tbase, tiles, mtile, tf, lane, &ftl, &ftt, &fcm);
     const MCMaskQ cq = q;
-    uint32_t c0 = e0, c1 = e1;  // LIST: this tile's first two rounds
+    uint32_t c0 = e0, c1 = e1;  // LIST: this tile's first round
     if (LIST) {
       e0 = e1 = 0u;
       if (tn < t_end) mc_list_issue(mcell, h_tl, lane, &e0, &e1);
@@ -1234,103 +1240,133 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     }
     const MCForeign fc{processed, s_ownsort, s_cand, el.ttile, meta, vbase, vrel};
     int vcarry = 0, tcarry = 0, fcarry = 0;
+    // CPL mixed cells per lane and round, cell base + CPL * lane + j: LIST
+    // takes rounds of 128 cells (two consecutive cells per lane), which
+    // halves the rounds of a typical tile and fills the vertex and triangle
+    // passes better; else rounds of 64.
+    constexpr int CPL = LIST ? 2 : 1, RC = 64 * CPL;
+    constexpr int RV = LIST ? MC_RVL : MC_RV, RT = LIST ? MC_RTL : MC_RT, NPC = LIST ? 3 : 1;
     if constexpr (LIST) {
-      // The earlier-tile edges of the first two rounds' cells (entries c0,
-      // c1) in one pass at the start of the tile, so that its loads of the
+      // The earlier-tile edges of the first round's cells (entries c0, c1)
+      // in one pass at the start of the tile, so that its loads of the
       // earlier tiles' face meta wait once per tile; later rounds resolve
       // their own.
       wave_lds_sync();
       uint32_t fm0 = 0u, fm1 = 0u;
-      if (lane < n) fm0 = (uint32_t)(s_emn[c0 >> 9] & 0xFFFu) & el.fitem[s_ccls[c0 & 511u]];
-      if (64 + lane < n) fm1 = (uint32_t)(s_emn[c1 >> 9] & 0xFFFu) & el.fitem[s_ccls[c1 & 511u]];
+      if (2 * lane < n) fm0 = (uint32_t)(s_emn[c0 >> 9] & 0xFFFu) & el.fitem[s_ccls[c0 & 511u]];
+      if (2 * lane + 1 < n) fm1 = (uint32_t)(s_emn[c1 >> 9] & 0xFFFu) & el.fitem[s_ccls[c1 & 511u]];
       mc_foreign_pass(fm0, (int)(c0 & 511u), fm1, (int)(c1 & 511u), lane, el.flist32, &fcarry, el, s_eslot, fc);
     }
-    for (int base = 0; base < n; base += 64) {
-      const int i = base + lane;
-      uint32_t ent = 0u;
+    for (int base = 0; base < n; base += RC) {
+      uint32_t ent[CPL];
       if constexpr (LIST) {
-        // this round's entry, and the load of the round after next
-        ent = c0;
-        c0 = c1;
-        c1 = 0u;
-        if (base + 128 + lane < n) c1 = mcell[(size_t)moff + base + 128 + lane];
-        if (i < n) el.rlist[lane] = (uint16_t)(ent & 511u);
+        // this round's entries, and the loads of the next round's
+        ent[0] = c0;
+        ent[CPL - 1] = c1;
+        c0 = c1 = 0u;
+        const int nx = base + RC + 2 * lane;
+        if (nx < n) c0 = mcell[(size_t)moff + nx];
+        if (nx + 1 < n) c1 = mcell[(size_t)moff + nx + 1];
       } else {
+        ent[0] = 0u;
         mc_round_list(&rem, &rk, base, lane, el.rlist);
+        wave_lds_sync();
       }
-      wave_lds_sync();
-      int c = 0, packed = 0, ntr = 0;
-      uint32_t owned = 0u, cube = 0u, fm = 0u;
-      if (i < n) {
-        c = LIST ? (int)(ent & 511u) : el.rlist[lane];
-        const int ccls = s_ccls[c];
-        cube = LIST ? (ent >> 9) : mc_case_rows(el.rows, c >> 6, (c >> 3) & 7, c & 7);
-        el.rcube[lane] = (uint8_t)cube;
-        const uint32_t emn = s_emn[cube];
-        owned = (emn & 0xFFFu) & el.self[ccls];
-        fm = (emn & 0xFFFu) & el.fitem[ccls];
-        ntr = (int)(emn >> 12);
-        packed = __popc(owned) | (ntr << 16);
+      int cl[CPL], ntr[CPL];
+      uint32_t owned[CPL], fm[CPL], cube[CPL];
+      int pv = 0, pt = 0;
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        cl[j] = 0;
+        ntr[j] = 0;
+        owned[j] = fm[j] = cube[j] = 0u;
+        if (base + CPL * lane + j < n) {
+          cl[j] = LIST ? (int)(ent[j] & 511u) : el.rlist[lane];
+          const int ccls = s_ccls[cl[j]];
+          cube[j] = LIST ? (ent[j] >> 9) : mc_case_rows(el.rows, cl[j] >> 6, (cl[j] >> 3) & 7, cl[j] & 7);
+          if (!LIST) el.rcube[lane] = (uint8_t)cube[j];
+          const uint32_t emn = s_emn[cube[j]];
+          owned[j] = (emn & 0xFFFu) & el.self[ccls];
+          fm[j] = (emn & 0xFFFu) & el.fitem[ccls];
+          ntr[j] = (int)(emn >> 12);
+        }
+        pv += __popc(owned[j]);
+        pt += ntr[j];
       }
+      const int packed = pv | (pt << 16);
       const int incl = wave_incl_scan(packed);
       const int excl = incl - packed, tot = __builtin_amdgcn_readlane(incl, 63);
       const int rv = tot & 0xFFFF, rt = tot >> 16;
-      const int vr = excl & 0xFFFF, tr = excl >> 16;
-      const int cx = c >> 6, cy = (c >> 3) & 7, cz = c & 7;
-      const bool listed = rv <= MC_RV && rt <= MC_RT;  // uniform
-      if (listed && i < n) {
+      const int vr = excl & 0xFFFF, tr = excl >> 16;  // the lane's first vertex and triangle in the round
+      const bool listed = rv <= RV && rt <= RT;       // uniform
+      if (listed) {
         int k = vr;
-        for (uint32_t m = owned; m; m &= m - 1u, ++k) el.vlist[k] = (uint16_t)((c << 4) | (__ffs(m) - 1));
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+          for (uint32_t m = owned[j]; m; m &= m - 1u, ++k) el.vlist[k] = (uint16_t)((cl[j] << 4) | (__ffs(m) - 1));
       }
-      // LIST: the corner values of the round's vertices (at most MC_RV = 2 x
+      // LIST: the corner values of the round's vertices (at most RV = 3 x
       // 64) are requested now and consumed after the foreign edges and the
       // triangle list are done.
-      MCCorner pc0{}, pc1{};
+      // (values in pva, pvb; the presence bits of vertex j at bits 2 j, 2 j + 1 of pok)
+      float pva[NPC], pvb[NPC];
+      uint32_t pok = 0u;
+#pragma unroll
+      for (int j = 0; j < NPC; ++j) pva[j] = pvb[j] = 0.0f;
       if (LIST && listed) {
         wave_lds_sync();
-        if (lane < rv) {
-          const int it = el.vlist[lane];
-          pc0 = mc_corner_load(leaves, el.tleaf, (it >> 10) & 7, (it >> 7) & 7, (it >> 4) & 7, it & 15, s_einfo);
-        }
-        if (64 + lane < rv) {
-          const int it = el.vlist[64 + lane];
-          pc1 = mc_corner_load(leaves, el.tleaf, (it >> 10) & 7, (it >> 7) & 7, (it >> 4) & 7, it & 15, s_einfo);
-        }
+#pragma unroll
+        for (int j = 0; j < NPC; ++j)
+          if (lane + 64 * j < rv) {
+            const int it = el.vlist[lane + 64 * j];
+            const MCCorner c =
+                mc_corner_load(leaves, el.tleaf, (it >> 10) & 7, (it >> 7) & 7, (it >> 4) & 7, it & 15, s_einfo);
+            pva[j] = c.va;
+            pvb[j] = c.vb;
+            pok |= (uint32_t)c.ok << (2 * j);
+          }
       }
-      // Edges owned in an earlier tile (LIST: those of the first 128 cells
-      // were resolved at the start of the tile).
-      if (!LIST || base >= 128) mc_foreign_pass(fm, c, 0u, 0, lane, LIST ? el.flist32 : el.flist, &fcarry, el, s_eslot, fc);
+      // Edges owned in an earlier tile (LIST: those of the first round's
+      // cells were resolved at the start of the tile).
+      if (!LIST || base >= RC)
+        mc_foreign_pass(fm[0], cl[0], CPL > 1 ? fm[CPL - 1] : 0u, CPL > 1 ? cl[CPL - 1] : 0, lane,
+                        LIST ? el.flist32 : el.flist, &fcarry, el, s_eslot, fc);
       if (listed) {
         // (the triangle list shares its LDS with the foreign list above)
-        if (i < n) {
-          if constexpr (LIST) {
-            const uint64_t row = s_trin[cube];
-            for (int q2 = 0; q2 < ntr; ++q2)
-              el.tlist32[tr + q2] = (uint32_t)c | (((uint32_t)(row >> (12 * q2)) & 0xFFFu) << 9);
-          } else {
-            for (int q2 = 0; q2 < ntr; ++q2) el.tlist[tr + q2] = (uint16_t)(lane | (q2 << 6));
+        {
+          int k = tr;
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) {
+            if constexpr (LIST) {
+              const uint64_t row = s_trin[cube[j]];
+              for (int q2 = 0; q2 < ntr[j]; ++q2, ++k)
+                el.tlist32[k] = (uint32_t)cl[j] | (((uint32_t)(row >> (12 * q2)) & 0xFFFu) << 9);
+            } else {
+              for (int q2 = 0; q2 < ntr[j]; ++q2, ++k) el.tlist[k] = (uint16_t)(lane | (q2 << 6));
+            }
           }
         }
         wave_lds_sync();
         if constexpr (LIST) {
-          // at most two vertices per lane, with their corner values in pc0, pc1
+          // at most three vertices per lane, with their corner values requested above
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
+          for (int j = 0; j < NPC; ++j) {
             const int k = lane + 64 * j;
             if (k < rv) {
               const int vi = vb + vcarry + k;
-              const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
-              const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
+              const int it = el.vlist[k], cv = it >> 4, ve = it & 15;
+              const int ux = cv >> 6, uy = (cv >> 3) & 7, uz = cv & 7;
               el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
               if (vi < vmax)
-                mc_vertex_store(j == 0 ? pc0 : pc1, bg, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
+                mc_vertex_store(MCCorner{pva[j], pvb[j], (int)((pok >> (2 * j)) & 3u)}, bg, ux, uy, uz, ve, to, vs,
+                                iso, s_einfo, verts + 3 * (size_t)vi);
             }
           }
         } else {
           for (int k = lane; k < rv; k += 64) {
             const int vi = vb + vcarry + k;
-            const int it = el.vlist[k], cc = it >> 4, ve = it & 15;
-            const int ux = cc >> 6, uy = (cc >> 3) & 7, uz = cc & 7;
+            const int it = el.vlist[k], cv = it >> 4, ve = it & 15;
+            const int ux = cv >> 6, uy = (cv >> 3) & 7, uz = cv & 7;
             el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
             if (vi < vmax)
               mc_write_vertex(leaves, bg, el.tleaf, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
@@ -1350,10 +1386,10 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
           } else {
             const uint32_t it = el.tlist[k];
             const int rc = (int)(it & 63u), q2 = (int)(it >> 6);
-            const int cc = el.rlist[rc];
-            ux = cc >> 6;
-            uy = (cc >> 3) & 7;
-            uz = cc & 7;
+            const int cv = el.rlist[rc];
+            ux = cv >> 6;
+            uy = (cv >> 3) & 7;
+            uz = cv & 7;
             tri_e = (uint32_t)(s_trin[el.rcube[rc]] >> (12 * q2));
           }
           // the three corners' table reads go out together; a corner on an
@@ -1383,24 +1419,33 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
         // Dense round: cell by cell, every vertex (and its table slot)
         // before any triangle of the round.
         wave_lds_sync();
-        if (i < n) {
+        {
           int vk = vcarry + vr;
-          for (uint32_t m = owned; m; m &= m - 1u, ++vk) {
-            const int ve = __ffs(m) - 1;
-            el.edge[mc_edge_slot(cx, cy, cz, s_eslot[ve])] = (uint16_t)vk;
-            if (vb + vk < vmax)
-              mc_write_vertex(leaves, bg, el.tleaf, cx, cy, cz, ve, to, vs, iso, s_einfo,
-                              verts + 3 * (size_t)(vb + vk));
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) {
+            const int cx = cl[j] >> 6, cy = (cl[j] >> 3) & 7, cz = cl[j] & 7;
+            for (uint32_t m = owned[j]; m; m &= m - 1u, ++vk) {
+              const int ve = __ffs(m) - 1;
+              el.edge[mc_edge_slot(cx, cy, cz, s_eslot[ve])] = (uint16_t)vk;
+              if (vb + vk < vmax)
+                mc_write_vertex(leaves, bg, el.tleaf, cx, cy, cz, ve, to, vs, iso, s_einfo,
+                                verts + 3 * (size_t)(vb + vk));
+            }
           }
         }
         wave_lds_sync();
-        if (i < n) {
-          const uint64_t row = s_trin[cube];
+        {
           int ti = tb + tcarry + tr;
-          for (int q2 = 0; q2 < ntr && ti < tmax; ++q2, ++ti)
-            for (int jj = 0; jj < 3; ++jj)
-              tris[3 * (size_t)ti + jj] = (uint32_t)mc_corner_index(
-                  cx, cy, cz, (int)((row >> (4 * (3 * q2 + jj))) & 15u), el, s_eslot, vbrel, fc);
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) {
+            const int cx = cl[j] >> 6, cy = (cl[j] >> 3) & 7, cz = cl[j] & 7;
+            const uint64_t row = s_trin[cube[j]];
+            for (int q2 = 0; q2 < ntr[j]; ++q2, ++ti)
+              if (ti < tmax)
+                for (int jj = 0; jj < 3; ++jj)
+                  tris[3 * (size_t)ti + jj] = (uint32_t)mc_corner_index(
+                      cx, cy, cz, (int)((row >> (4 * (3 * q2 + jj))) & 15u), el, s_eslot, vbrel, fc);
+          }
         }
       }
       wave_lds_sync();
