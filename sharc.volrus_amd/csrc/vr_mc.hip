@@ -99,7 +99,8 @@ hipError_t vr_upload_mc_tables(const uint16_t* edge_mask, const uint8_t* ntri, c
       mx = 72;
       my = 8;
     }
-    eslot[e] = k0 | (mx << 11) | (my << 18);
+    // in bytes of the 16-bit table: twice the slot base, x and y strides
+    eslot[e] = (2 * k0) | ((2 * mx) << 12) | ((2 * my) << 20);
   }
   const int rep[3] = {0, 3, 7};
   for (int c = 0; c < 27; ++c)
@@ -711,10 +712,18 @@ __device__ __forceinline__ uint32_t mc_case_rows(const uint32_t* rows, int x, in
          (((r10 >> 1) & 1u) << 5) | (((r11 >> 1) & 1u) << 6) | (((r01 >> 1) & 1u) << 7);
 }
 
-// Slot of local edge e of cell (x, y, z) in the tile's lattice-edge table
-// (c_mc_eslot): every cell containing the same lattice edge gets the same slot.
-__device__ __forceinline__ int mc_edge_slot(int x, int y, int z, uint32_t es) {
-  return (int)(es & 0x7FFu) + x * (int)((es >> 11) & 0x7Fu) + y * (int)(es >> 18) + z;
+// Byte offset of local edge e of cell (x, y, z) in the tile's lattice-edge
+// table of 16-bit entries (es = c_mc_eslot[e]: twice the slot base | twice
+// the x stride << 12 | twice the y stride << 20, so no shift remains):
+// every cell containing the same lattice edge gets the same slot.
+__device__ __forceinline__ uint32_t mc_edge_off(int x, int y, int z, uint32_t es) {
+  return (es & 0xFFFu) + (uint32_t)x * ((es >> 12) & 0xFFu) + (uint32_t)y * (es >> 20) + 2u * (uint32_t)z;
+}
+__device__ __forceinline__ uint16_t& mc_edge_at(uint16_t* edge, uint32_t off) {
+  return *reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(edge) + off);
+}
+__device__ __forceinline__ uint32_t mc_edge_get(const uint16_t* edge, uint32_t off) {
+  return *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(edge) + off);
 }
 #define MC_EDGES 1944
 
@@ -1074,7 +1083,7 @@ __device__ __forceinline__ int mc_corner_known(uint32_t v, const MCEmitLDS& el, 
 // Vertex index (relative to the grid) on local edge e of cell (x, y, z).
 __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const MCEmitLDS& el,
                                                const uint32_t* s_eslot, int vbrel, const MCForeign& fc) {
-  const uint32_t v = el.edge[mc_edge_slot(x, y, z, s_eslot[e])];
+  const uint32_t v = mc_edge_get(el.edge, mc_edge_off(x, y, z, s_eslot[e]));
   if (v == 0xFFFFu)
     return mc_foreign_index(x, y, z, e, fc.processed, fc.s_ownsort, fc.s_cand, fc.ttile, fc.meta, fc.vbase, fc.vrel);
   const int f = el.foreign[(v & 0xFFu) < MC_FSIDE ? (v & 0xFFu) : 0];
@@ -1110,13 +1119,13 @@ __device__ __forceinline__ void mc_foreign_pass(uint32_t fm0, int c0, uint32_t f
       const int it = flist[k2], cc = it >> 4, fe = it & 15;
       const int fx = cc >> 6, fy = (cc >> 3) & 7, fz = cc & 7;
       const int slot = *fcarry + fb + k2;
-      const int es = mc_edge_slot(fx, fy, fz, s_eslot[fe]);
+      const uint32_t es = mc_edge_off(fx, fy, fz, s_eslot[fe]);
       if (slot < MC_FSIDE) {
         el.foreign[slot] =
             mc_foreign_index(fx, fy, fz, fe, fc.processed, fc.s_ownsort, fc.s_cand, fc.ttile, fc.meta, fc.vbase, fc.vrel);
-        el.edge[es] = (uint16_t)(0x8000u | (uint32_t)slot);
+        mc_edge_at(el.edge, es) = (uint16_t)(0x8000u | (uint32_t)slot);
       } else {
-        el.edge[es] = 0xFFFFu;
+        mc_edge_at(el.edge, es) = 0xFFFFu;
       }
     }
     wave_lds_sync();
@@ -1356,7 +1365,7 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
               const int vi = vb + vcarry + k;
               const int it = el.vlist[k], cv = it >> 4, ve = it & 15;
               const int ux = cv >> 6, uy = (cv >> 3) & 7, uz = cv & 7;
-              el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
+              mc_edge_at(el.edge, mc_edge_off(ux, uy, uz, s_eslot[ve])) = (uint16_t)(vcarry + k);
               if (vi < vmax)
                 mc_vertex_store(MCCorner{pva[j], pvb[j], (int)((pok >> (2 * j)) & 3u)}, bg, ux, uy, uz, ve, to, vs,
                                 iso, s_einfo, verts + 3 * (size_t)vi);
@@ -1367,7 +1376,7 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
             const int vi = vb + vcarry + k;
             const int it = el.vlist[k], cv = it >> 4, ve = it & 15;
             const int ux = cv >> 6, uy = (cv >> 3) & 7, uz = cv & 7;
-            el.edge[mc_edge_slot(ux, uy, uz, s_eslot[ve])] = (uint16_t)(vcarry + k);
+            mc_edge_at(el.edge, mc_edge_off(ux, uy, uz, s_eslot[ve])) = (uint16_t)(vcarry + k);
             if (vi < vmax)
               mc_write_vertex(leaves, bg, el.tleaf, ux, uy, uz, ve, to, vs, iso, s_einfo, verts + 3 * (size_t)vi);
           }
@@ -1396,8 +1405,9 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
           // edge past the side table (0xFFFF) is resolved again after them
           const int e0 = (int)(tri_e & 15u), e1 = (int)((tri_e >> 4) & 15u), e2 = (int)((tri_e >> 8) & 15u);
           const uint32_t s0 = s_eslot[e0], s1 = s_eslot[e1], s2 = s_eslot[e2];
-          const uint32_t v0 = el.edge[mc_edge_slot(ux, uy, uz, s0)], v1 = el.edge[mc_edge_slot(ux, uy, uz, s1)],
-                         v2 = el.edge[mc_edge_slot(ux, uy, uz, s2)];
+          const uint32_t v0 = mc_edge_get(el.edge, mc_edge_off(ux, uy, uz, s0)),
+                         v1 = mc_edge_get(el.edge, mc_edge_off(ux, uy, uz, s1)),
+                         v2 = mc_edge_get(el.edge, mc_edge_off(ux, uy, uz, s2));
           int i0 = mc_corner_known(v0, el, vbrel), i1 = mc_corner_known(v1, el, vbrel),
               i2 = mc_corner_known(v2, el, vbrel);
           if (v0 == 0xFFFFu || v1 == 0xFFFFu || v2 == 0xFFFFu) {
@@ -1426,7 +1436,7 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
             const int cx = cl[j] >> 6, cy = (cl[j] >> 3) & 7, cz = cl[j] & 7;
             for (uint32_t m = owned[j]; m; m &= m - 1u, ++vk) {
               const int ve = __ffs(m) - 1;
-              el.edge[mc_edge_slot(cx, cy, cz, s_eslot[ve])] = (uint16_t)vk;
+              mc_edge_at(el.edge, mc_edge_off(cx, cy, cz, s_eslot[ve])) = (uint16_t)vk;
               if (vb + vk < vmax)
                 mc_write_vertex(leaves, bg, el.tleaf, cx, cy, cz, ve, to, vs, iso, s_einfo,
                                 verts + 3 * (size_t)(vb + vk));
