@@ -365,8 +365,9 @@ __device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const Vel
   // Uniform velocity on a diagonal transform: component a of a departure
   // point depends on the voxel's coordinate on axis a alone, so the cell
   // index and fraction on each axis are computed once per leaf for the 8
-  // local coordinates, lane l (mod 8) holding those of coordinate l, and
-  // each voxel gathers its three with lane permutes.
+  // local coordinates, lane 8 a + l holding those of coordinate l on axis a
+  // (lanes from 24 on repeat axis 2), and each voxel gathers its three with
+  // lane permutes.
   // The tables also tell which sides of the brick the departure cells
   // reach: the halo on the - side of axis a is read only when some cell
   // starts at local -1, the + side only when one starts at 7 (a cell
@@ -377,25 +378,33 @@ __device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const Vel
   // When each axis reaches at most one side, only those sides are staged
   // (stage_halo_sided: about half the tasks).
   const bool sep = VK == VK_UNIFORM && in.diag;
-  int sep_i[3] = {0, 0, 0};
-  float sep_f[3] = {0.0f, 0.0f, 0.0f};
+  int sep_i = 0;
+  float sep_f = 0.0f;
   uint32_t slots = 0x7FFFFFFu;
   bool sided = false;
   int sd[3] = {0, 0, 0};
   if (sep) {
-    const int l = lane & 7;
-    const int o[3] = {ox, oy, oz};
+    const int l = lane & 7, ax = lane < 8 ? 0 : (lane < 16 ? 1 : 2);
+    // the lane's axis parameters, selected among values that an empty asm
+    // makes opaque: a select among loads of the views would become a load at
+    // a lane-dependent subscript, which puts the views in scratch memory
+    double p[9] = {in.mat[0], in.mat[5], in.mat[10], in.mat[3], in.mat[7], in.mat[11], vel.a[0], vel.a[1], vel.a[2]};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) asm volatile("" : "+v"(p[k]));
+    const double m = ax == 0 ? p[0] : (ax == 1 ? p[1] : p[2]);
+    const double c = ax == 0 ? p[3] : (ax == 1 ? p[4] : p[5]);
+    const double u = ax == 0 ? p[6] : (ax == 1 ? p[7] : p[8]);
+    const int o = ax == 0 ? ox : (ax == 1 ? oy : oz);
+    const double ix = departure_1d<SCHEME>(m * (double)(o + l) + c, u, dt, tf) * in.inv_vs;
+    const double fl = floor(ix);
+    sep_f = (float)(ix - fl);
+    sep_i = f64_to_i32(fl);
+    const int b = sep_i - o;
+    const uint64_t lo = __ballot(b == -1), hi = __ballot(b == 7);
     uint32_t side[3];  // bit d + 1: side d (-1, 0, +1) of the axis is reached
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double w = in.mat[5 * a] * (double)(o[a] + l) + in.mat[4 * a + 3];
-      const double ix = departure_1d<SCHEME>(w, vel.a[a], dt, tf) * in.inv_vs;
-      const double fl = floor(ix);
-      sep_f[a] = (float)(ix - fl);
-      sep_i[a] = f64_to_i32(fl);
-      const int b = sep_i[a] - o[a];
-      side[a] = (__ballot(b == -1) ? 1u : 0u) | 2u | (__ballot(b == 7) ? 4u : 0u);
-    }
+    for (int a = 0; a < 3; ++a)
+      side[a] = (((lo >> (8 * a)) & 0xFFu) ? 1u : 0u) | 2u | (((hi >> (8 * a)) & 0xFFu) ? 4u : 0u);
     const uint32_t yz = ((side[1] & 1u) ? side[2] : 0u) | (side[2] << 3) | ((side[1] & 4u) ? side[2] << 6 : 0u);
     slots = ((side[0] & 1u) ? yz : 0u) | (yz << 9) | ((side[0] & 4u) ? yz << 18 : 0u);
     sided = side[0] != 7u && side[1] != 7u && side[2] != 7u;
@@ -451,8 +460,8 @@ __device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const Vel
     const int vx = off >> 6, vy = (off >> 3) & 7, vz = off & 7;
     if (sep) {
       // the whole wave permutes (a lane past the list reads harmlessly)
-      const int ci = __shfl(sep_i[0], vx), cj = __shfl(sep_i[1], vy), ck = __shfl(sep_i[2], vz);
-      const float fx = __shfl(sep_f[0], vx), fy = __shfl(sep_f[1], vy), fz = __shfl(sep_f[2], vz);
+      const int ci = __shfl(sep_i, vx), cj = __shfl(sep_i, 8 + vy), ck = __shfl(sep_i, 16 + vz);
+      const float fx = __shfl(sep_f, vx), fy = __shfl(sep_f, 8 + vy), fz = __shfl(sep_f, 16 + vz);
       if (i < total) r[q] = sample_departure(in, s.brick, ox, oy, oz, ci, cj, ck, fx, fy, fz);
     } else if (i < total) {
       r[q] = advect_voxel<VK, SCHEME>(in, vel, s.brick, ox, oy, oz, vx, vy, vz, dt, tf, tab, s.vtab);
