@@ -1255,17 +1255,7 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
     // passes better; else rounds of 64.
     constexpr int CPL = LIST ? 2 : 1, RC = 64 * CPL;
     constexpr int RV = LIST ? MC_RVL : MC_RV, RT = LIST ? MC_RTL : MC_RT, NPC = LIST ? 3 : 1;
-    if constexpr (LIST) {
-      // The earlier-tile edges of the first round's cells (entries c0, c1)
-      // in one pass at the start of the tile, so that its loads of the
-      // earlier tiles' face meta wait once per tile; later rounds resolve
-      // their own.
-      wave_lds_sync();
-      uint32_t fm0 = 0u, fm1 = 0u;
-      if (2 * lane < n) fm0 = (uint32_t)(s_emn[c0 >> 9] & 0xFFFu) & el.fitem[s_ccls[c0 & 511u]];
-      if (2 * lane + 1 < n) fm1 = (uint32_t)(s_emn[c1 >> 9] & 0xFFFu) & el.fitem[s_ccls[c1 & 511u]];
-      mc_foreign_pass(fm0, (int)(c0 & 511u), fm1, (int)(c1 & 511u), lane, el.flist32, &fcarry, el, s_eslot, fc);
-    }
+    if constexpr (LIST) wave_lds_sync();  // the tile's lists in LDS before the first round reads them
     for (int base = 0; base < n; base += RC) {
       uint32_t ent[CPL];
       if constexpr (LIST) {
@@ -1335,11 +1325,12 @@ __global__ __launch_bounds__(MC_ETHREADS) __attribute__((amdgpu_waves_per_eu(6, 
             pok |= (uint32_t)c.ok << (2 * j);
           }
       }
-      // Edges owned in an earlier tile (LIST: those of the first round's
-      // cells were resolved at the start of the tile).
-      if (!LIST || base >= RC)
-        mc_foreign_pass(fm[0], cl[0], CPL > 1 ? fm[CPL - 1] : 0u, CPL > 1 ? cl[CPL - 1] : 0, lane,
-                        LIST ? el.flist32 : el.flist, &fcarry, el, s_eslot, fc);
+      // Edges owned in an earlier tile, resolved after the corner values are
+      // requested, so that the wait for the earlier tiles' face meta covers
+      // both (a round of 128 cells holds a typical tile's cells: one wait
+      // per tile).
+      mc_foreign_pass(fm[0], cl[0], CPL > 1 ? fm[CPL - 1] : 0u, CPL > 1 ? cl[CPL - 1] : 0, lane,
+                      LIST ? el.flist32 : el.flist, &fcarry, el, s_eslot, fc);
       if (listed) {
         // (the triangle list shares its LDS with the foreign list above)
         {
