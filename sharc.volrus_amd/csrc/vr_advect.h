@@ -326,9 +326,9 @@ __device__ __forceinline__ AdvectLoads<NW> advect_issue(const VR_GLOBAL NanoLeaf
 // Advects leaf li of `in` into out[li] with NW waves (the whole block,
 // tid in [0, 64 NW)) from its loads `ld` (advect_issue): the active voxels are listed by a ballot/popcount
 // prefix over the 8 mask words and spread densely over the block's lanes
-// (one pass for a band leaf with 4 waves); the results and, at the
-// inactive positions, the background are gathered in the brick's interior
-// and leave as the whole output leaf in 16-byte stores.
+// (one pass for a band leaf with 4 waves); the results are gathered in the
+// brick's interior and leave, with the background at the inactive
+// positions, as the whole output leaf in 16-byte stores.
 // Output origin and mask are the input's (the topology does not change,
 // src/tools/advection.rs:34-53).  SIGNS: also writes the leaf's sign mask for `iso`
 // to sg[0..15] (bit (o & 31) of word o >> 5 = value at offset o < iso,
@@ -444,10 +444,10 @@ __device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const Vel
     stage_halo_reg<NT>(in, nbv, s.brick, tid, slots);
   __syncthreads();
   // Results stay in registers until every voxel has sampled the brick
-  // (at most 512 / NT per thread), then they and the background go into
-  // the brick's interior, and the output leaf leaves with 16-byte stores:
-  // two per lane of a one-wave block instead of one store per list pass
-  // for the results and one per slab for the background.
+  // (at most 512 / NT per thread), then they go into the brick's interior,
+  // and the output leaf leaves with 16-byte stores: two per lane of a
+  // one-wave block instead of one store per list pass for the results and
+  // one per slab for the background.
   constexpr int RQ = 512 / NT;
   float r[RQ];
   const int wfirst = __builtin_amdgcn_readfirstlane(wave * 64);
@@ -477,17 +477,21 @@ __device__ __forceinline__ void advect_leaf_loaded(const GridView& in, const Vel
       s.brick[((off >> 6) + 1) * 100 + (((off >> 3) & 7) + 1) * 10 + (off & 7) + 1] = r[q];
     }
   }
-#pragma unroll
-  for (int x = 0; x < 8; ++x)
-    if ((x % NW) == wave && !((mk[x] >> lane) & 1ull)) s.brick[(x + 1) * 100 + ((lane >> 3) + 1) * 10 + (lane & 7) + 1] = bg;
   __syncthreads();
+  // The output leaf: the results where the mask is set, the background
+  // elsewhere (the brick still holds the input there).  A float4's four
+  // mask bits are byte y & 3 of header dword 4 + 2 x + y / 4, fetched by
+  // the whole wave with a lane permute.
 #pragma unroll
   for (int q = 0; q < OQ; ++q) {
     const int e = tid + q * NT;
+    const int x = (e >> 4) & 7, y = (e >> 1) & 7, z = (e & 1) * 4;
+    const uint32_t w = (uint32_t)__shfl((int)ld.hdr, 4 + 2 * x + (y >> 2));
     if (e < 128) {
-      const int x = e >> 4, y = (e >> 1) & 7, z = (e & 1) * 4;
+      const uint32_t act = w >> ((y & 3) * 8 + z);
       const float* b = s.brick + (x + 1) * 100 + (y + 1) * 10 + (z + 1);
-      reinterpret_cast<float4*>(O.values)[e] = make_float4(b[0], b[1], b[2], b[3]);
+      reinterpret_cast<float4*>(O.values)[e] = make_float4((act & 1u) ? b[0] : bg, (act & 2u) ? b[1] : bg,
+                                                           (act & 4u) ? b[2] : bg, (act & 8u) ? b[3] : bg);
     }
   }
   if (SIGNS && tid < 16) sg[tid] = s.sgn[tid];
