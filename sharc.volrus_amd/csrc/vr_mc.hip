@@ -1011,9 +1011,18 @@ struct MCEmitLDS {
   uint16_t self[28];             // per cell class: edges the cell owns itself
   uint16_t fitem[28];            // per cell class: edges owned in an earlier tile that this cell resolves
   uint16_t edge[MC_EDGES];       // lattice-edge table of the tile
-  int foreign[MC_FSIDE];         // vertex index (relative to the grid) of edges owned in an earlier tile
-  uint16_t vlist[MC_RVL];        // cell << 4 | edge
+  union {
+    struct {
+      int foreign[MC_FSIDE];     // vertex index (relative to the grid) of edges owned in an earlier tile
+      uint16_t vlist[MC_RVL];    // cell << 4 | edge
+    };
+    // the side table read at any 7-bit slot (an entry's low 7 bits), so a
+    // lookup needs no range check; slots past MC_FSIDE read the vertex list,
+    // and their values are never used
+    int fwin[128];
+  };
 };
+static_assert(MC_FSIDE <= 128 && MC_FSIDE * 4 + MC_RVL * 2 >= 128 * 4, "side-table window inside the LDS of a wave");
 
 // Tile t's lists and header in one vector load per array: lanes < 27 get
 // its neighbour leaves (*l), tiles (*m) and class masks (*k); lanes 27..34
@@ -1076,7 +1085,7 @@ struct MCForeign {
 // 0xFFFF: 0x8000 | side-table slot, or the vertex relative to the tile's
 // first (vbrel = that vertex relative to the grid's first).
 __device__ __forceinline__ int mc_corner_known(uint32_t v, const MCEmitLDS& el, int vbrel) {
-  const int f = el.foreign[(v & 0xFFu) < MC_FSIDE ? (v & 0xFFu) : 0];
+  const int f = el.fwin[v & 0x7Fu];
   return (v & 0x8000u) ? f : vbrel + (int)v;
 }
 
@@ -1086,7 +1095,7 @@ __device__ __forceinline__ int mc_corner_index(int x, int y, int z, int e, const
   const uint32_t v = mc_edge_get(el.edge, mc_edge_off(x, y, z, s_eslot[e]));
   if (v == 0xFFFFu)
     return mc_foreign_index(x, y, z, e, fc.processed, fc.s_ownsort, fc.s_cand, fc.ttile, fc.meta, fc.vbase, fc.vrel);
-  const int f = el.foreign[(v & 0xFFu) < MC_FSIDE ? (v & 0xFFu) : 0];
+  const int f = el.fwin[v & 0x7Fu];
   return (v & 0x8000u) ? f : vbrel + (int)v;
 }
 
